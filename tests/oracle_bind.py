@@ -1,0 +1,202 @@
+"""ctypes bindings for the oracle (oracle/liboracle.so) and, where built, the compiled reference
+kernels (oracle/_ref/libtgref.so).  Test infrastructure: imported by tests/, the smoke check and
+bench.py's cpu_baseline leg only.
+
+Array conventions (match include/tg_gravity.h):
+  positions : int32 [n, 3]  (fixed point, x = p * 2^-32 + 0.5)
+  forces    : float64 [n, 4] = (phi, gx, gy, gz)  -- the reference `force` struct
+  multipoles: structured [n] of (m: float32[17], x: int32[3])  -- the reference `multi_src`
+  expansion : float64 [35]
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_DIR = os.path.join(os.path.dirname(HERE), "oracle")
+
+MULTI_DTYPE = np.dtype([("m", np.float32, (17,)), ("x", np.int32, (3,))])
+assert MULTI_DTYPE.itemsize == 80
+
+
+class Params(C.Structure):
+    _fields_ = [("ewald", C.c_int), ("soft_len", C.c_double), ("m", C.c_double)]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+_lib = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            build_oracle()
+        L = C.CDLL(path)
+        vp, i64, ci = C.c_void_p, C.c_int64, C.c_int
+        for name in ("PP_direct",):
+            for pre in ("tgo_", "tgo64_"):
+                f = getattr(L, pre + name)
+                f.argtypes = [vp, vp, vp, i64, vp, i64, ci]
+        for name in ("PC_direct", "PC_ewald", "PP_ewald"):
+            for pre in ("tgo_", "tgo64_"):
+                f = getattr(L, pre + name)
+                f.argtypes = [vp, vp, vp, i64, vp, i64]
+        for name in ("CC_direct", "CC_ewald", "CP_direct", "CP_ewald"):
+            for pre in ("tgo_", "tgo64_"):
+                f = getattr(L, pre + name)
+                f.argtypes = [vp, vp, vp, vp, i64]
+        for name in ("tgo_PP_direct", "tgo_PC_direct", "tgo_PC_ewald", "tgo_PP_ewald", "tgo_CC_direct", "tgo_CC_ewald",
+                     "tgo_CP_direct", "tgo_CP_ewald"):
+            getattr(L, name).restype = C.c_uint64
+        for name in ("tgo64_PP_direct", "tgo64_PC_direct", "tgo64_PC_ewald", "tgo64_PP_ewald", "tgo64_CC_direct",
+                     "tgo64_CC_ewald", "tgo64_CP_direct", "tgo64_CP_ewald"):
+            getattr(L, name).restype = None
+        L.tgo_green_direct.argtypes = [vp, vp]
+        L.tgo_green_ewald.argtypes = [vp, vp]
+        L.tgo64_green_direct.argtypes = [vp, vp]
+        L.tgo64_green_ewald.argtypes = [vp, vp]
+        L.tgo_erfcexp.argtypes = [C.c_float, vp]
+        L.tgo_erfcexp.restype = C.c_float
+        L.tgo_exp.argtypes = [C.c_float]
+        L.tgo_exp.restype = C.c_float
+        L.tgo_sin.argtypes = [C.c_float]
+        L.tgo_sin.restype = C.c_float
+        L.tgo_direct64.argtypes = [ci, C.c_double, vp, i64, vp, vp, i64, vp, vp]
+        L.tgo_ewald_corr64.argtypes = [vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+KINDS = ("PP_direct", "PC_direct", "CC_direct", "CP_direct", "PP_ewald", "PC_ewald", "CC_ewald", "CP_ewald")
+
+
+def call_kernel(prefix, kind, params, x, y, f=None, L=None, do_phi=1, libobj=None):
+    """Run one kernel of the oracle family `prefix` ("tgo_" or "tgo64_").
+    PP/PC kinds: x int32[ns,3]; returns forces float64[ns,4] (accumulated onto f if given).
+    CC/CP kinds: x int32[3];    returns expansion float64[35] (accumulated onto L if given)."""
+    Lb = libobj or lib()
+    P = Params(int(params["ewald"]), float(params["soft_len"]), float(params["m"]))
+    fn = getattr(Lb, prefix + kind)
+    if kind.startswith("PP") or kind.startswith("PC"):
+        x = np.ascontiguousarray(x, np.int32).reshape(-1, 3)
+        ns = x.shape[0]
+        out = np.zeros((ns, 4), np.float64) if f is None else np.array(f, np.float64, copy=True).reshape(ns, 4)
+        if kind.startswith("PP"):
+            y = np.ascontiguousarray(y, np.int32).reshape(-1, 3)
+            args = [C.byref(P), _p(out), _p(x), ns, _p(y), y.shape[0]]
+            if kind == "PP_direct":
+                args.append(int(do_phi))
+        else:
+            y = np.ascontiguousarray(y, MULTI_DTYPE)
+            args = [C.byref(P), _p(out), _p(x), ns, _p(y), y.shape[0]]
+        rc = fn(*args)
+        return out, rc
+    x = np.ascontiguousarray(x, np.int32).reshape(3)
+    out = np.zeros(35, np.float64) if L is None else np.array(L, np.float64, copy=True).reshape(35)
+    if kind.startswith("CC"):
+        y = np.ascontiguousarray(y, MULTI_DTYPE)
+    else:
+        y = np.ascontiguousarray(y, np.int32).reshape(-1, 3)
+    rc = fn(C.byref(P), _p(out), _p(x), _p(y), y.shape[0])
+    return out, rc
+
+
+def green_direct(dx, fp64=False):
+    if fp64:
+        d = np.ascontiguousarray(dx, np.float64)
+        D = np.zeros(35, np.float64)
+        lib().tgo64_green_direct(_p(d), _p(D))
+    else:
+        d = np.ascontiguousarray(dx, np.float32)
+        D = np.zeros(35, np.float32)
+        lib().tgo_green_direct(_p(d), _p(D))
+    return D
+
+
+def green_ewald(dx, fp64=False):
+    if fp64:
+        d = np.ascontiguousarray(dx, np.float64)
+        D = np.zeros(35, np.float64)
+        lib().tgo64_green_ewald(_p(d), _p(D))
+    else:
+        d = np.ascontiguousarray(dx, np.float32)
+        D = np.zeros(35, np.float32)
+        lib().tgo_green_ewald(_p(d), _p(D))
+    return D
+
+
+def direct64(periodic, h, xs, ys, m):
+    xs = np.ascontiguousarray(xs, np.int32).reshape(-1, 3)
+    ys = np.ascontiguousarray(ys, np.int32).reshape(-1, 3)
+    m = np.ascontiguousarray(np.broadcast_to(m, (ys.shape[0],)), np.float64)
+    phi = np.zeros(xs.shape[0], np.float64)
+    g = np.zeros((xs.shape[0], 3), np.float64)
+    lib().tgo_direct64(int(periodic), float(h), _p(xs), xs.shape[0], _p(ys), _p(m), ys.shape[0], _p(phi), _p(g))
+    return phi, g
+
+
+def ewald_corr64(dx):
+    d = np.ascontiguousarray(dx, np.float64)
+    phi = np.zeros(1, np.float64)
+    g = np.zeros(3, np.float64)
+    lib().tgo_ewald_corr64(_p(d), _p(phi), _p(g))
+    return phi[0], g
+
+
+# ---- compiled reference (this container only) ----
+REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libtgref.so")
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib(path=REF_LIB):
+    L = C.CDLL(path)
+    vp, i64, ci = C.c_void_p, C.c_int64, C.c_int
+    L.tgref_set_options.argtypes = [ci, C.c_double, C.c_double, C.c_uint64, C.c_double]
+    L.tgref_PP_direct.argtypes = [vp, vp, i64, vp, i64, ci]
+    for n in ("PC_direct", "PC_ewald", "PP_ewald"):
+        getattr(L, "tgref_" + n).argtypes = [vp, vp, i64, vp, i64]
+    for n in ("CC_direct", "CC_ewald", "CP_direct", "CP_ewald"):
+        getattr(L, "tgref_" + n).argtypes = [vp, vp, vp, i64]
+    for n in KINDS:
+        getattr(L, "tgref_" + n).restype = C.c_uint64
+    return L
+
+
+def call_ref(L, kind, x, y, f=None, Lx=None, do_phi=1):
+    fn = getattr(L, "tgref_" + kind)
+    if kind.startswith("PP") or kind.startswith("PC"):
+        x = np.ascontiguousarray(x, np.int32).reshape(-1, 3)
+        ns = x.shape[0]
+        out = np.zeros((ns, 4), np.float64) if f is None else np.array(f, np.float64, copy=True).reshape(ns, 4)
+        if kind.startswith("PP"):
+            y = np.ascontiguousarray(y, np.int32).reshape(-1, 3)
+            args = [_p(out), _p(x), ns, _p(y), y.shape[0]]
+            if kind == "PP_direct":
+                args.append(int(do_phi))
+        else:
+            y = np.ascontiguousarray(y, MULTI_DTYPE)
+            args = [_p(out), _p(x), ns, _p(y), y.shape[0]]
+        rc = fn(*args)
+        return out, rc
+    x = np.ascontiguousarray(x, np.int32).reshape(3)
+    out = np.zeros(35, np.float64) if Lx is None else np.array(Lx, np.float64, copy=True).reshape(35)
+    if kind.startswith("CC"):
+        y = np.ascontiguousarray(y, MULTI_DTYPE)
+    else:
+        y = np.ascontiguousarray(y, np.int32).reshape(-1, 3)
+    rc = fn(_p(out), _p(x), _p(y), y.shape[0])
+    return out, rc
