@@ -1,0 +1,434 @@
+// tg_kernels.hip -- gfx950 kernels for tigergrav's eight interaction kinds.
+//
+// Mapping to CDNA4:
+//  * leaf kinds (PP, PC): one 64-lane wave per item; lane = (sink, phase).  A leaf of n <= 64
+//    sinks is padded to the next power of two nsp and the wave runs 64/nsp phases, each phase
+//    taking every (64/nsp)-th source of the tile, so small leaves still fill the wave.  Sources
+//    are staged 64 at a time through LDS and read back as broadcasts.  Per-tile partial sums are
+//    float, the running sum FP64 (the reference accumulates float terms in FP64 lanes,
+//    src/gravity.cpp:259-261), and phases are merged with cross-lane FP64 shuffles.
+//  * node kinds (CC, CP): one wave per item, lane = source; 35 float accumulators per lane,
+//    merged across lanes in FP64 through LDS.
+//  * Ewald kinds use tg::green_ewald / green_ewald1 (tg_math.hpp).
+#include "tg_kernels.hpp"
+#include "tg_math.hpp"
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+namespace tg {
+
+__constant__ float c_real[TG_NREAL * 3];
+__constant__ float c_four[TG_NFOUR * 38];
+__constant__ double c_ser[5 * TG_NSER];
+
+// ---- per-device lattice tables -------------------------------------------------------------
+static void build_tables(std::vector<float>& real, std::vector<float>& four, std::vector<double>& ser) {
+	real.clear();
+	for (int i = -4; i <= 4; i++)
+		for (int j = -4; j <= 4; j++)
+			for (int k = -4; k <= 4; k++) {
+				const int n2 = i * i + j * j + k * k;
+				if (n2 == 0 || n2 > 11) continue;
+				real.push_back(float(i));
+				real.push_back(float(j));
+				real.push_back(float(k));
+			}
+	four.clear();
+	for (int i = -3; i <= 3; i++)
+		for (int j = -3; j <= 3; j++)
+			for (int k = -3; k <= 3; k++) {
+				const int h2 = i * i + j * j + k * k;
+				if (h2 == 0 || h2 > 9) continue;
+				// keep one of each +-h pair: first non-zero component positive
+				const int first = i != 0 ? i : (j != 0 ? j : k);
+				if (first < 0) continue;
+				const double h[3] = {double(i), double(j), double(k)};
+				const double c0 = 2.0 * std::exp(-M_PI * M_PI * h2 / 4.0) / h2;  // x2: the -h partner
+				float row[38];
+				row[0] = float(i);
+				row[1] = float(j);
+				row[2] = float(k);
+				row[3] = float(-c0 / M_PI);
+				for (int a = 0; a < 3; a++) row[4 + a] = float(2.0 * h[a] * c0);
+				for (int t = 0; t < 6; t++) row[7 + t] = float(4.0 * M_PI * h[T2[t].i[0]] * h[T2[t].i[1]] * c0);
+				for (int t = 0; t < 10; t++)
+					row[13 + t] = float(-8.0 * M_PI * M_PI * h[T3[t].i[0]] * h[T3[t].i[1]] * h[T3[t].i[2]] * c0);
+				for (int t = 0; t < 15; t++)
+					row[23 + t] = float(-16.0 * M_PI * M_PI * M_PI * h[T4[t].i[0]] * h[T4[t].i[1]] * h[T4[t].i[2]] *
+							h[T4[t].i[3]] * c0);
+				four.insert(four.end(), row, row + 38);
+			}
+	// erf(2r)/r = sum_q c_q u^q, u = r^2, c_q = (2/sqrt(pi)) (-1)^q 2^(2q+1) / (q! (2q+1));
+	// d_k = (r^-1 d/dr)^k of it = sum_q c_q (2q)(2q-2)...(2q-2k+2) u^(q-k)
+	ser.assign(5 * TG_NSER, 0.0);
+	for (int k = 0; k < 5; k++)
+		for (int j = 0; j < TG_NSER; j++) {
+			const int q = j + k;
+			double c = 2.0 / std::sqrt(M_PI) * ((q & 1) ? -1.0 : 1.0) * std::ldexp(1.0, 2 * q + 1) / std::tgamma(q + 1.0) /
+			           (2 * q + 1);
+			for (int t = 0; t < k; t++) c *= double(2 * q - 2 * t);
+			ser[k * TG_NSER + j] = c;
+		}
+}
+
+hipError_t init_device_tables(int device) {
+	static std::mutex mtx;
+	static std::vector<int> done;
+	std::lock_guard<std::mutex> lock(mtx);
+	if ((int) done.size() <= device) done.resize(device + 1, 0);
+	if (done[device]) return hipSuccess;
+	std::vector<float> real, four;
+	std::vector<double> ser;
+	build_tables(real, four, ser);
+	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
+	hipError_t e;
+	if ((e = hipSetDevice(device)) != hipSuccess) return e;
+	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;
+	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_four), four.data(), four.size() * sizeof(float))) != hipSuccess) return e;
+	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ser), ser.data(), ser.size() * sizeof(double))) != hipSuccess) return e;
+	done[device] = 1;
+	return hipSuccess;
+}
+
+// ---- separations -----------------------------------------------------------------------------
+// position difference a - b in position units (2^32 per box), as float:
+//   WRAP: the int32 wrap-around difference (periodic minimum image, src/gravity.cpp:221-225);
+//   else: the exact 33-bit difference, rounded once (src/gravity.cpp:227-229).
+template <bool WRAP>
+__device__ __forceinline__ float sepu(int a, int b) {
+	const unsigned d = unsigned(a) - unsigned(b);
+	if (WRAP) return float(int(d));
+	const bool neg = a < b;
+	const float f = float(neg ? 0u - d : d);
+	return neg ? -f : f;
+}
+
+__device__ __forceinline__ int ceil_log2_64(int n) { return n <= 1 ? 0 : 32 - __clz(n - 1); }
+
+__device__ __forceinline__ double wave_xor(double v, int off) { return __shfl_xor(v, off, 64); }
+
+// ---- PP direct -------------------------------------------------------------------------------
+// Spline-softened (GADGET-2 cubic spline, support h) potential and acceleration
+// (src/gravity.cpp:231-287).  Beyond h it is Newton's 1/r; the softened branch is taken only by
+// the lanes that need it.
+template <bool WRAP, bool PHI>
+__global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int lane = threadIdx.x;
+	const int ns = it.y;
+	const int lg = ceil_log2_64(ns);
+	const int nsp = 1 << lg;
+	const int P = 64 >> lg;
+	const int si = lane & (nsp - 1);
+	const int ph = lane >> lg;
+	const int4 me = A.sinks[it.x + min(si, ns - 1)];
+	__shared__ int4 tile[64];
+	float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
+	double dax = 0.0, day = 0.0, daz = 0.0, dpp = 0.0;
+	for (int base = it.z; base < it.w; base += 64) {
+		const int n = min(64, it.w - base);
+		__syncthreads();
+		if (lane < n) tile[lane] = A.srcs[A.list ? A.list[base + lane] : base + lane];
+		__syncthreads();
+		for (int k = ph; k < n; k += P) {
+			const int4 s = tile[k];
+			const float fx = sepu<WRAP>(me.x, s.x);
+			const float fy = sepu<WRAP>(me.y, s.y);
+			const float fz = sepu<WRAP>(me.z, s.z);
+			const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+			const float w = __int_as_float(s.w);
+			if (r2 > A.h2u) {
+				const float ri = __builtin_amdgcn_rsqf(r2);
+				const float wri = w * ri;
+				const float wri3 = wri * ri * ri;
+				ax = fmaf(fx, wri3, ax);
+				ay = fmaf(fy, wri3, ay);
+				az = fmaf(fz, wri3, az);
+				if (PHI) pp += wri;
+			} else {
+				const float ru = sqrtf(r2);
+				const float q = ru * A.hinvu;
+				float f, p;
+				if (q < 0.5f) {
+					const float q2 = q * q;
+					f = fmaf(q2, fmaf(32.0f, q, -192.0f / 5.0f), 32.0f / 3.0f) * A.hinv3u;
+					p = fmaf(q2, fmaf(q2, fmaf(-32.0f / 5.0f, q, 48.0f / 5.0f), -16.0f / 3.0f), 14.0f / 5.0f) * A.hinvu;
+				} else {
+					const float ri = 1.0f / ru;
+					const float q3 = q * q * q;
+					f = fmaf(fmaf(fmaf(fmaf(-32.0f / 3.0f, q, 192.0f / 5.0f), q, -48.0f), q, 64.0f / 3.0f), q3, -1.0f / 15.0f) *
+					    (ri * ri * ri);
+					p = fmaf(fmaf(fmaf(fmaf(fmaf(32.0f / 15.0f, q, -48.0f / 5.0f), q, 16.0f), q, -32.0f / 3.0f), q * q, 16.0f / 5.0f), q,
+					         -1.0f / 15.0f) * ri;
+				}
+				const float wf = w * f;
+				ax = fmaf(fx, wf, ax);
+				ay = fmaf(fy, wf, ay);
+				az = fmaf(fz, wf, az);
+				if (PHI) pp = fmaf(w, p, pp);
+			}
+		}
+		dax += ax;
+		day += ay;
+		daz += az;
+		dpp += pp;
+		ax = ay = az = pp = 0.f;
+	}
+	for (int off = nsp; off < 64; off <<= 1) {
+		dax += wave_xor(dax, off);
+		day += wave_xor(day, off);
+		daz += wave_xor(daz, off);
+		dpp += wave_xor(dpp, off);
+	}
+	if (ph == 0 && si < ns) {
+		// position units: g = -sum w dx r^-3 = -2^64 sum (w du ru^-3), phi = -2^32 sum (w / ru)
+		double* o = A.out + 4 * (size_t) (it.x + si);
+		if (PHI) o[0] += -4294967296.0 * dpp;
+		o[1] += -18446744073709551616.0 * dax;
+		o[2] += -18446744073709551616.0 * day;
+		o[3] += -18446744073709551616.0 * daz;
+	}
+}
+
+hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s) {
+	if (a.nitems <= 0) return hipSuccess;
+	if (a.wrap) {
+		if (do_phi)
+			hipLaunchKernelGGL((k_pp_direct<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
+		else
+			hipLaunchKernelGGL((k_pp_direct<true, false>), dim3(a.nitems), dim3(64), 0, s, a);
+	} else {
+		if (do_phi)
+			hipLaunchKernelGGL((k_pp_direct<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
+		else
+			hipLaunchKernelGGL((k_pp_direct<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+	}
+	return hipGetLastError();
+}
+
+// ---- PC (multipole -> particle) ----------------------------------------------------------------
+template <bool EWALD, bool WRAP>
+__global__ __launch_bounds__(64) void k_pc(LeafArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int lane = threadIdx.x;
+	const int ns = it.y;
+	const int lg = ceil_log2_64(ns);
+	const int nsp = 1 << lg;
+	const int P = 64 >> lg;
+	const int si = lane & (nsp - 1);
+	const int ph = lane >> lg;
+	const int4 me = A.sinks[it.x + min(si, ns - 1)];
+	__shared__ float tile[64][20];
+	float phi = 0.f, g[3] = {0.f, 0.f, 0.f};
+	double dphi = 0.0, dg[3] = {0.0, 0.0, 0.0};
+	for (int base = it.z; base < it.w; base += 64) {
+		const int n = min(64, it.w - base);
+		__syncthreads();
+		if (lane < n) {
+			const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[base + lane] : base + lane));
+#pragma unroll
+			for (int q = 0; q < 5; q++) {
+				const float4 v = src[q];
+				tile[lane][4 * q] = v.x;
+				tile[lane][4 * q + 1] = v.y;
+				tile[lane][4 * q + 2] = v.z;
+				tile[lane][4 * q + 3] = v.w;
+			}
+		}
+		__syncthreads();
+		for (int k = ph; k < n; k += P) {
+			float M[17];
+#pragma unroll
+			for (int q = 0; q < 17; q++) M[q] = tile[k][q];
+			const float x[3] = {sepu<EWALD || WRAP>(me.x, __float_as_int(tile[k][17])) * POS_INVF,
+					sepu<EWALD || WRAP>(me.y, __float_as_int(tile[k][18])) * POS_INVF,
+					sepu<EWALD || WRAP>(me.z, __float_as_int(tile[k][19])) * POS_INVF};
+			float D[35];
+			if (EWALD)
+				green_ewald(D, x);
+			else
+				green_direct(D, x);
+			m2p(phi, g, M, D);
+		}
+		dphi += phi;
+		dg[0] += g[0];
+		dg[1] += g[1];
+		dg[2] += g[2];
+		phi = g[0] = g[1] = g[2] = 0.f;
+	}
+	for (int off = nsp; off < 64; off <<= 1) {
+		dphi += wave_xor(dphi, off);
+		dg[0] += wave_xor(dg[0], off);
+		dg[1] += wave_xor(dg[1], off);
+		dg[2] += wave_xor(dg[2], off);
+	}
+	if (ph == 0 && si < ns) {
+		double* o = A.out + 4 * (size_t) (it.x + si);
+		o[0] += dphi;
+		o[1] += dg[0];
+		o[2] += dg[1];
+		o[3] += dg[2];
+	}
+}
+
+hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s) {
+	if (a.nitems <= 0) return hipSuccess;
+	if (ewald)
+		hipLaunchKernelGGL((k_pc<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
+	else if (a.wrap)
+		hipLaunchKernelGGL((k_pc<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
+	else
+		hipLaunchKernelGGL((k_pc<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+	return hipGetLastError();
+}
+
+// ---- PP Ewald correction -------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int lane = threadIdx.x;
+	const int ns = it.y;
+	const int lg = ceil_log2_64(ns);
+	const int nsp = 1 << lg;
+	const int P = 64 >> lg;
+	const int si = lane & (nsp - 1);
+	const int ph = lane >> lg;
+	const int4 me = A.sinks[it.x + min(si, ns - 1)];
+	__shared__ int4 tile[64];
+	double dphi = 0.0, dg[3] = {0.0, 0.0, 0.0};
+	for (int base = it.z; base < it.w; base += 64) {
+		const int n = min(64, it.w - base);
+		__syncthreads();
+		if (lane < n) tile[lane] = A.srcs[A.list ? A.list[base + lane] : base + lane];
+		__syncthreads();
+		float phi = 0.f, g[3] = {0.f, 0.f, 0.f};
+		for (int k = ph; k < n; k += P) {
+			const int4 s = tile[k];
+			const float x[3] = {sepu<true>(me.x, s.x) * POS_INVF, sepu<true>(me.y, s.y) * POS_INVF,
+					sepu<true>(me.z, s.z) * POS_INVF};
+			float D0, D1[3];
+			green_ewald1(D0, D1, x);
+			const float w = __int_as_float(s.w);
+			phi = fmaf(w, D0, phi);
+			g[0] = fmaf(-w, D1[0], g[0]);
+			g[1] = fmaf(-w, D1[1], g[1]);
+			g[2] = fmaf(-w, D1[2], g[2]);
+		}
+		dphi += phi;
+		dg[0] += g[0];
+		dg[1] += g[1];
+		dg[2] += g[2];
+	}
+	for (int off = nsp; off < 64; off <<= 1) {
+		dphi += wave_xor(dphi, off);
+		dg[0] += wave_xor(dg[0], off);
+		dg[1] += wave_xor(dg[1], off);
+		dg[2] += wave_xor(dg[2], off);
+	}
+	if (ph == 0 && si < ns) {
+		double* o = A.out + 4 * (size_t) (it.x + si);
+		o[0] += dphi;
+		o[1] += dg[0];
+		o[2] += dg[1];
+		o[3] += dg[2];
+	}
+}
+
+hipError_t launch_pp_ewald(const LeafArgs& a, hipStream_t s) {
+	if (a.nitems <= 0) return hipSuccess;
+	hipLaunchKernelGGL(k_pp_ewald, dim3(a.nitems), dim3(64), 0, s, a);
+	return hipGetLastError();
+}
+
+// ---- node kinds: cross-lane FP64 reduction of 35 float accumulators --------------------------
+__device__ __forceinline__ void node_reduce_store(float* L, double* dst) {
+	__shared__ float red[35][65];
+	const int lane = threadIdx.x;
+#pragma unroll
+	for (int c = 0; c < 35; c++) red[c][lane] = L[c];
+	__syncthreads();
+	if (lane < 35) {
+		double s = 0.0;
+		for (int j = 0; j < 64; j++) s += double(red[lane][j]);
+		dst[lane] = s;
+	}
+}
+
+template <bool EWALD, bool WRAP>
+__global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int lane = threadIdx.x;
+	const int4 X = A.sinkpos[it.y];
+	float L[35];
+#pragma unroll
+	for (int c = 0; c < 35; c++) L[c] = 0.f;
+	for (int j = it.z + lane; j < it.w; j += 64) {
+		const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[j] : j));
+		float v[20];
+#pragma unroll
+		for (int q = 0; q < 5; q++) {
+			const float4 t = src[q];
+			v[4 * q] = t.x;
+			v[4 * q + 1] = t.y;
+			v[4 * q + 2] = t.z;
+			v[4 * q + 3] = t.w;
+		}
+		const float x[3] = {sepu<EWALD || WRAP>(X.x, __float_as_int(v[17])) * POS_INVF,
+				sepu<EWALD || WRAP>(X.y, __float_as_int(v[18])) * POS_INVF,
+				sepu<EWALD || WRAP>(X.z, __float_as_int(v[19])) * POS_INVF};
+		float D[35];
+		if (EWALD)
+			green_ewald(D, x);
+		else
+			green_direct(D, x);
+		m2l(L, v, D);
+	}
+	node_reduce_store(L, A.partial + 35 * (size_t) it.x);
+}
+
+template <bool EWALD, bool WRAP>
+__global__ __launch_bounds__(64) void k_cp(NodeArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int lane = threadIdx.x;
+	const int4 X = A.sinkpos[it.y];
+	float L[35];
+#pragma unroll
+	for (int c = 0; c < 35; c++) L[c] = 0.f;
+	for (int j = it.z + lane; j < it.w; j += 64) {
+		const int4 s = A.srcs[A.list ? A.list[j] : j];
+		const float x[3] = {sepu<EWALD || WRAP>(X.x, s.x) * POS_INVF, sepu<EWALD || WRAP>(X.y, s.y) * POS_INVF,
+				sepu<EWALD || WRAP>(X.z, s.z) * POS_INVF};
+		float D[35];
+		if (EWALD)
+			green_ewald(D, x);
+		else
+			green_direct(D, x);
+		p2l(L, __int_as_float(s.w), D);
+	}
+	node_reduce_store(L, A.partial + 35 * (size_t) it.x);
+}
+
+hipError_t launch_cc(const NodeArgs& a, bool ewald, hipStream_t s) {
+	if (a.nitems <= 0) return hipSuccess;
+	if (ewald)
+		hipLaunchKernelGGL((k_cc<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
+	else if (a.wrap)
+		hipLaunchKernelGGL((k_cc<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
+	else
+		hipLaunchKernelGGL((k_cc<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_cp(const NodeArgs& a, bool ewald, hipStream_t s) {
+	if (a.nitems <= 0) return hipSuccess;
+	if (ewald)
+		hipLaunchKernelGGL((k_cp<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
+	else if (a.wrap)
+		hipLaunchKernelGGL((k_cp<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
+	else
+		hipLaunchKernelGGL((k_cp<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+	return hipGetLastError();
+}
+
+}  // namespace tg

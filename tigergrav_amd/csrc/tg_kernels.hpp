@@ -1,0 +1,57 @@
+// tg_kernels.hpp -- launch interface of the gfx950 interaction kernels (host side).
+//
+// Work is described by int4 items.  Leaf kinds (PP/PC: sinks are particles):
+//   item = (first sink, sink count <= 64, list begin, list end)
+// and results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
+// atomics are needed while kernels of one stream run in order).
+// Node kinds (CC/CP: the sink is an expansion centre):
+//   item = (partial slot, sink position index, list begin, list end)
+// and each item writes its 35 FP64 partial coefficients to partial[35 * slot].
+// `list` holds source indices (particles for PP/CP, multipoles for PC/CC); NULL means the
+// identity list.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+struct MultiSrc {
+	float m[17];
+	int32_t x[3];
+};
+
+struct LeafArgs {
+	const int4* sinks;       // sink particles (x, y, z, mass bits)
+	const int4* srcs;        // source particles (PP)
+	const MultiSrc* multis;  // source multipoles (PC)
+	const int4* items;
+	const int* list;
+	double* out;             // 4 doubles per sink
+	int nitems;
+	int wrap;                // periodic minimum-image separations (direct kinds)
+	float h2u, hinvu, hinv3u;  // softening in position units (2^32 per box length)
+};
+
+struct NodeArgs {
+	const int4* sinkpos;     // expansion centres (x, y, z, unused)
+	const int4* srcs;        // source particles (CP)
+	const MultiSrc* multis;  // source multipoles (CC)
+	const int4* items;
+	const int* list;
+	double* partial;         // 35 doubles per item
+	int nitems;
+	int wrap;
+};
+
+enum Kind { PP_DIRECT = 0, PC_DIRECT, CC_DIRECT, CP_DIRECT, PP_EWALD, PC_EWALD, CC_EWALD, CP_EWALD, NKINDS };
+
+// one-time per-device setup of the Ewald lattice tables; returns hipSuccess or the error
+hipError_t init_device_tables(int device);
+
+hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s);
+hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s);
+hipError_t launch_pp_ewald(const LeafArgs& a, hipStream_t s);
+hipError_t launch_cc(const NodeArgs& a, bool ewald, hipStream_t s);
+hipError_t launch_cp(const NodeArgs& a, bool ewald, hipStream_t s);
+
+}  // namespace tg

@@ -1,0 +1,422 @@
+// tg_math.hpp -- device math for the gfx950 gravity kernels.
+//
+// Symmetric-tensor storage follows the reference layouts so results can be handed back
+// unchanged: expansion slots [0], [1+a], [4+s2], [10+s3], [20+s4] (tigergrav/expansion.hpp:77-137),
+// multipole slots [0], [1+s2], [7+s3] (tigergrav/multipole.hpp:56-82), where s2/s3/s4 rank the
+// sorted index tuple lexicographically.  Every loop over tensor slots is fully unrolled with
+// compile-time indices, so tensors live in VGPRs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+// ---- compile-time slot tables ----------------------------------------------------------------
+struct Tup {
+	int i[4];
+};
+constexpr Tup T2[6] = {{{0, 0}}, {{0, 1}}, {{0, 2}}, {{1, 1}}, {{1, 2}}, {{2, 2}}};
+constexpr Tup T3[10] = {{{0, 0, 0}}, {{0, 0, 1}}, {{0, 0, 2}}, {{0, 1, 1}}, {{0, 1, 2}},
+		{{0, 2, 2}}, {{1, 1, 1}}, {{1, 1, 2}}, {{1, 2, 2}}, {{2, 2, 2}}};
+constexpr Tup T4[15] = {{{0, 0, 0, 0}}, {{0, 0, 0, 1}}, {{0, 0, 0, 2}}, {{0, 0, 1, 1}}, {{0, 0, 1, 2}},
+		{{0, 0, 2, 2}}, {{0, 1, 1, 1}}, {{0, 1, 1, 2}}, {{0, 1, 2, 2}}, {{0, 2, 2, 2}}, {{1, 1, 1, 1}},
+		{{1, 1, 1, 2}}, {{1, 1, 2, 2}}, {{1, 2, 2, 2}}, {{2, 2, 2, 2}}};
+
+__host__ __device__ constexpr int s2(int a, int b) {
+	return a > b ? s2(b, a) : (a == 0 ? b : (a == 1 ? 2 + b : 5));
+}
+__host__ __device__ constexpr int s3sorted(int a, int b, int c) {
+	return a == 0 ? s2(b, c) : (a == 1 ? (b == 1 ? (c == 1 ? 6 : 7) : 8) : 9);
+}
+__host__ __device__ constexpr int s3(int a, int b, int c) {
+	// sort three values
+	return (a <= b) ? ((b <= c) ? s3sorted(a, b, c) : ((a <= c) ? s3sorted(a, c, b) : s3sorted(c, a, b)))
+	                : ((a <= c) ? s3sorted(b, a, c) : ((b <= c) ? s3sorted(b, c, a) : s3sorted(c, b, a)));
+}
+__host__ __device__ constexpr int s4sorted(int a, int b, int c, int d) {
+	return a == 0 ? s3sorted(b, c, d) : (a == 1 ? (b == 1 ? (c == 1 ? (d == 1 ? 10 : 11) : 12) : 13) : 14);
+}
+__host__ __device__ constexpr int min2(int a, int b) { return a < b ? a : b; }
+__host__ __device__ constexpr int max2(int a, int b) { return a < b ? b : a; }
+__host__ __device__ constexpr int s4(int a, int b, int c, int d) {
+	// sorting network for four values
+	return s4sorted(min2(min2(a, b), min2(c, d)),
+			min2(max2(min2(a, b), min2(c, d)), min2(max2(a, b), max2(c, d))),
+			max2(max2(min2(a, b), min2(c, d)), min2(max2(a, b), max2(c, d))),
+			max2(max2(a, b), max2(c, d)));
+}
+// expansion slots
+__host__ __device__ constexpr int E1(int a) { return 1 + a; }
+__host__ __device__ constexpr int E2(int a, int b) { return 4 + s2(a, b); }
+__host__ __device__ constexpr int E3(int a, int b, int c) { return 10 + s3(a, b, c); }
+__host__ __device__ constexpr int E4(int a, int b, int c, int d) { return 20 + s4(a, b, c, d); }
+// multipole slots
+__host__ __device__ constexpr int M2(int a, int b) { return 1 + s2(a, b); }
+__host__ __device__ constexpr int M3(int a, int b, int c) { return 7 + s3(a, b, c); }
+
+// number of distinct index values of a sorted tuple, and multiplicity (permutations) of it
+__host__ __device__ constexpr int ndistinct(const Tup& t, int n) {
+	return n == 2 ? 1 + (t.i[0] != t.i[1])
+	              : (n == 3 ? 1 + (t.i[0] != t.i[1]) + (t.i[1] != t.i[2]) : 0);
+}
+__host__ __device__ constexpr int mult2(const Tup& t) { return t.i[0] == t.i[1] ? 1 : 2; }
+__host__ __device__ constexpr int mult3(const Tup& t) {
+	return (t.i[0] == t.i[1] && t.i[1] == t.i[2]) ? 1 : ((t.i[0] == t.i[1] || t.i[1] == t.i[2]) ? 3 : 6);
+}
+// expansion_factors (tigergrav/expansion.hpp:163-182): multiplicity / k!
+__host__ __device__ constexpr float ef2(int s) { return mult2(T2[s]) * 0.5f; }
+__host__ __device__ constexpr float ef3(int s) { return mult3(T3[s]) * (1.0f / 6.0f); }
+
+constexpr double POS_INV = 1.0 / 4294967296.0;
+constexpr float POS_INVF = 1.0f / 4294967296.0f;
+
+// ---- derivative tensor from radial factors ----------------------------------------------------
+// For a radial function f(r), with d_k = (r^-1 d/dr)^k f, the Cartesian derivatives are
+//   D_a    = x_a d1
+//   D_ab   = x_a x_b d2 + delta_ab d1
+//   D_abc  = x_a x_b x_c d3 + (delta_ab x_c + delta_ac x_b + delta_bc x_a) d2
+//   D_abcd = x_a x_b x_c x_d d4 + (6 delta x x terms) d3 + (3 delta delta terms) d2
+// (the reference builds the same tensor by in-place slot updates, tigergrav/green.hpp:62-117).
+__device__ __forceinline__ void radial_tensor(float* D, float d0, float d1, float d2, float d3, float d4, const float* x) {
+	D[0] = d0;
+#pragma unroll
+	for (int a = 0; a < 3; a++) D[1 + a] = x[a] * d1;
+	float x2[6];
+#pragma unroll
+	for (int s = 0; s < 6; s++) {
+		const int a = T2[s].i[0], b = T2[s].i[1];
+		x2[s] = x[a] * x[b];
+		D[4 + s] = (a == b) ? fmaf(x2[s], d2, d1) : x2[s] * d2;
+	}
+	float x3[10];
+#pragma unroll
+	for (int s = 0; s < 10; s++) {
+		const int a = T3[s].i[0], b = T3[s].i[1], c = T3[s].i[2];
+		x3[s] = x2[s2(a, b)] * x[c];
+		float t = x3[s] * d3;
+		// delta terms: count equal pairs
+		if (a == b && b == c)
+			t = fmaf(3.0f * x[a], d2, t);
+		else if (a == b)
+			t = fmaf(x[c], d2, t);
+		else if (b == c)
+			t = fmaf(x[a], d2, t);
+		D[10 + s] = t;
+	}
+	const float x2d3[6] = {x2[0] * d3, x2[1] * d3, x2[2] * d3, x2[3] * d3, x2[4] * d3, x2[5] * d3};
+#pragma unroll
+	for (int s = 0; s < 15; s++) {
+		const int a = T4[s].i[0], b = T4[s].i[1], c = T4[s].i[2], d = T4[s].i[3];
+		float t = x3[s3(a, b, c)] * x[d] * d4;
+		// sum over the 6 index pairs (p,q) with equal values of x_(other two) d3
+		float dd = 0.0f;
+		int npairs = 0;
+		if (a == b) { dd += x2d3[s2(c, d)]; npairs++; }
+		if (a == c) { dd += x2d3[s2(b, d)]; npairs++; }
+		if (a == d) { dd += x2d3[s2(b, c)]; npairs++; }
+		if (b == c) { dd += x2d3[s2(a, d)]; npairs++; }
+		if (b == d) { dd += x2d3[s2(a, c)]; npairs++; }
+		if (c == d) { dd += x2d3[s2(a, b)]; npairs++; }
+		if (npairs) t += dd;
+		const int dd2 = (a == b && c == d) + (a == c && b == d) + (a == d && b == c);
+		if (dd2) t = fmaf(float(dd2), d2, t);
+		D[20 + s] = t;
+	}
+}
+
+// derivatives of -1/r (tigergrav/green.hpp:167-186): zero below r = 1e-9 like the reference
+__device__ __forceinline__ void green_direct(float* D, const float* x) {
+	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+	const float ri = (r2 > 1e-18f) ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+	const float ri2 = ri * ri;
+	const float d0 = -ri;
+	const float d1 = ri * ri2;
+	const float d2 = -3.0f * d1 * ri2;
+	const float d3 = -5.0f * d2 * ri2;
+	const float d4 = -7.0f * d3 * ri2;
+	radial_tensor(D, d0, d1, d2, d3, d4, x);
+}
+
+// ---- contractions (src/gravity.cpp:26-169) ----------------------------------------------------
+// M2L: L += contraction of the multipole M with D (reference gravity.cpp:26-90)
+__device__ __forceinline__ void m2l(float* L, const float* M, const float* D) {
+	float l0 = M[0] * D[0];
+#pragma unroll
+	for (int s = 0; s < 6; s++) l0 = fmaf(M[1 + s] * ef2(s), D[4 + s], l0);
+#pragma unroll
+	for (int s = 0; s < 10; s++) l0 = fmaf(-M[7 + s] * ef3(s), D[10 + s], l0);
+	L[0] += l0;
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		float la = M[0] * D[1 + a];
+#pragma unroll
+		for (int s = 0; s < 6; s++) la = fmaf(M[1 + s] * ef2(s), D[E3(a, T2[s].i[0], T2[s].i[1])], la);
+#pragma unroll
+		for (int s = 0; s < 10; s++) la = fmaf(-M[7 + s] * ef3(s), D[E4(a, T3[s].i[0], T3[s].i[1], T3[s].i[2])], la);
+		L[1 + a] += la;
+	}
+#pragma unroll
+	for (int t = 0; t < 6; t++) {
+		const int a = T2[t].i[0], b = T2[t].i[1];
+		float lab = M[0] * D[4 + t];
+#pragma unroll
+		for (int s = 0; s < 6; s++) lab = fmaf(M[1 + s] * ef2(s), D[E4(a, b, T2[s].i[0], T2[s].i[1])], lab);
+		L[4 + t] += lab;
+	}
+#pragma unroll
+	for (int s = 10; s < 35; s++) L[s] = fmaf(M[0], D[s], L[s]);
+}
+
+// P2L: L += m D (reference gravity.cpp:92-121)
+__device__ __forceinline__ void p2l(float* L, float m, const float* D) {
+#pragma unroll
+	for (int s = 0; s < 35; s++) L[s] = fmaf(m, D[s], L[s]);
+}
+
+// M2P: potential and acceleration at the sink (reference gravity.cpp:123-169).  The potential's
+// octupole term reproduces the reference's loop (a over all values, b <= c): each sorted triple
+// is weighted by its number of distinct index values.
+__host__ __device__ constexpr float wq3(int s) { return ndistinct(T3[s], 3) * ef3(s); }
+
+__device__ __forceinline__ void m2p(float& phi, float* g, const float* M, const float* D) {
+	float p = M[0] * D[0];
+#pragma unroll
+	for (int s = 0; s < 6; s++) p = fmaf(M[1 + s] * ef2(s), D[4 + s], p);
+#pragma unroll
+	for (int s = 0; s < 10; s++) p = fmaf(-M[7 + s] * wq3(s), D[10 + s], p);
+	phi += p;
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		float ga = -M[0] * D[1 + a];
+#pragma unroll
+		for (int s = 0; s < 6; s++) ga = fmaf(-M[1 + s] * ef2(s), D[E3(a, T2[s].i[0], T2[s].i[1])], ga);
+#pragma unroll
+		for (int s = 0; s < 10; s++) ga = fmaf(M[7 + s] * ef3(s), D[E4(a, T3[s].i[0], T3[s].i[1], T3[s].i[2])], ga);
+		g[a] += ga;
+	}
+}
+
+// ---- Ewald periodic correction ----------------------------------------------------------------
+// Lattice tables (filled once per device by the host, see tg_kernels.hip):
+//   real-space vectors n != 0 with |n|^2 <= 11 (170 of them); the reference sums |n|^2 <= 17 but the
+//   terms with |n|^2 >= 12 have |x - n| >= 2.6 for every wrapped x and contribute below 1e-10;
+//   Fourier vectors h != 0, |h|^2 <= 9, one of each +-h pair (61), with the 35 coefficient-tensor
+//   entries doubled to account for the partner (tigergrav/green.hpp:30-59).
+#define TG_NREAL 170
+#define TG_NFOUR 61
+#define TG_NSER 32
+extern __constant__ float c_real[TG_NREAL * 3];
+extern __constant__ float c_four[TG_NFOUR * 38];  // h_x, h_y, h_z, then 35 coefficients
+extern __constant__ double c_ser[5 * TG_NSER];    // series of erf(2r)/r radial factors in r^2
+
+// erfc for x >= 0 with relative error < 1.2e-7 everywhere (Numerical Recipes erfcc, Chebyshev
+// fit); e = exp(-x^2) is returned as a by-product.
+__device__ __forceinline__ float erfc_nr(float x, float& e) {
+	const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+	float p = 0.17087277f;
+	p = fmaf(p, t, -0.82215223f);
+	p = fmaf(p, t, 1.48851587f);
+	p = fmaf(p, t, -1.13520398f);
+	p = fmaf(p, t, 0.27886807f);
+	p = fmaf(p, t, -0.18628806f);
+	p = fmaf(p, t, 0.09678418f);
+	p = fmaf(p, t, 0.37409196f);
+	p = fmaf(p, t, 1.00002368f);
+	p = fmaf(p, t, -1.26551223f);
+	const float L2E = 1.4426950408889634f;
+	e = __builtin_amdgcn_exp2f(-x * x * L2E);
+	return t * e * __builtin_amdgcn_exp2f(p * L2E);
+}
+
+// radial factors d0..d4 of -erfc(2r)/r for a lattice image at distance r (r >= ~0.5)
+template <int ORDER>
+__device__ __forceinline__ void ewald_real_factors(float r2, float* d) {
+	const float ri = __builtin_amdgcn_rsqf(r2);
+	const float r = r2 * ri;
+	float e;
+	const float erfc = erfc_nr(2.0f * r, e);
+	const float expfactor = 2.2567583341910251f * r * e;  // 4/sqrt(pi) r exp(-4 r^2)
+	const float ri2 = ri * ri;
+	d[0] = -erfc * ri;
+	const float ri3 = ri2 * ri;
+	d[1] = (expfactor + erfc) * ri3;
+	if (ORDER >= 2) {
+		const float ri5 = ri3 * ri2, ri7 = ri5 * ri2, ri9 = ri7 * ri2;
+		const float r4 = r2 * r2;
+		d[2] = -fmaf(expfactor, fmaf(8.0f, r2, 3.0f), 3.0f * erfc) * ri5;
+		d[3] = fmaf(expfactor, 15.0f + fmaf(40.0f, r2, 64.0f * r4), 15.0f * erfc) * ri7;
+		d[4] = -fmaf(expfactor, fmaf(8.0f * r2, 35.0f + fmaf(56.0f, r2, 64.0f * r4), 105.0f), 105.0f * erfc) * ri9;
+	}
+}
+
+// radial factors of erf(2r)/r (= the n = 0 image minus the Newtonian -1/r it corrects), FP64
+// series in u = r^2, valid for r <= sqrt(3)/2 (every minimum-image separation)
+template <int ORDER>
+__device__ __forceinline__ void ewald_self_factors(double u, float* d) {
+#pragma unroll
+	for (int k = 0; k <= ORDER; k++) {
+		double acc = c_ser[k * TG_NSER + TG_NSER - 1];
+#pragma unroll
+		for (int q = TG_NSER - 2; q >= 0; q--) acc = fma(acc, u, c_ser[k * TG_NSER + q]);
+		d[k] = float(acc);
+	}
+}
+
+// accumulators of the real-space sum: A-terms are the x-monomial parts, B/C the delta parts
+struct EwaldAcc {
+	float A0, A1[3], B1, A2[6], B2[3], C2, A3[10], B3[6], A4[15];
+};
+
+__device__ __forceinline__ void acc_zero(EwaldAcc& s) {
+	s.A0 = s.B1 = s.C2 = 0.0f;
+#pragma unroll
+	for (int i = 0; i < 3; i++) s.A1[i] = s.B2[i] = 0.0f;
+#pragma unroll
+	for (int i = 0; i < 6; i++) s.A2[i] = s.B3[i] = 0.0f;
+#pragma unroll
+	for (int i = 0; i < 10; i++) s.A3[i] = 0.0f;
+#pragma unroll
+	for (int i = 0; i < 15; i++) s.A4[i] = 0.0f;
+}
+
+__device__ __forceinline__ void acc_add(EwaldAcc& s, const float* x, const float* d) {
+	s.A0 += d[0];
+	s.B1 += d[1];
+	s.C2 += d[2];
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		s.A1[a] = fmaf(x[a], d[1], s.A1[a]);
+		s.B2[a] = fmaf(x[a], d[2], s.B2[a]);
+	}
+	float x2[6];
+#pragma unroll
+	for (int t = 0; t < 6; t++) {
+		x2[t] = x[T2[t].i[0]] * x[T2[t].i[1]];
+		s.A2[t] = fmaf(x2[t], d[2], s.A2[t]);
+		s.B3[t] = fmaf(x2[t], d[3], s.B3[t]);
+	}
+#pragma unroll
+	for (int t = 0; t < 10; t++) {
+		const float x3 = x2[s2(T3[t].i[0], T3[t].i[1])] * x[T3[t].i[2]];
+		s.A3[t] = fmaf(x3, d[3], s.A3[t]);
+		const float x3d4 = x3 * d[4];
+		// the order-4 slots (abc,d) with d >= c
+#pragma unroll
+		for (int dd = 0; dd < 3; dd++)
+			if (dd >= T3[t].i[2]) {
+				const int q = s4sorted(T3[t].i[0], T3[t].i[1], T3[t].i[2], dd);
+				s.A4[q] = fmaf(x3d4, x[dd], s.A4[q]);
+			}
+	}
+}
+
+__device__ __forceinline__ void acc_assemble(const EwaldAcc& s, float* D) {
+	D[0] = s.A0;
+#pragma unroll
+	for (int a = 0; a < 3; a++) D[1 + a] = s.A1[a];
+#pragma unroll
+	for (int t = 0; t < 6; t++) D[4 + t] = s.A2[t] + (T2[t].i[0] == T2[t].i[1] ? s.B1 : 0.0f);
+#pragma unroll
+	for (int t = 0; t < 10; t++) {
+		const int a = T3[t].i[0], b = T3[t].i[1], c = T3[t].i[2];
+		float v = s.A3[t];
+		if (a == b) v += s.B2[c];
+		if (a == c) v += s.B2[b];
+		if (b == c) v += s.B2[a];
+		D[10 + t] = v;
+	}
+#pragma unroll
+	for (int t = 0; t < 15; t++) {
+		const int a = T4[t].i[0], b = T4[t].i[1], c = T4[t].i[2], d = T4[t].i[3];
+		float v = s.A4[t];
+		if (a == b) v += s.B3[s2(c, d)];
+		if (a == c) v += s.B3[s2(b, d)];
+		if (a == d) v += s.B3[s2(b, c)];
+		if (b == c) v += s.B3[s2(a, d)];
+		if (b == d) v += s.B3[s2(a, c)];
+		if (c == d) v += s.B3[s2(a, b)];
+		const int dd2 = (a == b && c == d) + (a == c && b == d) + (a == d && b == c);
+		if (dd2) v = fmaf(float(dd2), s.C2, v);
+		D[20 + t] = v;
+	}
+}
+
+// Full periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x.
+// The n = 0 image is combined with the subtracted Newtonian term analytically (erf(2r)/r), so
+// nothing cancels; at |x| <= 1e-6 the reference's convention D = (2.8372975, 0, ..., 0) applies.
+__device__ __forceinline__ void green_ewald(float* D, const float* x) {
+	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+	EwaldAcc s;
+	acc_zero(s);
+	{
+		float d[5];
+		ewald_self_factors<4>(double(r2), d);
+		acc_add(s, x, d);
+	}
+#pragma unroll 2
+	for (int i = 0; i < TG_NREAL; i++) {
+		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
+		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
+		float d[5];
+		ewald_real_factors<4>(q2, d);
+		acc_add(s, dx, d);
+	}
+	acc_assemble(s, D);
+	for (int i = 0; i < TG_NFOUR; i++) {
+		const float* H = c_four + 38 * i;
+		const float w = fmaf(H[2], x[2], fmaf(H[1], x[1], H[0] * x[0]));
+		const float sn = __builtin_amdgcn_sinf(w);  // sin(2 pi w): v_sin_f32 takes revolutions
+		const float cs = __builtin_amdgcn_cosf(w);
+		D[0] = fmaf(H[3], cs, D[0]);
+#pragma unroll
+		for (int a = 0; a < 3; a++) D[1 + a] = fmaf(H[4 + a], sn, D[1 + a]);
+#pragma unroll
+		for (int t = 0; t < 6; t++) D[4 + t] = fmaf(H[7 + t], cs, D[4 + t]);
+#pragma unroll
+		for (int t = 0; t < 10; t++) D[10 + t] = fmaf(H[13 + t], sn, D[10 + t]);
+#pragma unroll
+		for (int t = 0; t < 15; t++) D[20 + t] = fmaf(H[23 + t], cs, D[20 + t]);
+	}
+	D[0] += 0.78539816339744831f;  // pi/4
+	if (!(r2 > 1e-12f)) {
+		D[0] = 2.8372975f;
+#pragma unroll
+		for (int t = 1; t < 35; t++) D[t] = 0.0f;
+	}
+}
+
+// potential and first derivatives of the periodic correction only (PP_ewald)
+__device__ __forceinline__ void green_ewald1(float& D0, float* D1, const float* x) {
+	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+	float d[2];
+	ewald_self_factors<1>(double(r2), d);
+	float a0 = d[0];
+	float a1[3] = {x[0] * d[1], x[1] * d[1], x[2] * d[1]};
+#pragma unroll 4
+	for (int i = 0; i < TG_NREAL; i++) {
+		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
+		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
+		float e[2];
+		ewald_real_factors<1>(q2, e);
+		a0 += e[0];
+#pragma unroll
+		for (int a = 0; a < 3; a++) a1[a] = fmaf(dx[a], e[1], a1[a]);
+	}
+#pragma unroll 4
+	for (int i = 0; i < TG_NFOUR; i++) {
+		const float* H = c_four + 38 * i;
+		const float w = fmaf(H[2], x[2], fmaf(H[1], x[1], H[0] * x[0]));
+		const float sn = __builtin_amdgcn_sinf(w);
+		const float cs = __builtin_amdgcn_cosf(w);
+		a0 = fmaf(H[3], cs, a0);
+#pragma unroll
+		for (int a = 0; a < 3; a++) a1[a] = fmaf(H[4 + a], sn, a1[a]);
+	}
+	a0 += 0.78539816339744831f;
+	const bool z = !(r2 > 1e-12f);
+	D0 = z ? 2.8372975f : a0;
+#pragma unroll
+	for (int a = 0; a < 3; a++) D1[a] = z ? 0.0f : a1[a];
+}
+
+}  // namespace tg
