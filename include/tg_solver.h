@@ -1,0 +1,62 @@
+/*
+ * tg_solver.h -- C ABI of the batched, device-resident force solve.
+ *
+ * The reference evaluates one force solve as a recursive walk that calls the eight kernels once
+ * per tree node / leaf (tree::kick_fmm, src/tree.cpp:378-635, driven by solve_gravity,
+ * src/main.cpp:31-73).  This interface runs the same solve in one piece: the host restatement of
+ * the tree builds the kd-tree, multipoles and interaction lists (src/tree.cpp:122-296,378-635),
+ * uploads them, and every interaction of the solve runs on the device.  Types and status codes
+ * are those of tg_gravity.h.
+ */
+#ifndef TG_SOLVER_H
+#define TG_SOLVER_H
+#include <stdint.h>
+
+#include "tg_gravity.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tg_solver tg_solver;
+
+typedef struct tg_solve_info {
+	int64_t n, n_nodes, n_leaves, max_level;
+	/* interaction census as the reference counts it (src/tree.cpp:483-494,618-628), kinds in the
+	 * order PP_direct, PC_direct, CC_direct, CP_direct, PP_ewald, PC_ewald, CC_ewald, CP_ewald */
+	uint64_t count[8];
+	int64_t items[8];      /* device work items per kind */
+	int64_t list_len[8];   /* source indices per kind on the device */
+	int64_t list_bytes;    /* device bytes of lists and items */
+	double t_tree, t_multipoles, t_walk, t_upload;  /* seconds, last tg_solver_set_particles */
+} tg_solve_info;
+
+/* replaces the tree root construction + options set-up (src/main.cpp:133-174): kernel
+ * parameters, opening angle theta and leaf size parts_per_node for subsequent solves.
+ * p->device = -1 gives a host-only solver: tree, multipoles, lists and export, no solve. */
+int tg_solver_create(const tg_params* p, double theta, int parts_per_node, tg_solver** out);
+int tg_solver_destroy(tg_solver* s);
+/* particles (int32 x3 fixed point, n of them; mass NULL = all p->m): builds tree, multipoles and
+ * lists on the host (tree::refine, compute_multipoles, kick_fmm's list construction) and makes
+ * them resident on the device */
+int tg_solver_set_particles(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
+/* enqueue one force solve on the solver's stream (replaces tree::kick_fmm's kernel calls) */
+int tg_solver_solve(tg_solver* s);
+int tg_solver_synchronize(tg_solver* s);
+/* forces of the last solve, in input particle order (the reference's `force` per particle) */
+int tg_solver_get_forces(tg_solver* s, tg_force* f);
+/* device time (ms) of each kind's kernels in the last solve: 8 kinds in tg_solve_info order, then
+ * the local-expansion pass (reduce + L2L + L2P) */
+int tg_solver_kernel_ms(tg_solver* s, double* ms /* [9] */);
+int tg_solver_info(tg_solver* s, tg_solve_info* info);
+/* copy a named host array of the last build (tree order positions "pos", "order", "mass",
+ * "leaves", per node "node_range" "node_child" "node_parent" "node_level" "node_x" "node_m"
+ * "node_r" "node_boxid", and the lists "cc_d" "cp_d" "cc_e" "cp_e" "pc_d" "pp_d" "pc_e" "pp_e"
+ * with their "_ptr" CSR offsets).  Returns the size in bytes; dst may be NULL to query it. */
+int64_t tg_solver_export(tg_solver* s, const char* name, void* dst, int64_t capacity);
+const char* tg_solver_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

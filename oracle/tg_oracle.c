@@ -1024,3 +1024,176 @@ void tgo_direct64(int periodic, double h, const int32_t* xs, int64_t ns, const i
 		for (int d = 0; d < 3; d++) g[3 * i + d] = gi[d];
 	}
 }
+
+/* ========================================================================================== */
+/* One force solve over given interaction lists, the reference's way: node by node (parents     */
+/* first) L = L_parent << dX; L += CC_d, CP_d, CC_e, CP_e; at each leaf f = L2P(L) then         */
+/* += PC_d, PP_d, PC_e, PP_e (src/tree.cpp:400-401,481-491,603-626).                            */
+
+/* expansion<double>::operator<<= (tigergrav/expansion.hpp:185-233), in place */
+static void l2l_ref(double* me, const double* dX) {
+	for (int a = 0; a < 3; a++) {
+		me[0] += me[E1(a)] * dX[a];
+		for (int b = 0; b <= a; b++) {
+			me[0] += me[E2(a, b)] * dX[a] * dX[b] * EF64[E2(a, b)];
+			for (int c = 0; c <= b; c++) {
+				me[0] += me[E3(a, b, c)] * dX[a] * dX[b] * dX[c] * EF64[E3(a, b, c)];
+				for (int d = 0; d <= c; d++) me[0] += me[E4(a, b, c, d)] * dX[a] * dX[b] * dX[c] * dX[d] * EF64[E4(a, b, c, d)];
+			}
+		}
+	}
+	for (int a = 0; a < 3; a++)
+		for (int b = 0; b < 3; b++) {
+			me[E1(a)] += me[E2(a, b)] * dX[b];
+			for (int c = 0; c <= b; c++) {
+				me[E1(a)] += me[E3(a, b, c)] * dX[b] * dX[c] * EF64[E2(b, c)];
+				for (int d = 0; d <= c; d++) me[E1(a)] += me[E4(a, b, c, d)] * dX[b] * dX[c] * dX[d] * EF64[E3(b, c, d)];
+			}
+		}
+	for (int a = 0; a < 3; a++)
+		for (int b = 0; b <= a; b++)
+			for (int c = 0; c < 3; c++) {
+				me[E2(a, b)] += me[E3(a, b, c)] * dX[c];
+				for (int d = 0; d <= c; d++) me[E2(a, b)] += me[E4(a, b, c, d)] * dX[c] * dX[d] * EF64[E2(c, d)];
+			}
+	for (int a = 0; a < 3; a++)
+		for (int b = 0; b <= a; b++)
+			for (int c = 0; c <= b; c++)
+				for (int d = 0; d < 3; d++) me[E3(a, b, c)] += me[E4(a, b, c, d)] * dX[d];
+}
+
+/* expansion<double>::translate_L2 (tigergrav/expansion.hpp:236-269) */
+static void l2p_ref(const double* L, const double* dX, tgo_force* f) {
+	double phi = L[0];
+	for (int a = 0; a < 3; a++) {
+		phi += L[E1(a)] * dX[a];
+		for (int b = a; b < 3; b++) {
+			phi += L[E2(a, b)] * dX[a] * dX[b] * EF64[E2(a, b)];
+			for (int c = b; c < 3; c++) {
+				phi += L[E3(a, b, c)] * dX[a] * dX[b] * dX[c] * EF64[E3(a, b, c)];
+				for (int d = c; d < 3; d++) phi += L[E4(a, b, c, d)] * dX[a] * dX[b] * dX[c] * dX[d] * EF64[E4(a, b, c, d)];
+			}
+		}
+	}
+	f->phi = phi;
+	for (int a = 0; a < 3; a++) {
+		double g = -L[E1(a)];
+		for (int b = 0; b < 3; b++) {
+			g -= L[E2(a, b)] * dX[b];
+			for (int c = b; c < 3; c++) {
+				g -= L[E3(a, b, c)] * dX[b] * dX[c] * EF64[E2(b, c)];
+				for (int d = c; d < 3; d++) g -= L[E4(a, b, c, d)] * dX[b] * dX[c] * dX[d] * EF64[E3(b, c, d)];
+			}
+		}
+		f->g[a] = g;
+	}
+}
+
+void tgo_l2l(double* L, const double* dX) { l2l_ref(L, dX); }
+void tgo_l2p(const double* L, const double* dX, tgo_force* f) { l2p_ref(L, dX, f); }
+
+static int32_t* gather_pos(const int32_t* pos, const int32_t* ranges, int64_t b, int64_t e, int64_t* n) {
+	int64_t tot = 0;
+	for (int64_t k = b; k < e; k += 2) tot += ranges[k + 1] - ranges[k];
+	int32_t* out = malloc((size_t) (tot > 0 ? tot : 1) * 3 * sizeof(int32_t));
+	int64_t o = 0;
+	for (int64_t k = b; k < e; k += 2)
+		for (int32_t p = ranges[k]; p < ranges[k + 1]; p++, o++) memcpy(out + 3 * o, pos + 3 * (int64_t) p, 12);
+	*n = tot;
+	return out;
+}
+
+static tgo_multi_src* gather_multi(const int32_t* node_x, const float* node_m, const int32_t* ids, int64_t b, int64_t e) {
+	tgo_multi_src* out = malloc((size_t) (e > b ? e - b : 1) * sizeof(tgo_multi_src));
+	for (int64_t k = b; k < e; k++) {
+		memcpy(out[k - b].m, node_m + 17 * (int64_t) ids[k], 17 * sizeof(float));
+		memcpy(out[k - b].x, node_x + 3 * (int64_t) ids[k], 12);
+	}
+	return out;
+}
+
+/* kinds: 0 cc_d, 1 cp_d, 2 cc_e, 3 cp_e, 4 pc_d, 5 pp_d, 6 pc_e, 7 pp_e.  Nodes are ordered with
+ * parents before children.  With stride > 1 only every stride-th node's node-kind work and every
+ * stride-th leaf's leaf-kind work is done (a timing sample; results are then partial).
+ * Returns the reference-weighted flop count. */
+uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
+		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
+		const int64_t* const* ptr, const int32_t* const* lst, int stride, tgo_force* out, uint64_t* counts) {
+	double* L = calloc((size_t) nn * 35, sizeof(double));
+	uint64_t flop = 0, cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	/* node pass: levels are contiguous runs of nodes whose parents come earlier */
+	int64_t start = 0;
+	while (start < nn) {
+		int64_t end = start + 1;
+		/* nodes in [start, end) have parents < start */
+		while (end < nn && node_parent[end] < start) end++;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : flop)
+		for (int64_t n = start; n < end; n++) {
+			if (node_range[2 * n] == node_range[2 * n + 1]) continue;
+			double* Ln = L + 35 * n;
+			const int32_t p = node_parent[n];
+			if (p >= 0) {
+				memcpy(Ln, L + 35 * (int64_t) p, 35 * sizeof(double));
+				double dX[3];
+				for (int d = 0; d < 3; d++)
+					dX[d] = ((double) node_x[3 * n + d] * POS_INV + 0.5) - ((double) node_x[3 * (int64_t) p + d] * POS_INV + 0.5);
+				l2l_ref(Ln, dX);
+			}
+			if (stride > 1 && n % stride) continue;
+			for (int k = 0; k < 4; k++) {
+				const int64_t b = ptr[k][n], e = ptr[k][n + 1];
+				if (e == b) continue;
+				if (k == 0 || k == 2) {
+					tgo_multi_src* y = gather_multi(node_x, node_m, lst[k], b, e);
+					flop += (k == 0 ? tgo_CC_direct : tgo_CC_ewald)(P, Ln, node_x + 3 * n, y, e - b);
+					free(y);
+				} else {
+					int64_t nj;
+					int32_t* y = gather_pos(pos, lst[k], b, e, &nj);
+					flop += (k == 1 ? tgo_CP_direct : tgo_CP_ewald)(P, Ln, node_x + 3 * n, y, nj);
+					free(y);
+				}
+			}
+		}
+		start = end;
+	}
+	/* leaf pass */
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : flop)
+	for (int64_t n = 0; n < nn; n++) {
+		if (node_child[2 * n] >= 0) continue;
+		const int32_t pb = node_range[2 * n], pe = node_range[2 * n + 1];
+		if (pb == pe) continue;
+		if (stride > 1 && n % stride) continue;
+		const int64_t ns = pe - pb;
+		tgo_force* f = out + pb;
+		const int32_t* x = pos + 3 * (int64_t) pb;
+		for (int64_t i = 0; i < ns; i++) {
+			double dX[3];
+			for (int d = 0; d < 3; d++)
+				dX[d] = (double) (float) (((double) x[3 * i + d] * POS_INV + 0.5) - ((double) node_x[3 * n + d] * POS_INV + 0.5));
+			l2p_ref(L + 35 * n, dX, &f[i]);
+		}
+		for (int k = 4; k < 8; k++) {
+			const int64_t b = ptr[k][n], e = ptr[k][n + 1];
+			if (k == 6 && e == b) continue;
+			if (k == 7 && e == b) continue;
+			if (k == 4 || k == 6) {
+				tgo_multi_src* y = gather_multi(node_x, node_m, lst[k], b, e);
+				flop += (k == 4 ? tgo_PC_direct : tgo_PC_ewald)(P, f, x, ns, y, e - b);
+				free(y);
+			} else {
+				int64_t nj;
+				int32_t* y = gather_pos(pos, lst[k], b, e, &nj);
+				if (k == 5)
+					flop += tgo_PP_direct(P, f, x, ns, y, nj, 1);
+				else
+					flop += tgo_PP_ewald(P, f, x, ns, y, nj);
+				free(y);
+			}
+		}
+	}
+	(void) cnt;
+	(void) counts;
+	free(L);
+	return flop;
+}

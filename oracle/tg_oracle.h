@@ -85,3 +85,17 @@ void tgo_ewald_corr64(const double dx[3], double* phi, double g[3]);
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* the reference's L2L (in place) and L2P, FP64 */
+void tgo_l2l(double* L, const double* dX);
+void tgo_l2p(const double* L, const double* dX, tgo_force* f);
+/* a whole force solve over given lists (see tg_oracle.c) */
+uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
+		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
+		const int64_t* const* ptr, const int32_t* const* lst, int stride, tgo_force* out, uint64_t* counts);
+#ifdef __cplusplus
+}
+#endif

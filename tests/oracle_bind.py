@@ -200,3 +200,54 @@ def call_ref(L, kind, x, y, f=None, Lx=None, do_phi=1):
         y = np.ascontiguousarray(y, np.int32).reshape(-1, 3)
     rc = fn(_p(out), _p(x), _p(y), y.shape[0])
     return out, rc
+
+
+# ---- whole-solve restatements over exported tree arrays ----
+LIST_NAMES = ("cc_d", "cp_d", "cc_e", "cp_e", "pc_d", "pp_d", "pc_e", "pp_e")
+
+
+def _solve_args(T):
+    keep = []
+
+    def a(x, dt):
+        x = np.ascontiguousarray(x, dt)
+        keep.append(x)
+        return x
+
+    pos = a(T["pos"], np.int32)
+    nr = a(T["node_range"], np.int32)
+    nc = a(T["node_child"], np.int32)
+    npar = a(T["node_parent"], np.int32)
+    nx = a(T["node_x"], np.int32)
+    nm = a(T["node_m"], np.float32)
+    ptrs = (C.c_void_p * 8)(*[_p(a(T[n + "_ptr"], np.int64)) for n in LIST_NAMES])
+    lsts = (C.c_void_p * 8)(*[_p(a(T[n], np.int32)) if len(T[n]) else _p(a(np.zeros(1), np.int32)) for n in LIST_NAMES])
+    keep += [ptrs, lsts]
+    return pos, nr, nc, npar, nx, nm, ptrs, lsts, keep
+
+
+def oracle_solve(params, T, stride=1):
+    """The reference's solve sequence with the oracle's kernels (tgo_solve); forces in tree order."""
+    L = lib()
+    L.tgo_solve.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [C.c_void_p, C.c_void_p,
+                                                                                                 C.c_int, C.c_void_p,
+                                                                                                 C.c_void_p]
+    L.tgo_solve.restype = C.c_uint64
+    pos, nr, nc, npar, nx, nm, ptrs, lsts, keep = _solve_args(T)
+    P = Params(int(params["ewald"]), float(params["soft_len"]), float(params["m"]))
+    out = np.zeros((pos.shape[0], 4), np.float64)
+    flop = L.tgo_solve(C.byref(P), pos.shape[0], _p(pos), nr.shape[0], _p(nr), _p(nc), _p(npar), _p(nx), _p(nm), ptrs,
+                       lsts, int(stride), _p(out), None)
+    return out, flop
+
+
+def ref_solve(R, T, stride=1):
+    """The same solve with the compiled reference's own L2L/L2P and kernels (tgref_solve)."""
+    R.tgref_solve.argtypes = [C.c_int64, C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [C.c_void_p, C.c_void_p, C.c_int,
+                                                                                       C.c_void_p]
+    R.tgref_solve.restype = C.c_uint64
+    pos, nr, nc, npar, nx, nm, ptrs, lsts, keep = _solve_args(T)
+    out = np.zeros((pos.shape[0], 4), np.float64)
+    flop = R.tgref_solve(pos.shape[0], _p(pos), nr.shape[0], _p(nr), _p(nc), _p(npar), _p(nx), _p(nm), ptrs, lsts,
+                         int(stride), _p(out))
+    return out, flop

@@ -35,9 +35,19 @@ def sources():
     return out
 
 
+# host-only C++ (the tree restatement): g++, contraction off so every product and sum rounds
+# where the reference's does; HIP host API users get the ROCm include path
+HOSTCXX = os.environ.get("CXX", "g++")
+HOST = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
+        f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}", "-I/opt/rocm/include", "-Wall"]
+
+
 def _compile(src):
     obj = os.path.join(BUILDDIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-c", "-x", "hip"] + COMMON + DEVICE + ["-fopenmp", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [HOSTCXX, "-c"] + HOST + [src, "-o", obj]
+    else:
+        cmd = [HIPCC, "-c", "-x", "hip"] + COMMON + DEVICE + [src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {shlex.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -52,7 +62,8 @@ def build(verbose=False):
         objs = list(ex.map(_compile, srcs))
     out = os.path.join(LIBDIR, LIBNAME)
     tmp = out + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fopenmp", "-o", tmp] + objs
+    cmd = [HOSTCXX, "-shared", "-fopenmp", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lamdhip64",
+                                                                 "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {shlex.join(cmd)}\n{r.stdout}\n{r.stderr}")

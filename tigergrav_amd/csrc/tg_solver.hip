@@ -1,0 +1,558 @@
+// tg_solver.hip -- the batched force solve: every interaction of one solve on the device.
+//
+// Stream order (all kernels on the solver's stream):
+//   1. CC/CP (direct, Ewald) node items -> 35-coefficient partials per item
+//   2. reduce partials per node -> dL[node]                  (the reference's L += Lacc.sum())
+//   3. L2L level by level: L[n] = shift(L[parent]) + dL[n]   (src/tree.cpp:400-401,481-491)
+//   4. L2P at every leaf particle, minus phi_self            (src/tree.cpp:603-611, gravity.cpp:295)
+//   5. PC direct, PP direct, PC Ewald, PP Ewald leaf items    (src/tree.cpp:616-626)
+// which is the reference's per-node accumulation order.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "tg_gravity.h"
+#include "tg_local.hpp"
+#include "tg_solver.hpp"
+#include "tg_solver.h"
+
+namespace tg {
+
+hipError_t DeviceArray::alloc(size_t n) {
+	if (n <= bytes && p) return hipSuccess;
+	release();
+	if (n == 0) return hipSuccess;
+	hipError_t e = hipMalloc(&p, n);
+	if (e == hipSuccess) bytes = n;
+	return e;
+}
+
+void DeviceArray::release() {
+	if (p) hipFree(p);
+	p = nullptr;
+	bytes = 0;
+}
+
+// ---- kernels ---------------------------------------------------------------------------------
+__global__ void k_reduce(const int64_t* __restrict__ slot_ptr, const double* __restrict__ partial, double* __restrict__ dL,
+		int nnodes) {
+	const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (idx >= int64_t(nnodes) * 35) return;
+	const int64_t nd = idx / 35;
+	const int c = int(idx - nd * 35);
+	double s = 0.0;
+	for (int64_t k = slot_ptr[nd]; k < slot_ptr[nd + 1]; k++) s += partial[35 * k + c];
+	dL[idx] = s;
+}
+
+__global__ __launch_bounds__(64) void k_l2l(const int* __restrict__ nodes, int count, const int* __restrict__ parent,
+		const int4* __restrict__ pos, const double* __restrict__ dL, double* __restrict__ L) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	const int n = nodes[i];
+	const int p = parent[n];
+	double out[35];
+	if (p < 0) {
+#pragma unroll
+		for (int t = 0; t < 35; t++) out[t] = 0.0;
+	} else {
+		const int4 a = pos[n], b = pos[p];
+		const double d[3] = {double(int64_t(a.x) - int64_t(b.x)) * POS_INV, double(int64_t(a.y) - int64_t(b.y)) * POS_INV,
+				double(int64_t(a.z) - int64_t(b.z)) * POS_INV};
+		double Lp[35];
+#pragma unroll
+		for (int t = 0; t < 35; t++) Lp[t] = L[35 * size_t(p) + t];
+		l2l(out, Lp, d);
+	}
+#pragma unroll
+	for (int t = 0; t < 35; t++) L[35 * size_t(n) + t] = out[t] + dL[35 * size_t(n) + t];
+}
+
+// leaf item: (node, pbegin, pend, 0)
+__global__ __launch_bounds__(64) void k_l2p(const int4* __restrict__ leaves, const int4* __restrict__ pos,
+		const int4* __restrict__ parts, const double* __restrict__ L, double* __restrict__ out, double phi_self) {
+	const int4 lf = leaves[blockIdx.x];
+	const int4 c = pos[lf.x];
+	double Ll[35];
+#pragma unroll
+	for (int t = 0; t < 35; t++) Ll[t] = L[35 * size_t(lf.x) + t];
+	for (int i = lf.y + threadIdx.x; i < lf.z; i += 64) {
+		const int4 q = parts[i];
+		// the reference rounds the separation to float (src/tree.cpp:607)
+		const double d[3] = {double(float(double(int64_t(q.x) - int64_t(c.x)) * POS_INV)),
+				double(float(double(int64_t(q.y) - int64_t(c.y)) * POS_INV)),
+				double(float(double(int64_t(q.z) - int64_t(c.z)) * POS_INV))};
+		double phi, g[3];
+		l2p(phi, g, Ll, d);
+		double* o = out + 4 * size_t(i);
+		o[0] = phi - phi_self;
+		o[1] = g[0];
+		o[2] = g[1];
+		o[3] = g[2];
+	}
+}
+
+// ---- host side ---------------------------------------------------------------------------------
+#define SCHK(call)                                           \
+	do {                                                     \
+		hipError_t e_ = (call);                              \
+		if (e_ != hipSuccess) {                              \
+			err = std::string(#call) + ": " + hipGetErrorString(e_); \
+			return e_ == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;   \
+		}                                                    \
+	} while (0)
+
+static double now() {
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int Solver::init(int dev) {
+	device = dev;
+	if (dev < 0) return TG_OK;  // host-only: tree, multipoles and lists, no device work
+	int nd = 0;
+	if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) {
+		err = "no HIP device";
+		return TG_ENODEV;
+	}
+	if (dev < 0 || dev >= nd) {
+		err = "device ordinal out of range";
+		return TG_EINVAL;
+	}
+	SCHK(hipSetDevice(dev));
+	SCHK(init_device_tables(dev));
+	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+	ev.resize(2 * 16);
+	for (auto& e : ev) SCHK(hipEventCreate(&e));
+	return TG_OK;
+}
+
+Solver::~Solver() {
+	if (stream) {
+		hipSetDevice(device);
+		hipStreamSynchronize(stream);
+		for (auto& e : ev)
+			if (e) hipEventDestroy(e);
+		d_parts.release();
+		d_multis.release();
+		d_nodepos.release();
+		d_parent.release();
+		d_dL.release();
+		d_L.release();
+		d_out.release();
+		d_partial.release();
+		d_node_slot_ptr.release();
+		d_level_nodes.release();
+		d_leaf_items.release();
+		for (auto& w : work) {
+			w.items.release();
+			w.list.release();
+		}
+		hipStreamDestroy(stream);
+	}
+}
+
+int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
+	n = count;
+	TreeParams P;
+	P.ewald = ewald;
+	P.theta = theta;
+	P.soft_len = soft_len;
+	P.parts_per_node = parts_per_node;
+	double t0 = now();
+	tree.build(x, mass, count, P, m);
+	double t1 = now();
+	tree.compute_multipoles();
+	double t2 = now();
+	tree.walk();
+	double t3 = now();
+	t_tree = t1 - t0;
+	t_multipoles = t2 - t1;
+	t_walk = t3 - t2;
+	if (device < 0) return TG_OK;
+	int rc = upload();
+	t_upload = now() - t3;
+	return rc;
+}
+
+template <class T>
+static hipError_t put(DeviceArray& d, const std::vector<T>& h, hipStream_t s) {
+	hipError_t e = d.alloc(std::max<size_t>(h.size() * sizeof(T), 16));
+	if (e != hipSuccess) return e;
+	if (h.empty()) return hipSuccess;
+	return hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+int Solver::upload() {
+	SCHK(hipSetDevice(device));
+	const Tree& T = tree;
+	const int64_t nn = int64_t(T.nodes.size());
+	// particles (x, y, z, mass bits), tree order
+	std::vector<int4> parts(n);
+	for (int64_t i = 0; i < n; i++) {
+		int w;
+		std::memcpy(&w, &T.mass[i], 4);
+		parts[i] = make_int4(T.pos[3 * i], T.pos[3 * i + 1], T.pos[3 * i + 2], w);
+	}
+	std::vector<MultiSrc> multis(nn);
+	std::vector<int4> npos(nn);
+	std::vector<int> parent(nn);
+	for (int64_t i = 0; i < nn; i++) {
+		const Node& nd = T.nodes[i];
+		std::memcpy(multis[i].m, nd.m, sizeof(nd.m));
+		std::memcpy(multis[i].x, nd.x, sizeof(nd.x));
+		npos[i] = make_int4(nd.x[0], nd.x[1], nd.x[2], 0);
+		parent[i] = nd.parent;
+	}
+	// node kinds: items (slot, node, lb, le); all slots of a node are contiguous
+	const Lists& Ls = T.lists;
+	std::vector<int4> items[NKINDS];
+	std::vector<int> lists[NKINDS];
+	std::vector<int64_t> slot_ptr(nn + 1, 0);
+	int64_t slot = 0;
+	auto expand = [](std::vector<int>& dst, const std::vector<int32_t>& ranges, int64_t b, int64_t e) {
+		for (int64_t k = b; k < e; k += 2)
+			for (int32_t p = ranges[k]; p < ranges[k + 1]; p++) dst.push_back(p);
+	};
+	for (int64_t nd = 0; nd < nn; nd++) {
+		struct {
+			int kind;
+			const std::vector<int64_t>* ptr;
+			const std::vector<int32_t>* v;
+			bool ranges;
+		} src[4] = {{CC_DIRECT, &Ls.cc_d_ptr, &Ls.cc_d, false}, {CP_DIRECT, &Ls.cp_d_ptr, &Ls.cp_d, true},
+				{CC_EWALD, &Ls.cc_e_ptr, &Ls.cc_e, false}, {CP_EWALD, &Ls.cp_e_ptr, &Ls.cp_e, true}};
+		for (auto& s : src) {
+			std::vector<int>& L = lists[s.kind];
+			const int64_t lb = int64_t(L.size());
+			if (s.ranges)
+				expand(L, *s.v, (*s.ptr)[nd], (*s.ptr)[nd + 1]);
+			else
+				L.insert(L.end(), s.v->begin() + (*s.ptr)[nd], s.v->begin() + (*s.ptr)[nd + 1]);
+			const int64_t le = int64_t(L.size());
+			for (int64_t b = lb; b < le; b += node_chunk)
+				items[s.kind].push_back(make_int4(int(slot++), int(nd), int(b), int(std::min<int64_t>(le, b + node_chunk))));
+		}
+		slot_ptr[nd + 1] = slot;
+	}
+	n_slots = slot;
+	// leaf kinds: items (first sink, count <= 64, lb, le)
+	std::vector<int4> leaf_items;
+	for (int32_t lf : T.leaves) {
+		const Node& nd = T.nodes[lf];
+		if (nd.empty()) continue;
+		leaf_items.push_back(make_int4(lf, nd.pbegin, nd.pend, 0));
+		struct {
+			int kind;
+			const std::vector<int64_t>* ptr;
+			const std::vector<int32_t>* v;
+			bool ranges;
+		} src[4] = {{PC_DIRECT, &Ls.pc_d_ptr, &Ls.pc_d, false}, {PP_DIRECT, &Ls.pp_d_ptr, &Ls.pp_d, true},
+				{PC_EWALD, &Ls.pc_e_ptr, &Ls.pc_e, false}, {PP_EWALD, &Ls.pp_e_ptr, &Ls.pp_e, true}};
+		for (auto& s : src) {
+			std::vector<int>& L = lists[s.kind];
+			const int64_t lb = int64_t(L.size());
+			if (s.ranges)
+				expand(L, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
+			else
+				L.insert(L.end(), s.v->begin() + (*s.ptr)[lf], s.v->begin() + (*s.ptr)[lf + 1]);
+			const int64_t le = int64_t(L.size());
+			if (le == lb) continue;
+			for (int32_t b = nd.pbegin; b < nd.pend; b += 64)
+				items[s.kind].push_back(make_int4(b, std::min<int32_t>(64, nd.pend - b), int(lb), int(le)));
+		}
+	}
+	// heaviest leaf items first: the dispatcher hands out work groups in order
+	for (int k : {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD}) {
+		std::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
+			const int64_t ca = int64_t(a.w - a.z) * (1 << (a.y <= 1 ? 0 : 32 - __builtin_clz(a.y - 1)));
+			const int64_t cb = int64_t(b.w - b.z) * (1 << (b.y <= 1 ? 0 : 32 - __builtin_clz(b.y - 1)));
+			return ca > cb;
+		});
+	}
+	n_leaf_items = int(leaf_items.size());
+	// levels (nodes are stored breadth first, so levels are contiguous)
+	std::vector<int> level_nodes(nn);
+	level_ptr.assign(T.max_level + 2, 0);
+	for (int64_t i = 0; i < nn; i++) {
+		level_nodes[i] = int(i);
+		level_ptr[T.nodes[i].level + 1] = i + 1;
+	}
+	for (size_t l = 1; l < level_ptr.size(); l++) level_ptr[l] = std::max(level_ptr[l], level_ptr[l - 1]);
+	hipStream_t s = stream;
+	SCHK(put(d_parts, parts, s));
+	SCHK(put(d_multis, multis, s));
+	SCHK(put(d_nodepos, npos, s));
+	SCHK(put(d_parent, parent, s));
+	SCHK(put(d_node_slot_ptr, slot_ptr, s));
+	SCHK(put(d_level_nodes, level_nodes, s));
+	SCHK(put(d_leaf_items, leaf_items, s));
+	list_bytes = 0;
+	for (int k = 0; k < NKINDS; k++) {
+		SCHK(put(work[k].items, items[k], s));
+		SCHK(put(work[k].list, lists[k], s));
+		work[k].nitems = int(items[k].size());
+		work[k].nlist = int64_t(lists[k].size());
+		list_bytes += int64_t(lists[k].size() * sizeof(int) + items[k].size() * sizeof(int4));
+	}
+	SCHK(d_dL.alloc(std::max<size_t>(nn * 35 * sizeof(double), 16)));
+	SCHK(d_L.alloc(std::max<size_t>(nn * 35 * sizeof(double), 16)));
+	SCHK(d_partial.alloc(std::max<size_t>(n_slots * 35 * sizeof(double), 16)));
+	SCHK(d_out.alloc(std::max<size_t>(n * 4 * sizeof(double), 16)));
+	SCHK(hipStreamSynchronize(s));
+	return TG_OK;
+}
+
+int Solver::enqueue_solve(bool timed) {
+	SCHK(hipSetDevice(device));
+	hipStream_t s = stream;
+	const int64_t nn = int64_t(tree.nodes.size());
+	int e = 0;
+	auto mark = [&](int k) -> hipError_t { return timed ? hipEventRecord(ev[k], s) : hipSuccess; };
+	NodeArgs na{};
+	na.sinkpos = d_nodepos.as<int4>();
+	na.srcs = d_parts.as<int4>();
+	na.multis = d_multis.as<MultiSrc>();
+	na.partial = d_partial.as<double>();
+	na.wrap = ewald;
+	const int node_kinds[4] = {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD};
+	for (int k : node_kinds) {
+		na.items = work[k].items.as<int4>();
+		na.list = work[k].list.as<int>();
+		na.nitems = work[k].nitems;
+		SCHK(mark(2 * k));
+		if (k == CC_DIRECT || k == CC_EWALD)
+			SCHK(launch_cc(na, k == CC_EWALD, s));
+		else
+			SCHK(launch_cp(na, k == CP_EWALD, s));
+		SCHK(mark(2 * k + 1));
+	}
+	e = 2 * NKINDS;
+	SCHK(mark(e));
+	if (nn > 0) {
+		const int64_t tot = nn * 35;
+		hipLaunchKernelGGL(k_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, d_node_slot_ptr.as<int64_t>(),
+				d_partial.as<double>(), d_dL.as<double>(), int(nn));
+		SCHK(hipGetLastError());
+		for (size_t l = 0; l + 1 < level_ptr.size(); l++) {
+			const int cnt = int(level_ptr[l + 1] - level_ptr[l]);
+			if (cnt == 0) continue;
+			hipLaunchKernelGGL(k_l2l, dim3((cnt + 63) / 64), dim3(64), 0, s, d_level_nodes.as<int>() + level_ptr[l], cnt,
+					d_parent.as<int>(), d_nodepos.as<int4>(), d_dL.as<double>(), d_L.as<double>());
+			SCHK(hipGetLastError());
+		}
+		if (n_leaf_items > 0) {
+			const double phi_self = (-315.0 / 128.0) * m / soft_len;
+			hipLaunchKernelGGL(k_l2p, dim3(n_leaf_items), dim3(64), 0, s, d_leaf_items.as<int4>(), d_nodepos.as<int4>(),
+					d_parts.as<int4>(), d_L.as<double>(), d_out.as<double>(), phi_self);
+			SCHK(hipGetLastError());
+		}
+	}
+	SCHK(mark(e + 1));
+	LeafArgs la{};
+	la.sinks = d_parts.as<int4>();
+	la.srcs = d_parts.as<int4>();
+	la.multis = d_multis.as<MultiSrc>();
+	la.out = d_out.as<double>();
+	la.wrap = ewald;
+	const double hu = soft_len * 4294967296.0;
+	la.h2u = float(hu * hu);
+	la.hinvu = float(1.0 / hu);
+	la.hinv3u = float(1.0 / (hu * hu * hu));
+	const int leaf_kinds[4] = {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD};
+	for (int k : leaf_kinds) {
+		la.items = work[k].items.as<int4>();
+		la.list = work[k].list.as<int>();
+		la.nitems = work[k].nitems;
+		SCHK(mark(2 * k));
+		if (k == PC_DIRECT || k == PC_EWALD)
+			SCHK(launch_pc(la, k == PC_EWALD, s));
+		else if (k == PP_DIRECT)
+			SCHK(launch_pp_direct(la, true, s));
+		else
+			SCHK(launch_pp_ewald(la, s));
+		SCHK(mark(2 * k + 1));
+	}
+	return TG_OK;
+}
+
+// per-kind kernel times (ms) of the last timed solve: 8 kinds then the local-expansion pass
+int Solver::kernel_times(std::vector<float>& ms) {
+	SCHK(hipEventSynchronize(ev[2 * NKINDS + 1]));
+	ms.assign(NKINDS + 1, 0.0f);
+	for (int k = 0; k < NKINDS + 1; k++) SCHK(hipEventElapsedTime(&ms[k], ev[2 * k], ev[2 * k + 1]));
+	return TG_OK;
+}
+
+int Solver::forces(double* out4) {
+	SCHK(hipSetDevice(device));
+	std::vector<double> h(n * 4);
+	if (n > 0) SCHK(hipMemcpyAsync(h.data(), d_out.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+	SCHK(hipStreamSynchronize(stream));
+	for (int64_t i = 0; i < n; i++) {
+		const int64_t o = tree.order[i];
+		for (int c = 0; c < 4; c++) out4[4 * o + c] = h[4 * i + c];
+	}
+	return TG_OK;
+}
+
+}  // namespace tg
+
+// ---- C ABI -------------------------------------------------------------------------------------
+struct tg_solver {
+	tg::Solver s;
+};
+
+namespace {
+thread_local std::string g_solver_err;
+int fail(tg_solver* h, int rc) {
+	if (rc != TG_OK && h) g_solver_err = h->s.err;
+	return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int tg_solver_create(const tg_params* p, double theta, int parts_per_node, tg_solver** out) {
+	if (!p || !out || !(theta > 0.0) || parts_per_node < 1 || !(p->soft_len > 0.0)) return TG_EINVAL;
+	*out = nullptr;
+	tg_solver* h = new tg_solver();
+	h->s.ewald = p->ewald ? 1 : 0;
+	h->s.soft_len = p->soft_len;
+	h->s.m = p->m;
+	h->s.theta = theta;
+	h->s.parts_per_node = parts_per_node;
+	int rc = h->s.init(p->device);
+	if (rc != TG_OK) {
+		g_solver_err = h->s.err;
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return TG_OK;
+}
+
+int tg_solver_destroy(tg_solver* h) {
+	delete h;
+	return TG_OK;
+}
+
+int tg_solver_set_particles(tg_solver* h, const int32_t* x, const float* mass, int64_t n) {
+	if (!h || n < 0 || (n > 0 && !x) || n > (int64_t(1) << 31) - 64) return TG_EINVAL;
+	return fail(h, h->s.set_particles(x, mass, n));
+}
+
+int tg_solver_solve(tg_solver* h) {
+	if (!h || h->s.device < 0) return TG_EINVAL;
+	return fail(h, h->s.enqueue_solve(true));
+}
+
+int tg_solver_synchronize(tg_solver* h) {
+	if (!h || h->s.device < 0) return TG_EINVAL;
+	hipError_t e = hipStreamSynchronize(h->s.stream);
+	if (e != hipSuccess) {
+		g_solver_err = hipGetErrorString(e);
+		return TG_EHIP;
+	}
+	return TG_OK;
+}
+
+int tg_solver_get_forces(tg_solver* h, tg_force* f) {
+	if (!h || h->s.device < 0 || (h->s.n > 0 && !f)) return TG_EINVAL;
+	return fail(h, h->s.forces(reinterpret_cast<double*>(f)));
+}
+
+int tg_solver_kernel_ms(tg_solver* h, double* ms) {
+	if (!h || h->s.device < 0 || !ms) return TG_EINVAL;
+	std::vector<float> v;
+	int rc = h->s.kernel_times(v);
+	if (rc != TG_OK) return fail(h, rc);
+	for (size_t k = 0; k < v.size(); k++) ms[k] = v[k];
+	return TG_OK;
+}
+
+int tg_solver_info(tg_solver* h, tg_solve_info* info) {
+	if (!h || !info) return TG_EINVAL;
+	const tg::Tree& T = h->s.tree;
+	std::memset(info, 0, sizeof(*info));
+	info->n = h->s.n;
+	info->n_nodes = int64_t(T.nodes.size());
+	info->n_leaves = int64_t(T.leaves.size());
+	info->max_level = T.max_level;
+	const tg::Census& C = T.census;
+	const uint64_t cnt[8] = {C.PP_direct, C.PC_direct, C.CC_direct, C.CP_direct, C.PP_ewald, C.PC_ewald, C.CC_ewald, C.CP_ewald};
+	for (int k = 0; k < 8; k++) info->count[k] = cnt[k];
+	for (int k = 0; k < 8; k++) {
+		info->items[k] = h->s.work[k].nitems;
+		info->list_len[k] = h->s.work[k].nlist;
+	}
+	info->list_bytes = h->s.list_bytes;
+	info->t_tree = h->s.t_tree;
+	info->t_multipoles = h->s.t_multipoles;
+	info->t_walk = h->s.t_walk;
+	info->t_upload = h->s.t_upload;
+	return TG_OK;
+}
+
+// copy a named host-side array of the last build into dst (capacity in bytes); returns the
+// array's size in bytes, or a negative status
+int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capacity) {
+	if (!h || !name) return TG_EINVAL;
+	const tg::Tree& T = h->s.tree;
+	std::vector<char> buf;
+	auto put = [&](const void* p, size_t bytes) {
+		buf.resize(bytes);
+		if (bytes) std::memcpy(buf.data(), p, bytes);
+	};
+	const std::string nm(name);
+	const size_t nn = T.nodes.size();
+	if (nm == "pos") put(T.pos.data(), T.pos.size() * 4);
+	else if (nm == "order") put(T.order.data(), T.order.size() * 8);
+	else if (nm == "mass") put(T.mass.data(), T.mass.size() * 4);
+	else if (nm == "leaves") put(T.leaves.data(), T.leaves.size() * 4);
+	else if (nm == "node_range" || nm == "node_child" || nm == "node_parent" || nm == "node_level" || nm == "node_x" ||
+	         nm == "node_m" || nm == "node_r" || nm == "node_boxid") {
+		std::vector<char> v;
+		for (size_t i = 0; i < nn; i++) {
+			const tg::Node& d = T.nodes[i];
+			const char* p = nullptr;
+			size_t b = 0;
+			int32_t tmp[2];
+			uint64_t bid[2];
+			if (nm == "node_range") { tmp[0] = d.pbegin; tmp[1] = d.pend; p = (const char*) tmp; b = 8; }
+			else if (nm == "node_child") { tmp[0] = d.child[0]; tmp[1] = d.child[1]; p = (const char*) tmp; b = 8; }
+			else if (nm == "node_parent") { p = (const char*) &d.parent; b = 4; }
+			else if (nm == "node_level") { p = (const char*) &d.level; b = 4; }
+			else if (nm == "node_x") { p = (const char*) d.x; b = 12; }
+			else if (nm == "node_m") { p = (const char*) d.m; b = 68; }
+			else if (nm == "node_r") { p = (const char*) &d.r; b = 4; }
+			else { bid[0] = uint64_t(d.boxid); bid[1] = uint64_t(d.boxid >> 64); p = (const char*) bid; b = 16; }
+			v.insert(v.end(), p, p + b);
+		}
+		buf.swap(v);
+	}
+#define TG_EXPORT_LIST(NAME)                                                   \
+	else if (nm == #NAME) put(T.lists.NAME.data(), T.lists.NAME.size() * 4);   \
+	else if (nm == #NAME "_ptr") put(T.lists.NAME##_ptr.data(), T.lists.NAME##_ptr.size() * 8);
+	TG_EXPORT_LIST(cc_d)
+	TG_EXPORT_LIST(cp_d)
+	TG_EXPORT_LIST(cc_e)
+	TG_EXPORT_LIST(cp_e)
+	TG_EXPORT_LIST(pc_d)
+	TG_EXPORT_LIST(pp_d)
+	TG_EXPORT_LIST(pc_e)
+	TG_EXPORT_LIST(pp_e)
+#undef TG_EXPORT_LIST
+	else return TG_EINVAL;
+	if (dst) {
+		if (capacity < int64_t(buf.size())) return TG_EINVAL;
+		std::memcpy(dst, buf.data(), buf.size());
+	}
+	return int64_t(buf.size());
+}
+
+const char* tg_solver_last_error(void) { return g_solver_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// tg_solver.hpp -- batched, device-resident force solve over the tree's interaction lists.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "tg_kernels.hpp"
+#include "tg_tree.hpp"
+
+namespace tg {
+
+struct DeviceArray {
+	void* p = nullptr;
+	size_t bytes = 0;
+	hipError_t alloc(size_t n);
+	void release();
+	template <class T>
+	T* as() const {
+		return static_cast<T*>(p);
+	}
+};
+
+// one kind's work on the device
+struct KindWork {
+	DeviceArray items;  // int4
+	DeviceArray list;   // int32 source indices
+	int nitems = 0;
+	int64_t nlist = 0;
+};
+
+struct Solver {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int ewald = 1;
+	double soft_len = 0.0, m = 0.0, theta = 0.7;
+	int parts_per_node = 64;
+	int node_chunk = 64;  // sources per CC/CP work item
+	Tree tree;
+	int64_t n = 0;
+	// device state
+	DeviceArray d_parts, d_multis, d_nodepos, d_parent, d_dL, d_L, d_out, d_partial, d_node_slot_ptr;
+	DeviceArray d_level_nodes, d_leaf_items;
+	KindWork work[NKINDS];
+	std::vector<int64_t> level_ptr;  // CSR of nodes by level (host)
+	int n_leaf_items = 0;
+	int64_t n_slots = 0;
+	int64_t list_bytes = 0;
+	// timings (seconds) of the last build and solve
+	double t_tree = 0, t_multipoles = 0, t_walk = 0, t_upload = 0;
+	std::vector<hipEvent_t> ev;
+	std::string err;
+
+	int init(int dev);
+	int set_particles(const int32_t* x, const float* mass, int64_t count);
+	int upload();
+	int enqueue_solve(bool timed);
+	int kernel_times(std::vector<float>& ms);
+	int forces(double* out4);  // N x (phi, gx, gy, gz), input order
+	~Solver();
+};
+
+}  // namespace tg
