@@ -1,0 +1,195 @@
+"""Benchmark: periodic force solve of 2,097,152 particles on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload zeldovich128] [--theta 0.7] [--leaf 64]
+
+Workload (default): 128^3 particles displaced by a Zel'dovich field of a seeded Gaussian random
+field with P(k) ~ k^-2 (cut at the grid Nyquist), rms displacement 0.3 cell, periodic (Ewald),
+theta = 0.7, leaf size 64, softening 0.02 N^(-1/3), equal masses 1/N.
+
+One step = one force solve on the device over the resident tree and interaction lists: the CC/CP
+node kernels, the local-expansion pass (reduce + L2L + L2P) and the PC/PP leaf kernels.  The tree,
+multipoles and lists are built once on the host before the timed region (their time is reported
+as the end-to-end part of `s_per_force_solve_end_to_end`).
+
+Prints one JSON line.  `value` is interactions per second (the reference's interaction census,
+src/tree.cpp:483-494,618-628, divided by the device time of one solve).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "interactions/s + s per force solve, 2M periodic, RMS |Δa|/|a| ≤1e-3"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# algorithmic flops per interaction (add/mul = 1, FMA = 2), counted from the reference formulas
+ALG_FLOPS = {"PP_direct": 76, "PC_direct": 478, "CC_direct": 621, "CP_direct": 321, "PP_ewald": 30057,
+             "PC_ewald": 127958, "CC_ewald": 128101, "CP_ewald": 127801}
+# algorithmic bytes per interaction for the dominant-kernel HBM line: a PP source is one 16-byte
+# particle record read once per sink group of <= 64, a multipole 80 bytes
+KINDS = ("PP_direct", "PC_direct", "CC_direct", "CP_direct", "PP_ewald", "PC_ewald", "CC_ewald", "CP_ewald")
+
+
+def make_workload(name):
+    from tigergrav_amd import ics
+
+    if name == "zeldovich128":
+        return ics.zeldovich(128, seed=3, n_s=-2.0, rms_cells=0.3), True, (
+            "zeldovich 128^3 = 2,097,152 particles, P(k)~k^-2 cut at grid Nyquist, rms displacement 0.3 cell, "
+            "seed 3, periodic Ewald")
+    if name == "plummer2M":
+        return ics.plummer_spheres(2 ** 21, seed=2), False, (
+            "8 Plummer spheres (a = 0.02) + 20% uniform background, 2,097,152 particles, seed 2, open boundaries")
+    if name == "uniform32k":
+        return ics.uniform(32768, seed=1), True, "uniform 32,768 particles, seed 1, periodic Ewald"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(s, ewald, h, m, theta, leaf, target_s=15.0):
+    """Time the reference's own compiled kernels (oracle/_ref, if built) -- else the oracle's C
+    restatement -- over a strided sample of the same interaction lists, on all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    T = s.tree_arrays()
+    N = T["pos"].shape[0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    kind = "reference" if ob.ref_available() else "port"
+    if kind == "reference":
+        R = ob.ref_lib()
+        R.tgref_set_options(int(ewald), h, 1.0, N, theta)
+        run = lambda stride: ob.ref_solve(R, T, stride)
+    else:
+        run = lambda stride: ob.oracle_solve({"ewald": int(ewald), "soft_len": h, "m": m}, T, stride)
+    # interactions in the sample (census restricted to sampled nodes / leaves)
+    nr = T["node_range"]
+    ns = (nr[:, 1] - nr[:, 0]).astype(np.int64)
+
+    def sample_count(stride):
+        tot = 0
+        nodes = np.arange(len(nr))
+        sel = nodes[nodes % stride == 0]
+        for k in ("cc_d", "cc_e"):
+            p = T[k + "_ptr"]
+            tot += int(np.sum(p[sel + 1] - p[sel]))
+        for k in ("cp_d", "cp_e"):
+            p, v = T[k + "_ptr"], T[k].reshape(-1, 2)
+            lens = v[:, 1] - v[:, 0]
+            cs = np.concatenate([[0], np.cumsum(lens)])
+            tot += int(np.sum(cs[p[sel + 1] // 2] - cs[p[sel] // 2]))
+        leaves = sel[T["node_child"][sel, 0] < 0]
+        for k in ("pc_d", "pc_e"):
+            p = T[k + "_ptr"]
+            tot += int(np.sum((p[leaves + 1] - p[leaves]) * ns[leaves]))
+        for k in ("pp_d", "pp_e"):
+            p, v = T[k + "_ptr"], T[k].reshape(-1, 2)
+            lens = v[:, 1] - v[:, 0]
+            cs = np.concatenate([[0], np.cumsum(lens)])
+            tot += int(np.sum((cs[p[leaves + 1] // 2] - cs[p[leaves] // 2]) * ns[leaves]))
+        return tot
+
+    stride = 64
+    t0 = time.time()
+    run(stride)
+    dt = time.time() - t0
+    # choose the stride that gives about target_s seconds of work on all threads
+    stride = max(1, min(64, int(stride * dt / (target_s / max(1, threads)) + 0.5) or 1))
+    stride = 1 if stride <= 1 else stride
+    t0 = time.time()
+    run(stride)
+    dt = time.time() - t0
+    cnt = sample_count(stride)
+    return {"value": cnt / dt, "unit": "interactions/s", "cores": threads, "kind": kind,
+            "sample": f"every {stride}th node's and leaf's interaction lists of the same solve ({cnt} interactions, "
+                      f"{dt:.2f} s wall on {threads} threads)",
+            "s_per_force_solve_extrapolated": dt * sum(s.info()["count"].values()) / max(cnt, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="zeldovich128")
+    ap.add_argument("--theta", type=float, default=0.7)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
+              "decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    x, ewald, desc = make_workload(args.workload)
+    N = x.shape[0]
+    h, m = ics.soft_len(N), 1.0 / N
+    s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
+    t0 = time.time()
+    s.set_particles(x)
+    t_host = time.time() - t0
+    info = s.info()
+    count = info["count"]
+    total = sum(count.values())
+
+    for _ in range(args.warmup):
+        s.solve()
+    s.synchronize()
+    # per-kind device times from HIP events on the solver's stream (one more solve)
+    s.solve()
+    s.synchronize()
+    kms = s.kernel_ms()
+    # timed region: K solves back to back on the resident data
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.solve()
+    s.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = total / (ms / 1e3)
+
+    dom = max(KINDS, key=lambda k: kms[k])
+    achieved = count[dom] * ALG_FLOPS[dom] / (kms[dom] / 1e3) / 1e12
+    roofline = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "flops_per_interaction": ALG_FLOPS[dom], "interactions": count[dom], "kernel_ms": kms[dom]}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            tr = json.load(open(traffic_file))
+            key = f"{args.workload}:{dom}"
+            if key in tr:
+                roofline["traffic"] = tr[key]
+        except Exception:
+            pass
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "interactions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "N": N, "theta": args.theta, "leaf": args.leaf, "ewald": ewald, "softening": h,
+                   "accumulation": "f64", "parallelism": "single GPU"},
+        "s_per_force_solve": ms / 1e3,
+        "s_per_force_solve_end_to_end": t_host + ms / 1e3,
+        "host_build_s": {"tree": info["t_tree"], "multipoles": info["t_multipoles"], "lists": info["t_walk"],
+                         "upload": info["t_upload"]},
+        "interactions": count, "kernel_ms": kms,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
