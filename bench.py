@@ -29,11 +29,13 @@ sys.path.insert(0, ROOT)
 METRIC = "interactions/s + s per force solve, 2M periodic, RMS |Δa|/|a| ≤1e-3"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# algorithmic flops per interaction (add/mul = 1, FMA = 2), counted from the reference formulas
-ALG_FLOPS = {"PP_direct": 76, "PC_direct": 478, "CC_direct": 621, "CP_direct": 321, "PP_ewald": 30057,
+# algorithmic flops per interaction (add/sub/mul/div/sqrt/min/max = 1, FMA = 2), counted from the
+# reference formulas.  PP_direct: the far-field branch (r >= h) of src/gravity.cpp:231-287 with the
+# potential -- dX 3, r2 5, sqrt 1, 1/max(r, h/2) 2, rinv^3 2, g -= dX*M*rinv^3 9, phi -= M*rinv 2 --
+# which every pair of the 2M workloads but a few in 1e6 takes; the spline branch for r < h adds 52
+# and the reference's SIMD code evaluates all three branches for every pair (76 in total).
+ALG_FLOPS = {"PP_direct": 24, "PC_direct": 478, "CC_direct": 621, "CP_direct": 321, "PP_ewald": 30057,
              "PC_ewald": 127958, "CC_ewald": 128101, "CP_ewald": 127801}
-# algorithmic bytes per interaction for the dominant-kernel HBM line: a PP source is one 16-byte
-# particle record read once per sink group of <= 64, a multipole 80 bytes
 KINDS = ("PP_direct", "PC_direct", "CC_direct", "CP_direct", "PP_ewald", "PC_ewald", "CC_ewald", "CP_ewald")
 
 
@@ -95,20 +97,20 @@ def cpu_baseline(s, ewald, h, m, theta, leaf, target_s=15.0):
             tot += int(np.sum((cs[p[leaves + 1] // 2] - cs[p[leaves] // 2]) * ns[leaves]))
         return tot
 
+    # a probe on every 64th node's lists, then the largest sample (the whole solve if it fits) that
+    # takes about target_s seconds of wall time on all threads
     stride = 64
     t0 = time.time()
     run(stride)
     dt = time.time() - t0
-    # choose the stride that gives about target_s seconds of work on all threads
-    stride = max(1, min(64, int(stride * dt / (target_s / max(1, threads)) + 0.5) or 1))
-    stride = 1 if stride <= 1 else stride
+    stride = max(1, min(64, math.ceil(stride * dt / target_s)))
     t0 = time.time()
     run(stride)
     dt = time.time() - t0
     cnt = sample_count(stride)
     return {"value": cnt / dt, "unit": "interactions/s", "cores": threads, "kind": kind,
-            "sample": f"every {stride}th node's and leaf's interaction lists of the same solve ({cnt} interactions, "
-                      f"{dt:.2f} s wall on {threads} threads)",
+            "sample": ("the whole solve" if stride == 1 else f"every {stride}th node's and leaf's interaction lists of "
+                       "the same solve") + f" ({cnt} interactions, {dt:.2f} s wall on {threads} threads)",
             "s_per_force_solve_extrapolated": dt * sum(s.info()["count"].values()) / max(cnt, 1)}
 
 
@@ -144,12 +146,9 @@ def main():
     for _ in range(args.warmup):
         s.solve()
     s.synchronize()
-    # per-kind device times from HIP events on the solver's stream (one more solve)
-    s.solve()
-    s.synchronize()
-    kms = s.kernel_ms()
-    # timed region: K solves back to back on the resident data
-    s.synchronize()
+    s.collect_ms()  # drop the warmup solves' event times
+    # timed region: K solves back to back on the resident data; every solve records HIP events
+    # around each kind's kernels on the solver's stream
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.solve()
@@ -157,6 +156,8 @@ def main():
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     value = total / (ms / 1e3)
+    kms, nsolves = s.collect_ms()  # mean per-solve device time of each kind over the timed solves
+    assert nsolves == args.steps
 
     dom = max(KINDS, key=lambda k: kms[k])
     achieved = count[dom] * ALG_FLOPS[dom] / (kms[dom] / 1e3) / 1e12

@@ -48,6 +48,9 @@ int tg_solver_get_forces(tg_solver* s, tg_force* f);
 /* device time (ms) of each kind's kernels in the last solve: 8 kinds in tg_solve_info order, then
  * the local-expansion pass (reduce + L2L + L2P) */
 int tg_solver_kernel_ms(tg_solver* s, double* ms /* [9] */);
+/* sums of the same 9 device times over every solve enqueued since the previous call, and the
+ * number of those solves (waits for them); the sums restart at zero afterwards */
+int tg_solver_collect_ms(tg_solver* s, double* ms_sum /* [9] */, int64_t* solves);
 int tg_solver_info(tg_solver* s, tg_solve_info* info);
 /* copy a named host array of the last build (tree order positions "pos", "order", "mass",
  * "leaves", per node "node_range" "node_child" "node_parent" "node_level" "node_x" "node_m"

@@ -55,6 +55,7 @@ struct ThreadCtx {
 	hipStream_t stream = nullptr;
 	DevBuf sinks, srcs, multis, items, out;
 	std::vector<int4> hsinks, hsrcs, hitems;
+	std::vector<int2> hwaves;
 	std::vector<double> hout;
 	~ThreadCtx() {
 		// device memory is left to process teardown: a thread may outlive the HIP runtime
@@ -144,6 +145,8 @@ int leaf_call(const tg_params* p, tg_force* f, const int32_t* x, int64_t ns, con
 	a.out = static_cast<double*>(T->out.p);
 	a.nitems = int(ngroups);
 	a.wrap = p->ewald ? 1 : 0;
+	a.uniform = 1;
+	a.m = m;
 	const double hu = p->soft_len * 4294967296.0;
 	a.h2u = float(hu * hu);
 	a.hinvu = float(1.0 / hu);
@@ -173,18 +176,18 @@ int node_call(const tg_params* p, double* L, const int32_t* x, const void* y, in
 	if (rc) return rc;
 	const float m = float(p->m);
 	const bool multi = (kind == tg::CC_DIRECT || kind == tg::CC_EWALD);
+	// one wave (and one partial slot) per 64 sources
 	const int64_t chunk = 64;
 	const int64_t nitems = (nj + chunk - 1) / chunk;
-	T->hitems.resize(nitems);
-	for (int64_t i = 0; i < nitems; i++)
-		T->hitems[i] = make_int4(int(i), 0, int(i * chunk), int(std::min<int64_t>(nj, (i + 1) * chunk)));
+	T->hwaves.resize(nitems);
+	for (int64_t i = 0; i < nitems; i++) T->hwaves[i] = make_int2(int(i * chunk), int(std::min<int64_t>(nj, (i + 1) * chunk)));
 	T->hsinks.assign(1, pack(x, 0.0f));
 	HIPCHK(T->sinks.ensure(sizeof(int4)));
-	HIPCHK(T->items.ensure(nitems * sizeof(int4)));
+	HIPCHK(T->items.ensure(nitems * sizeof(int2)));
 	HIPCHK(T->out.ensure(nitems * 35 * sizeof(double)));
 	hipStream_t s = T->stream;
 	HIPCHK(hipMemcpyAsync(T->sinks.p, T->hsinks.data(), sizeof(int4), hipMemcpyHostToDevice, s));
-	HIPCHK(hipMemcpyAsync(T->items.p, T->hitems.data(), nitems * sizeof(int4), hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(T->items.p, T->hwaves.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, s));
 	if (multi) {
 		HIPCHK(T->multis.ensure(nj * sizeof(tg_multi_src)));
 		HIPCHK(hipMemcpyAsync(T->multis.p, y, nj * sizeof(tg_multi_src), hipMemcpyHostToDevice, s));
@@ -199,10 +202,12 @@ int node_call(const tg_params* p, double* L, const int32_t* x, const void* y, in
 	a.sinkpos = static_cast<const int4*>(T->sinks.p);
 	a.srcs = static_cast<const int4*>(T->srcs.p);
 	a.multis = static_cast<const tg::MultiSrc*>(T->multis.p);
-	a.items = static_cast<const int4*>(T->items.p);
+	a.waves = static_cast<const int2*>(T->items.p);
 	a.list = nullptr;
+	a.owner = nullptr;
+	a.slot_node = nullptr;
 	a.partial = static_cast<double*>(T->out.p);
-	a.nitems = int(nitems);
+	a.nwaves = int(nitems);
 	a.wrap = p->ewald ? 1 : 0;
 	switch (kind) {
 		case tg::CC_DIRECT: HIPCHK(tg::launch_cc(a, false, s)); break;

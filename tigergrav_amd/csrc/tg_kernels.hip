@@ -7,12 +7,14 @@
 //    are staged 64 at a time through LDS and read back as broadcasts.  Per-tile partial sums are
 //    float, the running sum FP64 (the reference accumulates float terms in FP64 lanes,
 //    src/gravity.cpp:259-261), and phases are merged with cross-lane FP64 shuffles.
-//  * node kinds (CC, CP): one wave per item, lane = source; 35 float accumulators per lane,
-//    merged across lanes in FP64 through LDS.
+//  * node kinds (CC, CP): lane = source; a wave packs the short lists of several expansion
+//    centres (<= 64 entries), 35 float accumulators per lane, merged per centre in FP64 through
+//    LDS by a segmented scan.
 //  * Ewald kinds use tg::green_ewald / green_ewald1 (tg_math.hpp).
 #include "tg_kernels.hpp"
 #include "tg_math.hpp"
 
+#include <climits>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -112,62 +114,87 @@ __device__ __forceinline__ double wave_xor(double v, int off) { return __shfl_xo
 // ---- PP direct -------------------------------------------------------------------------------
 // Spline-softened (GADGET-2 cubic spline, support h) potential and acceleration
 // (src/gravity.cpp:231-287).  Beyond h it is Newton's 1/r; the softened branch is taken only by
-// the lanes that need it.
-template <bool WRAP, bool PHI>
+// the lanes that need it.  The 64-source tile is stored phase-major in LDS, so every lane walks
+// consecutive entries with a wave-uniform trip count; a partial last tile is padded with
+// zero-weight entries.  With equal masses (UNIFORM) the mass multiplies the sums once at the end.
+template <bool USE_W, bool PHI>
+__device__ __forceinline__ void pp_pair(const int4& me, const int4& s, bool wrap_unused, float h2u, float hinvu, float hinv3u,
+		float& ax, float& ay, float& az, float& pp, float fx, float fy, float fz) {
+	const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+	const float w = __int_as_float(s.w);
+	if (r2 > h2u) {
+		const float ri = __builtin_amdgcn_rsqf(r2);
+		const float wri = USE_W ? w * ri : ri;
+		const float wri3 = wri * ri * ri;
+		ax = fmaf(fx, wri3, ax);
+		ay = fmaf(fy, wri3, ay);
+		az = fmaf(fz, wri3, az);
+		if (PHI) pp += wri;
+	} else {
+		const float ru = sqrtf(r2);
+		const float q = ru * hinvu;
+		float f, p;
+		if (q < 0.5f) {
+			const float q2 = q * q;
+			f = fmaf(q2, fmaf(32.0f, q, -192.0f / 5.0f), 32.0f / 3.0f) * hinv3u;
+			p = fmaf(q2, fmaf(q2, fmaf(-32.0f / 5.0f, q, 48.0f / 5.0f), -16.0f / 3.0f), 14.0f / 5.0f) * hinvu;
+		} else {
+			const float ri = 1.0f / ru;
+			const float q3 = q * q * q;
+			f = fmaf(fmaf(fmaf(fmaf(-32.0f / 3.0f, q, 192.0f / 5.0f), q, -48.0f), q, 64.0f / 3.0f), q3, -1.0f / 15.0f) *
+			    (ri * ri * ri);
+			p = fmaf(fmaf(fmaf(fmaf(fmaf(32.0f / 15.0f, q, -48.0f / 5.0f), q, 16.0f), q, -32.0f / 3.0f), q * q, 16.0f / 5.0f), q,
+			         -1.0f / 15.0f) * ri;
+		}
+		const float wf = w * f;
+		ax = fmaf(fx, wf, ax);
+		ay = fmaf(fy, wf, ay);
+		az = fmaf(fz, wf, az);
+		if (PHI) pp = fmaf(w, p, pp);
+	}
+}
+
+template <bool WRAP, bool PHI, bool UNIFORM>
 __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	const int4 it = A.items[blockIdx.x];
 	const int lane = threadIdx.x;
 	const int ns = it.y;
 	const int lg = ceil_log2_64(ns);
-	const int nsp = 1 << lg;
-	const int P = 64 >> lg;
+	const int nsp = 1 << lg;           // sinks padded to a power of two
+	const int lgP = 6 - lg;            // phases P = 64 / nsp
+	const int P = 1 << lgP;
 	const int si = lane & (nsp - 1);
 	const int ph = lane >> lg;
 	const int4 me = A.sinks[it.x + min(si, ns - 1)];
 	__shared__ int4 tile[64];
+	// source l of a tile goes to phase l % P, position l / P
+	const int dst = (lane & (P - 1)) * nsp + (lane >> lgP);
+	const int4* mine = tile + ph * nsp;
 	float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
 	double dax = 0.0, day = 0.0, daz = 0.0, dpp = 0.0;
 	for (int base = it.z; base < it.w; base += 64) {
 		const int n = min(64, it.w - base);
 		__syncthreads();
-		if (lane < n) tile[lane] = A.srcs[A.list ? A.list[base + lane] : base + lane];
+		int4 v = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);  // zero weight
+		if (lane < n) {
+			v = A.srcs[A.list ? A.list[base + lane] : base + lane];
+			if (UNIFORM) v.w = __float_as_int(1.0f);
+		}
+		tile[dst] = v;
 		__syncthreads();
-		for (int k = ph; k < n; k += P) {
-			const int4 s = tile[k];
-			const float fx = sepu<WRAP>(me.x, s.x);
-			const float fy = sepu<WRAP>(me.y, s.y);
-			const float fz = sepu<WRAP>(me.z, s.z);
-			const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-			const float w = __int_as_float(s.w);
-			if (r2 > A.h2u) {
-				const float ri = __builtin_amdgcn_rsqf(r2);
-				const float wri = w * ri;
-				const float wri3 = wri * ri * ri;
-				ax = fmaf(fx, wri3, ax);
-				ay = fmaf(fy, wri3, ay);
-				az = fmaf(fz, wri3, az);
-				if (PHI) pp += wri;
-			} else {
-				const float ru = sqrtf(r2);
-				const float q = ru * A.hinvu;
-				float f, p;
-				if (q < 0.5f) {
-					const float q2 = q * q;
-					f = fmaf(q2, fmaf(32.0f, q, -192.0f / 5.0f), 32.0f / 3.0f) * A.hinv3u;
-					p = fmaf(q2, fmaf(q2, fmaf(-32.0f / 5.0f, q, 48.0f / 5.0f), -16.0f / 3.0f), 14.0f / 5.0f) * A.hinvu;
-				} else {
-					const float ri = 1.0f / ru;
-					const float q3 = q * q * q;
-					f = fmaf(fmaf(fmaf(fmaf(-32.0f / 3.0f, q, 192.0f / 5.0f), q, -48.0f), q, 64.0f / 3.0f), q3, -1.0f / 15.0f) *
-					    (ri * ri * ri);
-					p = fmaf(fmaf(fmaf(fmaf(fmaf(32.0f / 15.0f, q, -48.0f / 5.0f), q, 16.0f), q, -32.0f / 3.0f), q * q, 16.0f / 5.0f), q,
-					         -1.0f / 15.0f) * ri;
-				}
-				const float wf = w * f;
-				ax = fmaf(fx, wf, ax);
-				ay = fmaf(fy, wf, ay);
-				az = fmaf(fz, wf, az);
-				if (PHI) pp = fmaf(w, p, pp);
+		if (n == 64) {
+#pragma unroll 4
+			for (int j = 0; j < nsp; j++) {
+				const int4 s = mine[j];
+				pp_pair<!UNIFORM, PHI>(me, s, WRAP, A.h2u, A.hinvu, A.hinv3u, ax, ay, az, pp, sepu<WRAP>(me.x, s.x),
+						sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
+			}
+		} else {
+			const int trip = (n + P - 1) >> lgP;
+			for (int j = 0; j < trip; j++) {
+				const int4 s = mine[j];
+				pp_pair<true, PHI>(me, s, WRAP, A.h2u, A.hinvu, A.hinv3u, ax, ay, az, pp, sepu<WRAP>(me.x, s.x),
+						sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
 			}
 		}
 		dax += ax;
@@ -184,27 +211,27 @@ __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	}
 	if (ph == 0 && si < ns) {
 		// position units: g = -sum w dx r^-3 = -2^64 sum (w du ru^-3), phi = -2^32 sum (w / ru)
+		const double ms = UNIFORM ? double(A.m) : 1.0;
 		double* o = A.out + 4 * (size_t) (it.x + si);
-		if (PHI) o[0] += -4294967296.0 * dpp;
-		o[1] += -18446744073709551616.0 * dax;
-		o[2] += -18446744073709551616.0 * day;
-		o[3] += -18446744073709551616.0 * daz;
+		if (PHI) o[0] += -4294967296.0 * ms * dpp;
+		o[1] += -18446744073709551616.0 * ms * dax;
+		o[2] += -18446744073709551616.0 * ms * day;
+		o[3] += -18446744073709551616.0 * ms * daz;
 	}
 }
 
 hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s) {
 	if (a.nitems <= 0) return hipSuccess;
+#define TG_PP(W, PH, U) hipLaunchKernelGGL((k_pp_direct<W, PH, U>), dim3(a.nitems), dim3(64), 0, s, a)
+	const bool u = a.uniform != 0;
 	if (a.wrap) {
-		if (do_phi)
-			hipLaunchKernelGGL((k_pp_direct<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
-		else
-			hipLaunchKernelGGL((k_pp_direct<true, false>), dim3(a.nitems), dim3(64), 0, s, a);
+		if (do_phi) { if (u) TG_PP(true, true, true); else TG_PP(true, true, false); }
+		else { if (u) TG_PP(true, false, true); else TG_PP(true, false, false); }
 	} else {
-		if (do_phi)
-			hipLaunchKernelGGL((k_pp_direct<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
-		else
-			hipLaunchKernelGGL((k_pp_direct<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+		if (do_phi) { if (u) TG_PP(false, true, true); else TG_PP(false, true, false); }
+		else { if (u) TG_PP(false, false, true); else TG_PP(false, false, false); }
 	}
+#undef TG_PP
 	return hipGetLastError();
 }
 
@@ -341,29 +368,46 @@ hipError_t launch_pp_ewald(const LeafArgs& a, hipStream_t s) {
 	return hipGetLastError();
 }
 
-// ---- node kinds: cross-lane FP64 reduction of 35 float accumulators --------------------------
-__device__ __forceinline__ void node_reduce_store(float* L, double* dst) {
+// ---- node kinds ----------------------------------------------------------------------------------
+// Segmented FP64 reduction of the lanes' 35 float accumulators by owner slot, through LDS.  The
+// entries of one slot are consecutive, so lanes 0..34 (one coefficient each) scan the wave once.
+__device__ __forceinline__ void node_segment_store(const float* L, int slot, int cnt, const int* slot_node_unused,
+		double* partial) {
 	__shared__ float red[35][65];
+	__shared__ int sl[64];
 	const int lane = threadIdx.x;
 #pragma unroll
 	for (int c = 0; c < 35; c++) red[c][lane] = L[c];
+	sl[lane] = slot;
 	__syncthreads();
 	if (lane < 35) {
-		double s = 0.0;
-		for (int j = 0; j < 64; j++) s += double(red[lane][j]);
-		dst[lane] = s;
+		int cur = sl[0];
+		double acc = 0.0;
+		for (int j = 0; j < cnt; j++) {
+			const int sj = sl[j];
+			if (sj != cur) {
+				partial[35 * (size_t) cur + lane] = acc;
+				cur = sj;
+				acc = 0.0;
+			}
+			acc += double(red[lane][j]);
+		}
+		partial[35 * (size_t) cur + lane] = acc;
 	}
 }
 
 template <bool EWALD, bool WRAP>
 __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
-	const int4 it = A.items[blockIdx.x];
+	const int2 wv = A.waves[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int4 X = A.sinkpos[it.y];
+	const int j = wv.x + lane;
 	float L[35];
 #pragma unroll
 	for (int c = 0; c < 35; c++) L[c] = 0.f;
-	for (int j = it.z + lane; j < it.w; j += 64) {
+	int slot = -1;
+	if (j < wv.y) {
+		slot = A.owner ? A.owner[j] : int(blockIdx.x);
+		const int4 X = A.sinkpos[A.slot_node ? A.slot_node[slot] : 0];
 		const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[j] : j));
 		float v[20];
 #pragma unroll
@@ -384,18 +428,21 @@ __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 			green_direct(D, x);
 		m2l(L, v, D);
 	}
-	node_reduce_store(L, A.partial + 35 * (size_t) it.x);
+	node_segment_store(L, slot, wv.y - wv.x, A.slot_node, A.partial);
 }
 
 template <bool EWALD, bool WRAP>
 __global__ __launch_bounds__(64) void k_cp(NodeArgs A) {
-	const int4 it = A.items[blockIdx.x];
+	const int2 wv = A.waves[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int4 X = A.sinkpos[it.y];
+	const int j = wv.x + lane;
 	float L[35];
 #pragma unroll
 	for (int c = 0; c < 35; c++) L[c] = 0.f;
-	for (int j = it.z + lane; j < it.w; j += 64) {
+	int slot = -1;
+	if (j < wv.y) {
+		slot = A.owner ? A.owner[j] : int(blockIdx.x);
+		const int4 X = A.sinkpos[A.slot_node ? A.slot_node[slot] : 0];
 		const int4 s = A.srcs[A.list ? A.list[j] : j];
 		const float x[3] = {sepu<EWALD || WRAP>(X.x, s.x) * POS_INVF, sepu<EWALD || WRAP>(X.y, s.y) * POS_INVF,
 				sepu<EWALD || WRAP>(X.z, s.z) * POS_INVF};
@@ -406,28 +453,28 @@ __global__ __launch_bounds__(64) void k_cp(NodeArgs A) {
 			green_direct(D, x);
 		p2l(L, __int_as_float(s.w), D);
 	}
-	node_reduce_store(L, A.partial + 35 * (size_t) it.x);
+	node_segment_store(L, slot, wv.y - wv.x, A.slot_node, A.partial);
 }
 
 hipError_t launch_cc(const NodeArgs& a, bool ewald, hipStream_t s) {
-	if (a.nitems <= 0) return hipSuccess;
+	if (a.nwaves <= 0) return hipSuccess;
 	if (ewald)
-		hipLaunchKernelGGL((k_cc<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((k_cc<true, true>), dim3(a.nwaves), dim3(64), 0, s, a);
 	else if (a.wrap)
-		hipLaunchKernelGGL((k_cc<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((k_cc<false, true>), dim3(a.nwaves), dim3(64), 0, s, a);
 	else
-		hipLaunchKernelGGL((k_cc<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((k_cc<false, false>), dim3(a.nwaves), dim3(64), 0, s, a);
 	return hipGetLastError();
 }
 
 hipError_t launch_cp(const NodeArgs& a, bool ewald, hipStream_t s) {
-	if (a.nitems <= 0) return hipSuccess;
+	if (a.nwaves <= 0) return hipSuccess;
 	if (ewald)
-		hipLaunchKernelGGL((k_cp<true, true>), dim3(a.nitems), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((k_cp<true, true>), dim3(a.nwaves), dim3(64), 0, s, a);
 	else if (a.wrap)
-		hipLaunchKernelGGL((k_cp<false, true>), dim3(a.nitems), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((k_cp<false, true>), dim3(a.nwaves), dim3(64), 0, s, a);
 	else
-		hipLaunchKernelGGL((k_cp<false, false>), dim3(a.nitems), dim3(64), 0, s, a);
+		hipLaunchKernelGGL((k_cp<false, false>), dim3(a.nwaves), dim3(64), 0, s, a);
 	return hipGetLastError();
 }
 

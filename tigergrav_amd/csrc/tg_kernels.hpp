@@ -4,11 +4,13 @@
 //   item = (first sink, sink count <= 64, list begin, list end)
 // and results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
 // atomics are needed while kernels of one stream run in order).
-// Node kinds (CC/CP: the sink is an expansion centre):
-//   item = (partial slot, sink position index, list begin, list end)
-// and each item writes its 35 FP64 partial coefficients to partial[35 * slot].
+// Node kinds (CC/CP: the sink is an expansion centre): a wave takes a run of <= 64 consecutive list
+// entries, waves[w] = [begin, end); entry j belongs to partial slot owner[j], whose expansion centre
+// is sinkpos[slot_node[slot]].  Every slot's entries lie inside one wave, so each slot's 35 FP64
+// coefficients are written exactly once (no atomics, deterministic).
 // `list` holds source indices (particles for PP/CP, multipoles for PC/CC); NULL means the
-// identity list.
+// identity list.  For node kinds owner == NULL means one slot per wave (slot = wave index) and
+// slot_node == NULL means every slot's centre is sinkpos[0].
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -29,6 +31,8 @@ struct LeafArgs {
 	double* out;             // 4 doubles per sink
 	int nitems;
 	int wrap;                // periodic minimum-image separations (direct kinds)
+	int uniform;             // all source masses equal to m (PP direct sums once, scales at the end)
+	float m;                 // the common mass when uniform
 	float h2u, hinvu, hinv3u;  // softening in position units (2^32 per box length)
 };
 
@@ -36,10 +40,12 @@ struct NodeArgs {
 	const int4* sinkpos;     // expansion centres (x, y, z, unused)
 	const int4* srcs;        // source particles (CP)
 	const MultiSrc* multis;  // source multipoles (CC)
-	const int4* items;
-	const int* list;
-	double* partial;         // 35 doubles per item
-	int nitems;
+	const int2* waves;       // [begin, end) runs of list entries, <= 64 each
+	const int* list;         // source index per entry
+	const int* owner;        // partial slot per entry
+	const int* slot_node;    // expansion-centre index per slot
+	double* partial;         // 35 doubles per slot
+	int nwaves;
 	int wrap;
 };
 

@@ -6,6 +6,8 @@
 //   L2L:  L'_t = sum_{|s| <= 4-|t|} L_{t u s} W_s(d)
 //   L2P:  phi  = sum_s L_s W_s(d),   g_a = -sum_{|s| <= 3} L_{s u a} W_s(d)
 #pragma once
+#include <utility>
+
 #include "tg_math.hpp"
 
 namespace tg {
@@ -38,51 +40,85 @@ __host__ __device__ constexpr int merge_slot(int t, int s) {
 	return slot_of_counts(count_idx(SL[t], 0) + count_idx(SL[s], 0), count_idx(SL[t], 1) + count_idx(SL[s], 1),
 			count_idx(SL[t], 2) + count_idx(SL[s], 2));
 }
-__host__ __device__ constexpr double fact(int n) { return n <= 1 ? 1.0 : n * fact(n - 1); }
+__host__ __device__ constexpr double fact4(int n) {  // n! for n <= 4
+	return n == 4 ? 24.0 : (n == 3 ? 6.0 : (n == 2 ? 2.0 : 1.0));
+}
 __host__ __device__ constexpr double slot_weight(int s) {
 	// mult(s)/|s|! = 1 / prod(count_v!)
-	return 1.0 / (fact(count_idx(SL[s], 0)) * fact(count_idx(SL[s], 1)) * fact(count_idx(SL[s], 2)));
+	return 1.0 / (fact4(count_idx(SL[s], 0)) * fact4(count_idx(SL[s], 1)) * fact4(count_idx(SL[s], 2)));
+}
+
+// Compile-time slot arithmetic: every index below is a template argument, so the expansions
+// are straight-line code over registers.
+template <int T, int S>
+struct Merge {
+	static constexpr bool valid = SL[T].ord + SL[S].ord <= 4;
+	static constexpr int value = valid ? merge_slot(T, S) : 0;
+};
+
+template <int S>
+__device__ __forceinline__ double taylor_weight(const double* d) {
+	constexpr double w0 = slot_weight(S);
+	constexpr int o = SL[S].ord;
+	double w = w0;
+	if constexpr (o > 0) w *= d[SL[S].i[0]];
+	if constexpr (o > 1) w *= d[SL[S].i[1]];
+	if constexpr (o > 2) w *= d[SL[S].i[2]];
+	if constexpr (o > 3) w *= d[SL[S].i[3]];
+	return w;
+}
+
+template <int... S>
+__device__ __forceinline__ void taylor_weights_seq(double* W, const double* d, std::integer_sequence<int, S...>) {
+	((W[S] = taylor_weight<S>(d)), ...);
 }
 
 __device__ __forceinline__ void taylor_weights(double* W, const double* d) {
-#pragma unroll
-	for (int s = 0; s < 35; s++) {
-		double w = slot_weight(s);
-#pragma unroll
-		for (int k = 0; k < 4; k++)
-			if (k < SL[s].ord) w *= d[SL[s].i[k]];
-		W[s] = w;
-	}
+	taylor_weights_seq(W, d, std::make_integer_sequence<int, 35>{});
+}
+
+// sum over the slots s with ord(t) + ord(s) <= 4 of L[t u s] W[s], in slot order
+template <int T, int... S>
+__device__ __forceinline__ double shift_row(const double* L, const double* W, std::integer_sequence<int, S...>) {
+	double acc = 0.0;
+	((acc = Merge<T, S>::valid ? fma(L[Merge<T, S>::value], W[S], acc) : acc), ...);
+	return acc;
+}
+
+template <int... T>
+__device__ __forceinline__ void shift_all(double* out, const double* L, const double* W, std::integer_sequence<int, T...>) {
+	((out[T] = shift_row<T>(L, W, std::make_integer_sequence<int, 35>{})), ...);
 }
 
 // L (35, FP64) re-expanded about a centre displaced by d: out = L2L(L, d)
 __device__ __forceinline__ void l2l(double* out, const double* L, const double* d) {
 	double W[35];
 	taylor_weights(W, d);
-#pragma unroll
-	for (int t = 0; t < 35; t++) {
-		double acc = 0.0;
-#pragma unroll
-		for (int s = 0; s < 35; s++)
-			if (SL[t].ord + SL[s].ord <= 4) acc = fma(L[merge_slot(t, s)], W[s], acc);
-		out[t] = acc;
-	}
+	shift_all(out, L, W, std::make_integer_sequence<int, 35>{});
+}
+
+// potential and acceleration at offset d: phi = sum_s L_s W_s, g_a = -sum_{|s| <= 3} L_{s u a} W_s
+template <int A, int... S>
+__device__ __forceinline__ double grad_row(const double* L, const double* W, std::integer_sequence<int, S...>) {
+	double acc = 0.0;
+	((acc = fma(L[Merge<S, 1 + A>::value], W[S], acc)), ...);
+	return -acc;
+}
+
+template <int... S>
+__device__ __forceinline__ double dot_seq(const double* L, const double* W, std::integer_sequence<int, S...>) {
+	double acc = 0.0;
+	((acc = fma(L[S], W[S], acc)), ...);
+	return acc;
 }
 
 __device__ __forceinline__ void l2p(double& phi, double* g, const double* L, const double* d) {
 	double W[35];
 	taylor_weights(W, d);
-	double p = 0.0;
-#pragma unroll
-	for (int s = 0; s < 35; s++) p = fma(L[s], W[s], p);
-	phi = p;
-#pragma unroll
-	for (int a = 0; a < 3; a++) {
-		double acc = 0.0;
-#pragma unroll
-		for (int s = 0; s < 20; s++) acc = fma(L[merge_slot(s, 1 + a)], W[s], acc);
-		g[a] = -acc;
-	}
+	phi = dot_seq(L, W, std::make_integer_sequence<int, 35>{});
+	g[0] = grad_row<0>(L, W, std::make_integer_sequence<int, 20>{});
+	g[1] = grad_row<1>(L, W, std::make_integer_sequence<int, 20>{});
+	g[2] = grad_row<2>(L, W, std::make_integer_sequence<int, 20>{});
 }
 
 }  // namespace tg

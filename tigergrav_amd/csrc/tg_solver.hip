@@ -3,7 +3,7 @@
 // Stream order (all kernels on the solver's stream):
 //   1. CC/CP (direct, Ewald) node items -> 35-coefficient partials per item
 //   2. reduce partials per node -> dL[node]                  (the reference's L += Lacc.sum())
-//   3. L2L level by level: L[n] = shift(L[parent]) + dL[n]   (src/tree.cpp:400-401,481-491)
+//   3. per leaf: L = sum over the root-to-leaf path of shift(dL[a]) (src/tree.cpp:400-401,481-491)
 //   4. L2P at every leaf particle, minus phi_self            (src/tree.cpp:603-611, gravity.cpp:295)
 //   5. PC direct, PP direct, PC Ewald, PP Ewald leaf items    (src/tree.cpp:616-626)
 // which is the reference's per-node accumulation order.
@@ -37,56 +37,70 @@ void DeviceArray::release() {
 }
 
 // ---- kernels ---------------------------------------------------------------------------------
-__global__ void k_reduce(const int64_t* __restrict__ slot_ptr, const double* __restrict__ partial, double* __restrict__ dL,
-		int nnodes) {
-	const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (idx >= int64_t(nnodes) * 35) return;
-	const int64_t nd = idx / 35;
-	const int c = int(idx - nd * 35);
-	double s = 0.0;
-	for (int64_t k = slot_ptr[nd]; k < slot_ptr[nd + 1]; k++) s += partial[35 * k + c];
-	dL[idx] = s;
+// Each node's own CC/CP contribution: the sum of its partial slots in slot order (the
+// reference adds each list's accumulator into the node's L, src/tree.cpp:400-401,481-491).
+// One thread per (node, coefficient); consecutive threads read consecutive doubles.
+__global__ __launch_bounds__(256) void k_node_sum(const int64_t* __restrict__ slot_ptr,
+		const double* __restrict__ partial, double* __restrict__ dL, int64_t nn) {
+	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+	if (i >= nn * 35) return;
+	const int64_t a = i / 35;
+	const int q = int(i - 35 * a);
+	double acc = 0.0;
+	for (int64_t z = slot_ptr[a]; z < slot_ptr[a + 1]; z++) acc += partial[35 * z + q];
+	dL[i] = acc;
 }
 
-__global__ __launch_bounds__(64) void k_l2l(const int* __restrict__ nodes, int count, const int* __restrict__ parent,
-		const int4* __restrict__ pos, const double* __restrict__ dL, double* __restrict__ L) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= count) return;
-	const int n = nodes[i];
-	const int p = parent[n];
-	double out[35];
-	if (p < 0) {
-#pragma unroll
-		for (int t = 0; t < 35; t++) out[t] = 0.0;
-	} else {
-		const int4 a = pos[n], b = pos[p];
-		const double d[3] = {double(int64_t(a.x) - int64_t(b.x)) * POS_INV, double(int64_t(a.y) - int64_t(b.y)) * POS_INV,
-				double(int64_t(a.z) - int64_t(b.z)) * POS_INV};
-		double Lp[35];
-#pragma unroll
-		for (int t = 0; t < 35; t++) Lp[t] = L[35 * size_t(p) + t];
-		l2l(out, Lp, d);
-	}
-#pragma unroll
-	for (int t = 0; t < 35; t++) L[35 * size_t(n) + t] = out[t] + dL[35 * size_t(n) + t];
-}
-
-// leaf item: (node, pbegin, pend, 0)
-__global__ __launch_bounds__(64) void k_l2p(const int4* __restrict__ leaves, const int4* __restrict__ pos,
-		const int4* __restrict__ parts, const double* __restrict__ L, double* __restrict__ out, double phi_self) {
+// The local-expansion pass of one leaf (block of 64).  The shift of a degree-4 Taylor polynomial
+// is exact, so shifts compose and the leaf's expansion is the sum over its root-to-leaf path of
+// every node's own contribution dL_a re-expanded about the leaf centre:
+//   L_leaf = sum_a shift(dL_a, X_leaf - X_a)
+// (the reference shifts the running sum node by node, src/tree.cpp:400-401; the two agree up to
+// rounding).  Lane k shifts the k-th node of the path with the unrolled 210-FMA re-expansion, so no
+// lane waits on a table; lanes 0..34 then add the path's terms coefficient by coefficient, root
+// first.  leaf item: (node, pbegin, pend, path begin); the path has level(node) + 1 <= TG_MAXDEPTH
+// entries.
+#define TG_MAXDEPTH 64
+static_assert(TG_MAXDEPTH <= 64, "one lane per path entry");
+__global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, const int* __restrict__ path,
+		const int* __restrict__ level, const int4* __restrict__ pos, const int4* __restrict__ parts,
+		const double* __restrict__ dL, double* __restrict__ out, double phi_self, int maxdepth) {
 	const int4 lf = leaves[blockIdx.x];
+	const int t = threadIdx.x;
+	const int depth = level[lf.x] + 1;
+	// dynamic LDS: [35][maxdepth] shifted contributions, then [35] the leaf's expansion
+	extern __shared__ double lds[];
+	double* Ls = lds + 35 * maxdepth;
 	const int4 c = pos[lf.x];
-	double Ll[35];
+	if (t < depth) {
+		const int a = path[lf.w + t];
+		const int4 A = pos[a];
+		const double d[3] = {double(int64_t(c.x) - int64_t(A.x)) * POS_INV, double(int64_t(c.y) - int64_t(A.y)) * POS_INV,
+				double(int64_t(c.z) - int64_t(A.z)) * POS_INV};
+		const double* src = dL + 35 * int64_t(a);
+		double La[35], Lt[35];
 #pragma unroll
-	for (int t = 0; t < 35; t++) Ll[t] = L[35 * size_t(lf.x) + t];
-	for (int i = lf.y + threadIdx.x; i < lf.z; i += 64) {
+		for (int q = 0; q < 35; q++) La[q] = src[q];
+		l2l(Lt, La, d);
+#pragma unroll
+		for (int q = 0; q < 35; q++) lds[q * maxdepth + t] = Lt[q];
+	}
+	__syncthreads();
+	if (t < 35) {
+		double acc = 0.0;
+		for (int k = 0; k < depth; k++) acc += lds[t * maxdepth + k];
+		Ls[t] = acc;
+	}
+	__syncthreads();
+	// evaluate at the leaf's particles
+	for (int i = lf.y + t; i < lf.z; i += 64) {
 		const int4 q = parts[i];
-		// the reference rounds the separation to float (src/tree.cpp:607)
+		// the reference rounds the offset from the leaf centre to float (src/tree.cpp:607)
 		const double d[3] = {double(float(double(int64_t(q.x) - int64_t(c.x)) * POS_INV)),
 				double(float(double(int64_t(q.y) - int64_t(c.y)) * POS_INV)),
 				double(float(double(int64_t(q.z) - int64_t(c.z)) * POS_INV))};
 		double phi, g[3];
-		l2p(phi, g, Ll, d);
+		l2p(phi, g, Ls, d);
 		double* o = out + 4 * size_t(i);
 		o[0] = phi - phi_self;
 		o[1] = g[0];
@@ -94,6 +108,8 @@ __global__ __launch_bounds__(64) void k_l2p(const int4* __restrict__ leaves, con
 		o[3] = g[2];
 	}
 }
+
+static size_t local_lds_bytes(int maxdepth) { return size_t(35 * maxdepth + 35) * sizeof(double); }
 
 // ---- host side ---------------------------------------------------------------------------------
 #define SCHK(call)                                           \
@@ -124,7 +140,7 @@ int Solver::init(int dev) {
 	SCHK(hipSetDevice(dev));
 	SCHK(init_device_tables(dev));
 	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-	ev.resize(2 * 16);
+	ev.resize(size_t(2) * TIMER_SPANS * TIMER_RING);
 	for (auto& e : ev) SCHK(hipEventCreate(&e));
 	return TG_OK;
 }
@@ -138,17 +154,18 @@ Solver::~Solver() {
 		d_parts.release();
 		d_multis.release();
 		d_nodepos.release();
-		d_parent.release();
-		d_dL.release();
-		d_L.release();
 		d_out.release();
 		d_partial.release();
 		d_node_slot_ptr.release();
-		d_level_nodes.release();
+		d_dL.release();
 		d_leaf_items.release();
+		d_slot_node.release();
+		d_path.release();
+		d_level.release();
 		for (auto& w : work) {
 			w.items.release();
 			w.list.release();
+			w.owner.release();
 		}
 		hipStreamDestroy(stream);
 	}
@@ -162,6 +179,7 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	P.soft_len = soft_len;
 	P.parts_per_node = parts_per_node;
 	double t0 = now();
+	uniform_mass = (mass == nullptr);
 	tree.build(x, mass, count, P, m);
 	double t1 = now();
 	tree.compute_multipoles();
@@ -198,24 +216,26 @@ int Solver::upload() {
 	}
 	std::vector<MultiSrc> multis(nn);
 	std::vector<int4> npos(nn);
-	std::vector<int> parent(nn);
 	for (int64_t i = 0; i < nn; i++) {
 		const Node& nd = T.nodes[i];
 		std::memcpy(multis[i].m, nd.m, sizeof(nd.m));
 		std::memcpy(multis[i].x, nd.x, sizeof(nd.x));
 		npos[i] = make_int4(nd.x[0], nd.x[1], nd.x[2], 0);
-		parent[i] = nd.parent;
 	}
-	// node kinds: items (slot, node, lb, le); all slots of a node are contiguous
+	// node kinds: each node's list is cut into chunks of <= node_chunk entries, one partial slot per
+	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64
 	const Lists& Ls = T.lists;
 	std::vector<int4> items[NKINDS];
-	std::vector<int> lists[NKINDS];
+	std::vector<int> lists[NKINDS], owner[NKINDS];
+	std::vector<int2> chunks[NKINDS];
+	std::vector<int> slot_node;
 	std::vector<int64_t> slot_ptr(nn + 1, 0);
 	int64_t slot = 0;
 	auto expand = [](std::vector<int>& dst, const std::vector<int32_t>& ranges, int64_t b, int64_t e) {
 		for (int64_t k = b; k < e; k += 2)
 			for (int32_t p = ranges[k]; p < ranges[k + 1]; p++) dst.push_back(p);
 	};
+	const int chunk = std::min(64, std::max(1, node_chunk));
 	for (int64_t nd = 0; nd < nn; nd++) {
 		struct {
 			int kind;
@@ -232,10 +252,33 @@ int Solver::upload() {
 			else
 				L.insert(L.end(), s.v->begin() + (*s.ptr)[nd], s.v->begin() + (*s.ptr)[nd + 1]);
 			const int64_t le = int64_t(L.size());
-			for (int64_t b = lb; b < le; b += node_chunk)
-				items[s.kind].push_back(make_int4(int(slot++), int(nd), int(b), int(std::min<int64_t>(le, b + node_chunk))));
+			for (int64_t b = lb; b < le; b += chunk) {
+				const int64_t e = std::min<int64_t>(le, b + chunk);
+				chunks[s.kind].push_back(make_int2(int(b), int(e)));
+				owner[s.kind].insert(owner[s.kind].end(), size_t(e - b), int(slot));
+				slot_node.push_back(int(nd));
+				slot++;
+			}
 		}
 		slot_ptr[nd + 1] = slot;
+	}
+	std::vector<int2> waves[NKINDS];
+	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
+		int2 cur = make_int2(0, 0);
+		bool open = false;
+		for (const int2& c : chunks[k]) {
+			if (open && c.y - cur.x > 64) {
+				waves[k].push_back(cur);
+				open = false;
+			}
+			if (!open) {
+				cur = c;
+				open = true;
+			} else {
+				cur.y = c.y;
+			}
+		}
+		if (open) waves[k].push_back(cur);
 	}
 	n_slots = slot;
 	// leaf kinds: items (first sink, count <= 64, lb, le)
@@ -273,33 +316,47 @@ int Solver::upload() {
 		});
 	}
 	n_leaf_items = int(leaf_items.size());
-	// levels (nodes are stored breadth first, so levels are contiguous)
-	std::vector<int> level_nodes(nn);
-	level_ptr.assign(T.max_level + 2, 0);
-	for (int64_t i = 0; i < nn; i++) {
-		level_nodes[i] = int(i);
-		level_ptr[T.nodes[i].level + 1] = i + 1;
+	// root-to-leaf paths for the local-expansion pass
+	std::vector<int> path, level(nn);
+	for (int64_t i = 0; i < nn; i++) level[i] = T.nodes[i].level;
+	if (T.max_level + 1 > TG_MAXDEPTH) {
+		err = "tree deeper than the local-expansion pass supports (64 levels)";
+		return TG_EINVAL;
 	}
-	for (size_t l = 1; l < level_ptr.size(); l++) level_ptr[l] = std::max(level_ptr[l], level_ptr[l - 1]);
+	for (auto& li : leaf_items) {
+		li.w = int(path.size());
+		std::vector<int> up;
+		for (int a = li.x; a >= 0; a = T.nodes[a].parent) up.push_back(a);
+		path.insert(path.end(), up.rbegin(), up.rend());
+	}
 	hipStream_t s = stream;
 	SCHK(put(d_parts, parts, s));
 	SCHK(put(d_multis, multis, s));
 	SCHK(put(d_nodepos, npos, s));
-	SCHK(put(d_parent, parent, s));
 	SCHK(put(d_node_slot_ptr, slot_ptr, s));
-	SCHK(put(d_level_nodes, level_nodes, s));
 	SCHK(put(d_leaf_items, leaf_items, s));
+	SCHK(put(d_slot_node, slot_node, s));
+	SCHK(put(d_path, path, s));
+	SCHK(put(d_level, level, s));
 	list_bytes = 0;
 	for (int k = 0; k < NKINDS; k++) {
-		SCHK(put(work[k].items, items[k], s));
+		const bool node_kind = (k == CC_DIRECT || k == CP_DIRECT || k == CC_EWALD || k == CP_EWALD);
+		if (node_kind) {
+			SCHK(put(work[k].items, waves[k], s));
+			SCHK(put(work[k].owner, owner[k], s));
+			work[k].nitems = int(waves[k].size());
+			list_bytes += int64_t(waves[k].size() * sizeof(int2) + owner[k].size() * sizeof(int));
+		} else {
+			SCHK(put(work[k].items, items[k], s));
+			work[k].nitems = int(items[k].size());
+			list_bytes += int64_t(items[k].size() * sizeof(int4));
+		}
 		SCHK(put(work[k].list, lists[k], s));
-		work[k].nitems = int(items[k].size());
 		work[k].nlist = int64_t(lists[k].size());
-		list_bytes += int64_t(lists[k].size() * sizeof(int) + items[k].size() * sizeof(int4));
+		list_bytes += int64_t(lists[k].size() * sizeof(int));
 	}
-	SCHK(d_dL.alloc(std::max<size_t>(nn * 35 * sizeof(double), 16)));
-	SCHK(d_L.alloc(std::max<size_t>(nn * 35 * sizeof(double), 16)));
 	SCHK(d_partial.alloc(std::max<size_t>(n_slots * 35 * sizeof(double), 16)));
+	SCHK(d_dL.alloc(std::max<size_t>(T.nodes.size() * 35 * sizeof(double), 16)));
 	SCHK(d_out.alloc(std::max<size_t>(n * 4 * sizeof(double), 16)));
 	SCHK(hipStreamSynchronize(s));
 	return TG_OK;
@@ -310,18 +367,28 @@ int Solver::enqueue_solve(bool timed) {
 	hipStream_t s = stream;
 	const int64_t nn = int64_t(tree.nodes.size());
 	int e = 0;
-	auto mark = [&](int k) -> hipError_t { return timed ? hipEventRecord(ev[k], s) : hipSuccess; };
+	if (timed && solves_recorded - solves_folded >= TIMER_RING) {
+		// the set about to be reused still holds an unfolded solve: add its times first
+		const int rc = fold_set(solves_folded);
+		if (rc != TG_OK) return rc;
+		solves_folded++;
+	}
+	hipEvent_t* evs = ev.data() + size_t(2) * TIMER_SPANS * size_t(solves_recorded % TIMER_RING);
+	if (timed) solves_recorded++;
+	auto mark = [&](int k) -> hipError_t { return timed ? hipEventRecord(evs[k], s) : hipSuccess; };
 	NodeArgs na{};
 	na.sinkpos = d_nodepos.as<int4>();
 	na.srcs = d_parts.as<int4>();
 	na.multis = d_multis.as<MultiSrc>();
 	na.partial = d_partial.as<double>();
+	na.slot_node = d_slot_node.as<int>();
 	na.wrap = ewald;
 	const int node_kinds[4] = {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD};
 	for (int k : node_kinds) {
-		na.items = work[k].items.as<int4>();
+		na.waves = work[k].items.as<int2>();
 		na.list = work[k].list.as<int>();
-		na.nitems = work[k].nitems;
+		na.owner = work[k].owner.as<int>();
+		na.nwaves = work[k].nitems;
 		SCHK(mark(2 * k));
 		if (k == CC_DIRECT || k == CC_EWALD)
 			SCHK(launch_cc(na, k == CC_EWALD, s));
@@ -331,24 +398,16 @@ int Solver::enqueue_solve(bool timed) {
 	}
 	e = 2 * NKINDS;
 	SCHK(mark(e));
-	if (nn > 0) {
-		const int64_t tot = nn * 35;
-		hipLaunchKernelGGL(k_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, d_node_slot_ptr.as<int64_t>(),
-				d_partial.as<double>(), d_dL.as<double>(), int(nn));
+	if (nn > 0 && n_leaf_items > 0) {
+		const double phi_self = (-315.0 / 128.0) * m / soft_len;
+		const int maxdepth = tree.max_level + 1;
+		hipLaunchKernelGGL(k_node_sum, dim3(unsigned((nn * 35 + 255) / 256)), dim3(256), 0, s,
+				d_node_slot_ptr.as<int64_t>(), d_partial.as<double>(), d_dL.as<double>(), nn);
 		SCHK(hipGetLastError());
-		for (size_t l = 0; l + 1 < level_ptr.size(); l++) {
-			const int cnt = int(level_ptr[l + 1] - level_ptr[l]);
-			if (cnt == 0) continue;
-			hipLaunchKernelGGL(k_l2l, dim3((cnt + 63) / 64), dim3(64), 0, s, d_level_nodes.as<int>() + level_ptr[l], cnt,
-					d_parent.as<int>(), d_nodepos.as<int4>(), d_dL.as<double>(), d_L.as<double>());
-			SCHK(hipGetLastError());
-		}
-		if (n_leaf_items > 0) {
-			const double phi_self = (-315.0 / 128.0) * m / soft_len;
-			hipLaunchKernelGGL(k_l2p, dim3(n_leaf_items), dim3(64), 0, s, d_leaf_items.as<int4>(), d_nodepos.as<int4>(),
-					d_parts.as<int4>(), d_L.as<double>(), d_out.as<double>(), phi_self);
-			SCHK(hipGetLastError());
-		}
+		hipLaunchKernelGGL(k_local, dim3(n_leaf_items), dim3(64), local_lds_bytes(maxdepth), s, d_leaf_items.as<int4>(),
+				d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(), d_dL.as<double>(),
+				d_out.as<double>(), phi_self, maxdepth);
+		SCHK(hipGetLastError());
 	}
 	SCHK(mark(e + 1));
 	LeafArgs la{};
@@ -357,6 +416,8 @@ int Solver::enqueue_solve(bool timed) {
 	la.multis = d_multis.as<MultiSrc>();
 	la.out = d_out.as<double>();
 	la.wrap = ewald;
+	la.uniform = uniform_mass ? 1 : 0;
+	la.m = float(m);
 	const double hu = soft_len * 4294967296.0;
 	la.h2u = float(hu * hu);
 	la.hinvu = float(1.0 / hu);
@@ -380,9 +441,35 @@ int Solver::enqueue_solve(bool timed) {
 
 // per-kind kernel times (ms) of the last timed solve: 8 kinds then the local-expansion pass
 int Solver::kernel_times(std::vector<float>& ms) {
-	SCHK(hipEventSynchronize(ev[2 * NKINDS + 1]));
-	ms.assign(NKINDS + 1, 0.0f);
-	for (int k = 0; k < NKINDS + 1; k++) SCHK(hipEventElapsedTime(&ms[k], ev[2 * k], ev[2 * k + 1]));
+	ms.assign(TIMER_SPANS, 0.0f);
+	if (solves_recorded == 0) return TG_OK;
+	const hipEvent_t* evs = ev.data() + size_t(2) * TIMER_SPANS * size_t((solves_recorded - 1) % TIMER_RING);
+	SCHK(hipEventSynchronize(evs[2 * TIMER_SPANS - 1]));
+	for (int k = 0; k < TIMER_SPANS; k++) SCHK(hipEventElapsedTime(&ms[k], evs[2 * k], evs[2 * k + 1]));
+	return TG_OK;
+}
+
+int Solver::fold_set(int64_t solve) {
+	const hipEvent_t* evs = ev.data() + size_t(2) * TIMER_SPANS * size_t(solve % TIMER_RING);
+	SCHK(hipEventSynchronize(evs[2 * TIMER_SPANS - 1]));
+	for (int k = 0; k < TIMER_SPANS; k++) {
+		float t = 0.0f;
+		SCHK(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]));
+		acc_ms[k] += t;
+	}
+	return TG_OK;
+}
+
+int Solver::collect_times(double* ms_sum, int64_t* count) {
+	while (solves_folded < solves_recorded) {
+		const int rc = fold_set(solves_folded);
+		if (rc != TG_OK) return rc;
+		solves_folded++;
+	}
+	for (int k = 0; k < TIMER_SPANS; k++) ms_sum[k] = acc_ms[k];
+	*count = solves_recorded;
+	acc_ms.assign(TIMER_SPANS, 0.0);
+	solves_recorded = solves_folded = 0;
 	return TG_OK;
 }
 
@@ -471,6 +558,11 @@ int tg_solver_kernel_ms(tg_solver* h, double* ms) {
 	if (rc != TG_OK) return fail(h, rc);
 	for (size_t k = 0; k < v.size(); k++) ms[k] = v[k];
 	return TG_OK;
+}
+
+int tg_solver_collect_ms(tg_solver* h, double* ms_sum, int64_t* solves) {
+	if (!h || h->s.device < 0 || !ms_sum || !solves) return TG_EINVAL;
+	return fail(h, h->s.collect_times(ms_sum, solves));
 }
 
 int tg_solver_info(tg_solver* h, tg_solve_info* info) {

@@ -24,8 +24,9 @@ struct DeviceArray {
 
 // one kind's work on the device
 struct KindWork {
-	DeviceArray items;  // int4
+	DeviceArray items;  // int4 leaf items (PP/PC) or int2 waves (CC/CP)
 	DeviceArray list;   // int32 source indices
+	DeviceArray owner;  // int32 partial slot per list entry (CC/CP)
 	int nitems = 0;
 	int64_t nlist = 0;
 };
@@ -36,27 +37,34 @@ struct Solver {
 	int ewald = 1;
 	double soft_len = 0.0, m = 0.0, theta = 0.7;
 	int parts_per_node = 64;
-	int node_chunk = 64;  // sources per CC/CP work item
+	bool uniform_mass = true;
+	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64)
 	Tree tree;
 	int64_t n = 0;
 	// device state
-	DeviceArray d_parts, d_multis, d_nodepos, d_parent, d_dL, d_L, d_out, d_partial, d_node_slot_ptr;
-	DeviceArray d_level_nodes, d_leaf_items;
+	DeviceArray d_parts, d_multis, d_nodepos, d_out, d_partial, d_node_slot_ptr;
+	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL;
 	KindWork work[NKINDS];
-	std::vector<int64_t> level_ptr;  // CSR of nodes by level (host)
 	int n_leaf_items = 0;
 	int64_t n_slots = 0;
 	int64_t list_bytes = 0;
 	// timings (seconds) of the last build and solve
 	double t_tree = 0, t_multipoles = 0, t_walk = 0, t_upload = 0;
+	// per-solve event sets: a ring of TIMER_RING solves, each 2 events per timed span (8 kinds,
+	// then the local-expansion pass); sets are folded into acc_ms before they are reused
+	static constexpr int TIMER_RING = 32, TIMER_SPANS = NKINDS + 1;
 	std::vector<hipEvent_t> ev;
+	int64_t solves_recorded = 0, solves_folded = 0;
+	std::vector<double> acc_ms = std::vector<double>(TIMER_SPANS, 0.0);
 	std::string err;
 
 	int init(int dev);
 	int set_particles(const int32_t* x, const float* mass, int64_t count);
 	int upload();
 	int enqueue_solve(bool timed);
-	int kernel_times(std::vector<float>& ms);
+	int kernel_times(std::vector<float>& ms);       // last timed solve
+	int collect_times(double* ms_sum, int64_t* n);  // sums over the timed solves since the last collect
+	int fold_set(int64_t solve);
 	int forces(double* out4);  // N x (phi, gx, gy, gz), input order
 	~Solver();
 };

@@ -40,11 +40,12 @@ def _bind():
     L.tg_solver_synchronize.argtypes = [vp]
     L.tg_solver_get_forces.argtypes = [vp, vp]
     L.tg_solver_kernel_ms.argtypes = [vp, vp]
+    L.tg_solver_collect_ms.argtypes = [vp, vp, vp]
     L.tg_solver_info.argtypes = [vp, C.POINTER(SolveInfo)]
     L.tg_solver_export.argtypes = [vp, C.c_char_p, vp, i64]
     L.tg_solver_export.restype = i64
     L.tg_solver_last_error.restype = C.c_char_p
-    for n in ("create", "destroy", "set_particles", "solve", "synchronize", "get_forces", "kernel_ms", "info"):
+    for n in ("create", "destroy", "set_particles", "solve", "synchronize", "get_forces", "kernel_ms", "collect_ms", "info"):
         getattr(L, "tg_solver_" + n).restype = ci
     _bound = True
     return L
@@ -104,6 +105,17 @@ class Solver:
         out = {k: float(ms[i]) for i, k in enumerate(KINDS)}
         out["local"] = float(ms[8])
         return out
+
+    def collect_ms(self):
+        """Mean device time (ms) per solve of each kind's kernels over every solve enqueued since
+        the previous call, and the number of those solves."""
+        ms = np.zeros(9, np.float64)
+        n = np.zeros(1, np.int64)
+        _check(lib().tg_solver_collect_ms(self._h, ptr(ms), ptr(n)), "tg_solver_collect_ms")
+        k = max(int(n[0]), 1)
+        out = {name: float(ms[i]) / k for i, name in enumerate(KINDS)}
+        out["local"] = float(ms[8]) / k
+        return out, int(n[0])
 
     def info(self):
         s = SolveInfo()
