@@ -56,7 +56,8 @@ def make_workload(name):
 
 def cpu_baseline(s, ewald, h, m, theta, leaf, target_s=15.0):
     """Time the reference's own compiled kernels (oracle/_ref, if built) -- else the oracle's C
-    restatement -- over a strided sample of the same interaction lists, on all host threads."""
+    restatement -- over a strided sample of the same interaction lists, on all host threads.
+    Returns the baseline record and, when the sample is the whole solve, its forces (input order)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
 
@@ -105,13 +106,20 @@ def cpu_baseline(s, ewald, h, m, theta, leaf, target_s=15.0):
     dt = time.time() - t0
     stride = max(1, min(64, math.ceil(stride * dt / target_s)))
     t0 = time.time()
-    run(stride)
+    forces_tree, _ = run(stride)
     dt = time.time() - t0
     cnt = sample_count(stride)
     return {"value": cnt / dt, "unit": "interactions/s", "cores": threads, "kind": kind,
             "sample": ("the whole solve" if stride == 1 else f"every {stride}th node's and leaf's interaction lists of "
                        "the same solve") + f" ({cnt} interactions, {dt:.2f} s wall on {threads} threads)",
-            "s_per_force_solve_extrapolated": dt * sum(s.info()["count"].values()) / max(cnt, 1)}
+            "s_per_force_solve_extrapolated": dt * sum(s.info()["count"].values()) / max(cnt, 1)}, (
+        _input_order(forces_tree, T["order"]) if stride == 1 else None)
+
+
+def _input_order(f_tree, order):
+    f = np.empty_like(f_tree)
+    f[order] = f_tree
+    return f
 
 
 def main():
@@ -123,11 +131,28 @@ def main():
     ap.add_argument("--theta", type=float, default=0.7)
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--accuracy-sample", type=int, default=10000,
+                    help="sinks checked against the FP64 direct sum (all of them if >= N)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run only a plain device-to-device copy of this many bytes (5 times): the reference "
+                         "point that scales the HBM byte counters of a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
               "decomposition is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_bytes > 0:
+        import torch
+
+        n = args.copy_bytes // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "copies": 5}), flush=True)
+        return
 
     from tigergrav_amd import ics
     from tigergrav_amd.solver import Solver
@@ -164,6 +189,21 @@ def main():
     roofline = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                 "flops_per_interaction": ALG_FLOPS[dom], "interactions": count[dom], "kernel_ms": kms[dom]}
+    # the ceilings this part sustains for the same instruction mix, measured now on this device
+    import ctypes as C
+    from tigergrav_amd._lib import lib
+
+    L = lib()
+    L.tg_calibrate.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    fma_tf, pair_rate = C.c_double(), C.c_double()
+    if L.tg_calibrate(0, 3, C.byref(fma_tf), C.byref(pair_rate)) == 0:
+        roofline["measured_fp32_fma_tflops"] = fma_tf.value
+        roofline["frac_of_measured_fma"] = achieved / fma_tf.value
+        if dom == "PP_direct":
+            rate = count[dom] / (kms[dom] / 1e3)
+            roofline["pp_pairs_per_s"] = rate
+            roofline["pp_pair_loop_ceiling_per_s"] = pair_rate.value
+            roofline["frac_of_pair_loop_ceiling"] = rate / pair_rate.value
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         try:
@@ -187,8 +227,24 @@ def main():
         "interactions": count, "kernel_ms": kms,
         "roofline": roofline,
     }
+    # force accuracy of the solve: FP64 direct sum (converged Ewald when periodic) on the GPU for a
+    # seeded sample of sinks, after the timed region
+    from tigergrav_amd.accuracy import direct_forces, error_stats
+
+    f = s.forces()
+    ns = min(N, args.accuracy_sample)
+    sinks = np.sort(np.random.default_rng(7).choice(N, ns, replace=False)) if ns < N else np.arange(N)
+    t0 = time.perf_counter()
+    fref = direct_forces(x, sinks, ewald, h, m)
+    acc = error_stats(f[sinks], fref)
+    acc.update({"reference": "FP64 direct sum over all N sources" + (", converged Ewald sum" if ewald else ""),
+                "sample": f"{ns} sinks (seed 7)", "s": time.perf_counter() - t0})
+    out["accuracy"] = acc
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
+        out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
+        if f_cpu is not None:
+            # the same solve done on the host by the CPU baseline: its error on the same sample
+            out["cpu_baseline"]["accuracy_rms"] = error_stats(f_cpu[sinks], fref)["rms"]
     print(json.dumps(out), flush=True)
 
 

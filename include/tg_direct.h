@@ -1,0 +1,30 @@
+/* tg_direct.h -- FP64 direct-summation forces, the accuracy reference of a force solve.
+ *
+ * Replaces the direct sum of the reference's solver test (src/main.cpp:137-164), whose result
+ * src/output.cpp:17-51 compares with the tree solve; here in FP64 on the GPU, for a sample of
+ * sinks against all particles. */
+#ifndef TG_DIRECT_H
+#define TG_DIRECT_H
+
+#include <stdint.h>
+
+#include "tg_gravity.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Potential and acceleration at particles sinks[0..ns) from all n particles x (int32 fixed point,
+ * 3 per particle, the reference's vect<pos_type>), masses mass[] (NULL: every mass p->m):
+ * spline-softened Newton with softening p->soft_len, plus the converged periodic (Ewald)
+ * correction when p->ewald.  The self pair contributes the spline's r = 0 value.  out[k] is the
+ * result for sinks[k]; FP64 throughout.  Returns TG_OK or a negative status. */
+int tg_direct_forces(const tg_params* p, const int32_t* x, const float* mass, int64_t n, const int64_t* sinks, int64_t ns,
+		tg_force* out);
+const char* tg_direct_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -55,7 +55,9 @@ int tg_solver_info(tg_solver* s, tg_solve_info* info);
 /* copy a named host array of the last build (tree order positions "pos", "order", "mass",
  * "leaves", per node "node_range" "node_child" "node_parent" "node_level" "node_x" "node_m"
  * "node_r" "node_boxid", and the lists "cc_d" "cp_d" "cc_e" "cp_e" "pc_d" "pp_d" "pc_e" "pp_e"
- * with their "_ptr" CSR offsets).  Returns the size in bytes; dst may be NULL to query it. */
+ * with their "_ptr" CSR offsets; for a host-only solver also "pp_d_items", the PP-direct work items
+ * (first sink, count | near-prefix length << 7, list begin, list end), and "pp_d_list", their
+ * flattened source indices).  Returns the size in bytes; dst may be NULL to query it. */
 int64_t tg_solver_export(tg_solver* s, const char* name, void* dst, int64_t capacity);
 const char* tg_solver_last_error(void);
 

@@ -117,7 +117,7 @@ int leaf_call(const tg_params* p, tg_force* f, const int32_t* x, int64_t ns, con
 	T->hitems.resize(ngroups);
 	for (int64_t g = 0; g < ngroups; g++) {
 		const int64_t b = g * 64;
-		T->hitems[g] = make_int4(int(b), int(std::min<int64_t>(64, ns - b)), 0, int(nj));
+		T->hitems[g] = make_int4(int(b), tg::pp_item_y(int(std::min<int64_t>(64, ns - b)), tg::PP_NEAR_ALL), 0, int(nj));
 	}
 	HIPCHK(T->sinks.ensure(ns * sizeof(int4)));
 	HIPCHK(T->items.ensure(ngroups * sizeof(int4)));

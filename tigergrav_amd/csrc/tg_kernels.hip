@@ -113,13 +113,15 @@ __device__ __forceinline__ double wave_xor(double v, int off) { return __shfl_xo
 
 // ---- PP direct -------------------------------------------------------------------------------
 // Spline-softened (GADGET-2 cubic spline, support h) potential and acceleration
-// (src/gravity.cpp:231-287).  Beyond h it is Newton's 1/r; the softened branch is taken only by
-// the lanes that need it.  The 64-source tile is stored phase-major in LDS, so every lane walks
-// consecutive entries with a wave-uniform trip count; a partial last tile is padded with
-// zero-weight entries.  With equal masses (UNIFORM) the mass multiplies the sums once at the end.
+// (src/gravity.cpp:231-287).  Beyond h it is Newton's 1/r.  A leaf item's source list starts with
+// the sources that may come within h of its sinks (the near prefix, PP_NEAR_LEN of the item);
+// tiles that reach into it run the per-lane softened branch, every later tile the straight-line
+// far field.  The 64-source tile is stored phase-major in LDS, so every lane walks consecutive
+// entries with a wave-uniform trip count; a partial last tile is padded with zero-weight entries.
+// With equal masses (UNIFORM) the mass multiplies the sums once at the end.
 template <bool USE_W, bool PHI>
-__device__ __forceinline__ void pp_pair(const int4& me, const int4& s, bool wrap_unused, float h2u, float hinvu, float hinv3u,
-		float& ax, float& ay, float& az, float& pp, float fx, float fy, float fz) {
+__device__ __forceinline__ void pp_pair(const int4& s, float h2u, float hinvu, float hinv3u, float& ax, float& ay, float& az,
+		float& pp, float fx, float fy, float fz) {
 	const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
 	const float w = __int_as_float(s.w);
 	if (r2 > h2u) {
@@ -154,11 +156,26 @@ __device__ __forceinline__ void pp_pair(const int4& me, const int4& s, bool wrap
 	}
 }
 
+// a pair known to be at least h apart
+template <bool USE_W, bool PHI>
+__device__ __forceinline__ void pp_far(const int4& s, float& ax, float& ay, float& az, float& pp, float fx, float fy, float fz) {
+	const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+	const float ri = __builtin_amdgcn_rsqf(r2);
+	const float wri = USE_W ? __int_as_float(s.w) * ri : ri;
+	const float wri3 = wri * ri * ri;
+	ax = fmaf(fx, wri3, ax);
+	ay = fmaf(fy, wri3, ay);
+	az = fmaf(fz, wri3, az);
+	if (PHI) pp += wri;
+}
+
 template <bool WRAP, bool PHI, bool UNIFORM>
 __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	const int4 it = A.items[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int ns = it.y;
+	const int ns = it.y & PP_COUNT_MASK;
+	const int near_len = int(unsigned(it.y) >> PP_NEAR_SHIFT);
+	const int near_end = near_len == PP_NEAR_ALL ? INT_MAX : it.z + near_len;
 	const int lg = ceil_log2_64(ns);
 	const int nsp = 1 << lg;           // sinks padded to a power of two
 	const int lgP = 6 - lg;            // phases P = 64 / nsp
@@ -172,29 +189,43 @@ __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	const int4* mine = tile + ph * nsp;
 	float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
 	double dax = 0.0, day = 0.0, daz = 0.0, dpp = 0.0;
+	// software pipeline: the next tile's source record (and the list index after it) are loaded
+	// while the current tile is evaluated
+	auto index_at = [&](int b) { return b + lane < it.w ? (A.list ? A.list[b + lane] : b + lane) : -1; };
+	auto source_at = [&](int j) {
+		int4 v = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);  // zero weight
+		if (j >= 0) {
+			v = A.srcs[j];
+			if (UNIFORM) v.w = __float_as_int(1.0f);
+		}
+		return v;
+	};
+	int4 next = source_at(index_at(it.z));
+	int next_idx = index_at(it.z + 64);
 	for (int base = it.z; base < it.w; base += 64) {
 		const int n = min(64, it.w - base);
 		__syncthreads();
-		int4 v = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);  // zero weight
-		if (lane < n) {
-			v = A.srcs[A.list ? A.list[base + lane] : base + lane];
-			if (UNIFORM) v.w = __float_as_int(1.0f);
-		}
-		tile[dst] = v;
+		tile[dst] = next;
 		__syncthreads();
-		if (n == 64) {
-#pragma unroll 4
-			for (int j = 0; j < nsp; j++) {
-				const int4 s = mine[j];
-				pp_pair<!UNIFORM, PHI>(me, s, WRAP, A.h2u, A.hinvu, A.hinv3u, ax, ay, az, pp, sepu<WRAP>(me.x, s.x),
-						sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
-			}
-		} else {
-			const int trip = (n + P - 1) >> lgP;
+		next = source_at(next_idx);
+		next_idx = index_at(base + 128);
+		const int trip = (n + P - 1) >> lgP;
+		if (base < near_end) {
 			for (int j = 0; j < trip; j++) {
 				const int4 s = mine[j];
-				pp_pair<true, PHI>(me, s, WRAP, A.h2u, A.hinvu, A.hinv3u, ax, ay, az, pp, sepu<WRAP>(me.x, s.x),
-						sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
+				pp_pair<true, PHI>(s, A.h2u, A.hinvu, A.hinv3u, ax, ay, az, pp, sepu<WRAP>(me.x, s.x), sepu<WRAP>(me.y, s.y),
+						sepu<WRAP>(me.z, s.z));
+			}
+		} else if (n == 64) {
+#pragma unroll 8
+			for (int j = 0; j < nsp; j++) {
+				const int4 s = mine[j];
+				pp_far<!UNIFORM, PHI>(s, ax, ay, az, pp, sepu<WRAP>(me.x, s.x), sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
+			}
+		} else {
+			for (int j = 0; j < trip; j++) {
+				const int4 s = mine[j];
+				pp_far<true, PHI>(s, ax, ay, az, pp, sepu<WRAP>(me.x, s.x), sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
 			}
 		}
 		dax += ax;
@@ -240,7 +271,7 @@ template <bool EWALD, bool WRAP>
 __global__ __launch_bounds__(64) void k_pc(LeafArgs A) {
 	const int4 it = A.items[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int ns = it.y;
+	const int ns = it.y & PP_COUNT_MASK;
 	const int lg = ceil_log2_64(ns);
 	const int nsp = 1 << lg;
 	const int P = 64 >> lg;
@@ -315,7 +346,7 @@ hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s) {
 __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 	const int4 it = A.items[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int ns = it.y;
+	const int ns = it.y & PP_COUNT_MASK;
 	const int lg = ceil_log2_64(ns);
 	const int nsp = 1 << lg;
 	const int P = 64 >> lg;

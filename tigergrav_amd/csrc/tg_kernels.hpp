@@ -2,6 +2,9 @@
 //
 // Work is described by int4 items.  Leaf kinds (PP/PC: sinks are particles):
 //   item = (first sink, sink count <= 64, list begin, list end)
+// The count field of every leaf item is read through PP_COUNT_MASK; for PP direct it also carries the length of the list's near prefix (sources that
+// may lie within the softening length): y = count | near_len << PP_NEAR_SHIFT, with
+// near_len == PP_NEAR_ALL meaning the whole list.
 // and results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
 // atomics are needed while kernels of one stream run in order).
 // Node kinds (CC/CP: the sink is an expansion centre): a wave takes a run of <= 64 consecutive list
@@ -16,6 +19,13 @@
 #include <stdint.h>
 
 namespace tg {
+
+constexpr int PP_COUNT_MASK = 127;
+constexpr int PP_NEAR_SHIFT = 7;
+constexpr int PP_NEAR_ALL = (1 << 25) - 1;  // all 25 upper bits set
+__host__ __device__ constexpr int pp_item_y(int count, int64_t near_len) {
+	return int(unsigned(count) | (unsigned(near_len >= PP_NEAR_ALL ? PP_NEAR_ALL : near_len) << PP_NEAR_SHIFT));
+}
 
 struct MultiSrc {
 	float m[17];

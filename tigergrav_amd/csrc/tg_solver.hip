@@ -189,7 +189,6 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	t_tree = t1 - t0;
 	t_multipoles = t2 - t1;
 	t_walk = t3 - t2;
-	if (device < 0) return TG_OK;
 	int rc = upload();
 	t_upload = now() - t3;
 	return rc;
@@ -204,7 +203,7 @@ static hipError_t put(DeviceArray& d, const std::vector<T>& h, hipStream_t s) {
 }
 
 int Solver::upload() {
-	SCHK(hipSetDevice(device));
+	if (device >= 0) SCHK(hipSetDevice(device));
 	const Tree& T = tree;
 	const int64_t nn = int64_t(T.nodes.size());
 	// particles (x, y, z, mass bits), tree order
@@ -281,7 +280,46 @@ int Solver::upload() {
 		if (open) waves[k].push_back(cur);
 	}
 	n_slots = slot;
-	// leaf kinds: items (first sink, count <= 64, lb, le)
+	// leaf kinds: items (first sink, count <= 64, lb, le).  A PP-direct list is reordered so the
+	// source ranges whose bounding box comes within the softening length of the sink leaf's box
+	// (the leaf itself among them) come first; the item records that near prefix's length.
+	std::vector<int32_t> leaf_at(size_t(n) + 1, -1);
+	for (int32_t lf : T.leaves)
+		if (!T.nodes[lf].empty()) leaf_at[T.nodes[lf].pbegin] = lf;
+	const double hnear = soft_len * 1.001 + 1e-9;  // margin over the kernel's single-precision test
+	auto range_box = [&](int32_t b, int32_t e, double* mn, double* mx) {
+		const int32_t lf = leaf_at[b];
+		if (lf >= 0 && T.nodes[lf].pend == e) {
+			for (int d = 0; d < 3; d++) {
+				mn[d] = T.nodes[lf].rmin[d];
+				mx[d] = T.nodes[lf].rmax[d];
+			}
+			return;
+		}
+		for (int d = 0; d < 3; d++) {
+			mn[d] = 1.0;
+			mx[d] = 0.0;
+		}
+		for (int32_t i = b; i < e; i++)
+			for (int d = 0; d < 3; d++) {
+				const double x = pos_to_double(T.pos[3 * size_t(i) + d]);
+				mn[d] = std::min(mn[d], x);
+				mx[d] = std::max(mx[d], x);
+			}
+	};
+	auto may_be_near = [&](const Node& sk, const double* mn, const double* mx) {
+		double d2 = 0.0;
+		for (int d = 0; d < 3; d++) {
+			const double h1 = 0.5 * (sk.rmax[d] - sk.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
+			if (ewald && h1 + h2 > 0.25) return true;  // minimum image of the centres is not decisive
+			double dc = 0.5 * (sk.rmax[d] + sk.rmin[d]) - 0.5 * (mx[d] + mn[d]);
+			if (ewald) dc -= std::nearbyint(dc);
+			const double gap = std::fabs(dc) - h1 - h2;
+			if (gap > 0.0) d2 += gap * gap;
+		}
+		return d2 <= hnear * hnear;
+	};
+	std::vector<int32_t> far_ranges;
 	std::vector<int4> leaf_items;
 	for (int32_t lf : T.leaves) {
 		const Node& nd = T.nodes[lf];
@@ -297,21 +335,42 @@ int Solver::upload() {
 		for (auto& s : src) {
 			std::vector<int>& L = lists[s.kind];
 			const int64_t lb = int64_t(L.size());
-			if (s.ranges)
+			int64_t near_len = 0;
+			if (s.kind == PP_DIRECT) {
+				far_ranges.clear();
+				for (int64_t k = (*s.ptr)[lf]; k < (*s.ptr)[lf + 1]; k += 2) {
+					const int32_t b = (*s.v)[k], e = (*s.v)[k + 1];
+					double mn[3], mx[3];
+					range_box(b, e, mn, mx);
+					if (may_be_near(nd, mn, mx)) {
+						for (int32_t q = b; q < e; q++) L.push_back(q);
+					} else {
+						far_ranges.push_back(b);
+						far_ranges.push_back(e);
+					}
+				}
+				near_len = int64_t(L.size()) - lb;
+				expand(L, far_ranges, 0, int64_t(far_ranges.size()));
+			} else if (s.ranges) {
 				expand(L, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
-			else
+			} else {
 				L.insert(L.end(), s.v->begin() + (*s.ptr)[lf], s.v->begin() + (*s.ptr)[lf + 1]);
+			}
 			const int64_t le = int64_t(L.size());
 			if (le == lb) continue;
-			for (int32_t b = nd.pbegin; b < nd.pend; b += 64)
-				items[s.kind].push_back(make_int4(b, std::min<int32_t>(64, nd.pend - b), int(lb), int(le)));
+			for (int32_t b = nd.pbegin; b < nd.pend; b += 64) {
+				const int cnt = std::min<int32_t>(64, nd.pend - b);
+				const int y = s.kind == PP_DIRECT ? pp_item_y(cnt, near_len) : cnt;
+				items[s.kind].push_back(make_int4(b, y, int(lb), int(le)));
+			}
 		}
 	}
 	// heaviest leaf items first: the dispatcher hands out work groups in order
 	for (int k : {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD}) {
 		std::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
-			const int64_t ca = int64_t(a.w - a.z) * (1 << (a.y <= 1 ? 0 : 32 - __builtin_clz(a.y - 1)));
-			const int64_t cb = int64_t(b.w - b.z) * (1 << (b.y <= 1 ? 0 : 32 - __builtin_clz(b.y - 1)));
+			const int na = a.y & PP_COUNT_MASK, nb = b.y & PP_COUNT_MASK;
+			const int64_t ca = int64_t(a.w - a.z) * (1 << (na <= 1 ? 0 : 32 - __builtin_clz(na - 1)));
+			const int64_t cb = int64_t(b.w - b.z) * (1 << (nb <= 1 ? 0 : 32 - __builtin_clz(nb - 1)));
 			return ca > cb;
 		});
 	}
@@ -328,6 +387,11 @@ int Solver::upload() {
 		std::vector<int> up;
 		for (int a = li.x; a >= 0; a = T.nodes[a].parent) up.push_back(a);
 		path.insert(path.end(), up.rbegin(), up.rend());
+	}
+	if (device < 0) {  // host-only: keep the PP-direct work for inspection
+		host_pp_items = std::move(items[PP_DIRECT]);
+		host_pp_list = std::move(lists[PP_DIRECT]);
+		return TG_OK;
 	}
 	hipStream_t s = stream;
 	SCHK(put(d_parts, parts, s));
@@ -636,6 +700,8 @@ int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capa
 	TG_EXPORT_LIST(pp_d)
 	TG_EXPORT_LIST(pc_e)
 	TG_EXPORT_LIST(pp_e)
+	else if (nm == "pp_d_items") put(h->s.host_pp_items.data(), h->s.host_pp_items.size() * sizeof(int4));
+	else if (nm == "pp_d_list") put(h->s.host_pp_list.data(), h->s.host_pp_list.size() * 4);
 #undef TG_EXPORT_LIST
 	else return TG_EINVAL;
 	if (dst) {

@@ -45,6 +45,9 @@ struct Solver {
 	DeviceArray d_parts, d_multis, d_nodepos, d_out, d_partial, d_node_slot_ptr;
 	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL;
 	KindWork work[NKINDS];
+	// host-only solvers (device < 0) keep the PP-direct items and flattened source list
+	std::vector<int4> host_pp_items;
+	std::vector<int> host_pp_list;
 	int n_leaf_items = 0;
 	int64_t n_slots = 0;
 	int64_t list_bytes = 0;
