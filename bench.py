@@ -46,6 +46,10 @@ def make_workload(name):
         return ics.zeldovich(128, seed=3, n_s=-2.0, rms_cells=0.3), True, (
             "zeldovich 128^3 = 2,097,152 particles, P(k)~k^-2 cut at grid Nyquist, rms displacement 0.3 cell, "
             "seed 3, periodic Ewald")
+    if name == "zeldovich256":
+        return ics.zeldovich(256, seed=5, n_s=-1.5, rms_cells=0.5), True, (
+            "zeldovich 256^3 = 16,777,216 particles, P(k)~k^-1.5 cut at grid Nyquist, rms displacement 0.5 cell, "
+            "seed 5, periodic Ewald")
     if name == "plummer2M":
         return ics.plummer_spheres(2 ** 21, seed=2), False, (
             "8 Plummer spheres (a = 0.02) + 20% uniform background, 2,097,152 particles, seed 2, open boundaries")
@@ -129,7 +133,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="zeldovich128")
     ap.add_argument("--theta", type=float, default=0.7)
-    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--leaf", type=int, default=64,
+                    help="particles per leaf; 0 autotunes leaf (32, 64, 128), CC/CP chunk and the PP kernel form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--accuracy-sample", type=int, default=10000,
                     help="sinks checked against the FP64 direct sum (all of them if >= N)")
@@ -160,7 +165,18 @@ def main():
     x, ewald, desc = make_workload(args.workload)
     N = x.shape[0]
     h, m = ics.soft_len(N), 1.0 / N
+    tuning, best = None, None
+    if args.leaf == 0:
+        # autotune leaf size, CC/CP chunk and PP kernel form on this very workload's device time
+        from tigergrav_amd.autotune import autotune
+
+        best, tuning = autotune(x, ewald, h, m, args.theta)
+        args.leaf = best["leaf"]
     s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
+    if best is not None:
+        from tigergrav_amd.autotune import apply
+
+        apply(s, best)
     t0 = time.time()
     s.set_particles(x)
     t_host = time.time() - t0
@@ -225,6 +241,7 @@ def main():
         "host_build_s": {"tree": info["t_tree"], "multipoles": info["t_multipoles"], "lists": info["t_walk"],
                          "upload": info["t_upload"]},
         "interactions": count, "kernel_ms": kms,
+        "tuning": {"best": best, "table": tuning} if tuning else None,
         "roofline": roofline,
     }
     # force accuracy of the solve: FP64 direct sum (converged Ewald when periodic) on the GPU for a

@@ -42,6 +42,13 @@ int tg_solver_destroy(tg_solver* s);
 int tg_solver_set_particles(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
 /* enqueue one force solve on the solver's stream (replaces tree::kick_fmm's kernel calls) */
 int tg_solver_solve(tg_solver* s);
+/* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent
+ * queue-driven kernel; "node_chunk" 1..64 sources per CC/CP partial slot (applies from the next
+ * tg_solver_set_particles); "resident" work groups of the persistent form */
+int tg_solver_tune(tg_solver* s, const char* knob, int value);
+/* the same solve replayed from a HIP graph captured on its first call after set_particles (one
+ * submission instead of nine launches; records no per-kind timings) */
+int tg_solver_solve_graph(tg_solver* s);
 int tg_solver_synchronize(tg_solver* s);
 /* forces of the last solve, in input particle order (the reference's `force` per particle) */
 int tg_solver_get_forces(tg_solver* s, tg_force* f);

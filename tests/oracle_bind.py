@@ -158,6 +158,11 @@ def ewald_corr64(dx):
 REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libtgref.so")
 
 
+# the same reference driver with its gravity_* calls bound to the GPU kernels through the
+# reference-signature shim (tigergrav_amd/integration/gravity_hip.cpp): the plain drop-in
+REF_HIP_LIB = os.path.join(ORACLE_DIR, "_ref", "libtgref_hip.so")
+
+
 def ref_available():
     return os.path.exists(REF_LIB)
 

@@ -24,19 +24,21 @@ namespace tg {
 __constant__ float c_real[TG_NREAL * 3];
 __constant__ float c_four[TG_NFOUR * 38];
 __constant__ double c_ser[5 * TG_NSER];
+__constant__ float c_four_f0;  // the Fourier part of the correction at x = 0: sum of the D0 coefficients
 
 // ---- per-device lattice tables -------------------------------------------------------------
 static void build_tables(std::vector<float>& real, std::vector<float>& four, std::vector<double>& ser) {
 	real.clear();
-	for (int i = -4; i <= 4; i++)
-		for (int j = -4; j <= 4; j++)
-			for (int k = -4; k <= 4; k++) {
-				const int n2 = i * i + j * j + k * k;
-				if (n2 == 0 || n2 > 11) continue;
-				real.push_back(float(i));
-				real.push_back(float(j));
-				real.push_back(float(k));
-			}
+	// ordered by |n|^2, so a prefix of the table holds every image within a given distance
+	for (int n2want = 1; n2want <= 11; n2want++)
+		for (int i = -4; i <= 4; i++)
+			for (int j = -4; j <= 4; j++)
+				for (int k = -4; k <= 4; k++) {
+					if (i * i + j * j + k * k != n2want) continue;
+					real.push_back(float(i));
+					real.push_back(float(j));
+					real.push_back(float(k));
+				}
 	four.clear();
 	for (int i = -3; i <= 3; i++)
 		for (int j = -3; j <= 3; j++)
@@ -75,6 +77,17 @@ static void build_tables(std::vector<float>& real, std::vector<float>& four, std
 		}
 }
 
+int ewald_images_within(double radius) {
+	int n = 0;
+	for (int i = -4; i <= 4; i++)
+		for (int j = -4; j <= 4; j++)
+			for (int k = -4; k <= 4; k++) {
+				const int n2 = i * i + j * j + k * k;
+				if (n2 > 0 && n2 <= 11 && double(n2) < radius * radius) n++;
+			}
+	return n;
+}
+
 hipError_t init_device_tables(int device) {
 	static std::mutex mtx;
 	static std::vector<int> done;
@@ -90,6 +103,9 @@ hipError_t init_device_tables(int device) {
 	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;
 	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_four), four.data(), four.size() * sizeof(float))) != hipSuccess) return e;
 	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ser), ser.data(), ser.size() * sizeof(double))) != hipSuccess) return e;
+	float f0 = 0.0f;
+	for (int i = 0; i < TG_NFOUR; i++) f0 += four[38 * i + 3];
+	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_four_f0), &f0, sizeof(f0))) != hipSuccess) return e;
 	done[device] = 1;
 	return hipSuccess;
 }
@@ -169,9 +185,9 @@ __device__ __forceinline__ void pp_far(const int4& s, float& ax, float& ay, floa
 	if (PHI) pp += wri;
 }
 
+// one PP item (<= 64 sinks of a leaf against its source list), by one 64-lane work group
 template <bool WRAP, bool PHI, bool UNIFORM>
-__global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
-	const int4 it = A.items[blockIdx.x];
+__device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* tile) {
 	const int lane = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
 	const int near_len = int(unsigned(it.y) >> PP_NEAR_SHIFT);
@@ -183,7 +199,6 @@ __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	const int si = lane & (nsp - 1);
 	const int ph = lane >> lg;
 	const int4 me = A.sinks[it.x + min(si, ns - 1)];
-	__shared__ int4 tile[64];
 	// source l of a tile goes to phase l % P, position l / P
 	const int dst = (lane & (P - 1)) * nsp + (lane >> lgP);
 	const int4* mine = tile + ph * nsp;
@@ -251,8 +266,51 @@ __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	}
 }
 
+// one work group per item; the dispatcher hands items out in array order (heaviest first)
+template <bool WRAP, bool PHI, bool UNIFORM>
+__global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
+	__shared__ int4 tile[64];
+	pp_item<WRAP, PHI, UNIFORM>(A, A.items[blockIdx.x], tile);
+}
+
+// persistent form: as many work groups as the machine keeps resident, each taking the next
+// item from a queue counter (A.queue, zero at launch) until the items run out
+template <bool WRAP, bool PHI, bool UNIFORM>
+__global__ __launch_bounds__(64) void k_pp_direct_queue(LeafArgs A) {
+	__shared__ int4 tile[64];
+	__shared__ int next;
+	for (;;) {
+		__syncthreads();
+		if (threadIdx.x == 0) next = atomicAdd(A.queue, 1);
+		__syncthreads();
+		const int k = next;
+		if (k >= A.nitems) break;
+		pp_item<WRAP, PHI, UNIFORM>(A, A.items[k], tile);
+	}
+}
+
+__global__ void k_zero_counter(int* c) {
+	if (threadIdx.x == 0) c[0] = 0;
+}
+
 hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s) {
 	if (a.nitems <= 0) return hipSuccess;
+	if (a.queue) {
+		// the counter is cleared by a kernel (not a memset) so the launch sequence stays graph-safe
+		hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, a.queue);
+		const int groups = a.nitems < a.resident ? a.nitems : a.resident;
+#define TG_PPQ(W, PH, U) hipLaunchKernelGGL((k_pp_direct_queue<W, PH, U>), dim3(groups), dim3(64), 0, s, a)
+		const bool u = a.uniform != 0;
+		if (a.wrap) {
+			if (do_phi) { if (u) TG_PPQ(true, true, true); else TG_PPQ(true, true, false); }
+			else { if (u) TG_PPQ(true, false, true); else TG_PPQ(true, false, false); }
+		} else {
+			if (do_phi) { if (u) TG_PPQ(false, true, true); else TG_PPQ(false, true, false); }
+			else { if (u) TG_PPQ(false, false, true); else TG_PPQ(false, false, false); }
+		}
+#undef TG_PPQ
+		return hipGetLastError();
+	}
 #define TG_PP(W, PH, U) hipLaunchKernelGGL((k_pp_direct<W, PH, U>), dim3(a.nitems), dim3(64), 0, s, a)
 	const bool u = a.uniform != 0;
 	if (a.wrap) {
@@ -343,10 +401,45 @@ hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s) {
 }
 
 // ---- PP Ewald correction -------------------------------------------------------------------------
+// The periodic correction between particle pairs (src/gravity.cpp:491-614), per item in three parts:
+//  1. structure factors of the item's sources for the 61 half-space Fourier vectors (lane = h):
+//     C_h = sum_j w_j cos(2 pi h.x_j), S_h = sum_j w_j sin(2 pi h.x_j), with the phase h.X taken
+//     exactly in 32-bit integer arithmetic (positions are 2^-32 fixed point) and sin/cos in
+//     revolutions;
+//  2. per pair the real-space part: the n = 0 term analytically and the images of the |n|-sorted
+//     table that can lie within 2.6 of any pair of the item (the item's image count), + pi/4;
+//  3. per sink the Fourier part sum_h c_h (cos_i C_h + sin_i S_h), which equals the per-pair sums
+//     sum_j w_j c_h cos(2 pi h.(x_i - x_j)) of the reference.
+// Pairs closer than 1e-6 take the reference's constant 2.8372975 and zero force (its zmask); their
+// share of the structure-factor sum (c_h per unit weight at coincidence) is taken back out.
+__device__ __forceinline__ void green_ewald1_real(float& D0, float* D1, const float* x, int nimg) {
+	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+	float d[2];
+	ewald_self_factors<1>(double(r2), d);
+	float a0 = d[0];
+	float a1[3] = {x[0] * d[1], x[1] * d[1], x[2] * d[1]};
+#pragma unroll 4
+	for (int i = 0; i < nimg; i++) {
+		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
+		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
+		float e[2];
+		ewald_real_factors<1>(q2, e);
+		a0 += e[0];
+#pragma unroll
+		for (int a = 0; a < 3; a++) a1[a] = fmaf(dx[a], e[1], a1[a]);
+	}
+	a0 += 0.78539816339744831f;
+	const bool z = !(r2 > 1e-12f);
+	D0 = z ? 2.8372975f - c_four_f0 : a0;
+#pragma unroll
+	for (int a = 0; a < 3; a++) D1[a] = z ? 0.0f : a1[a];
+}
+
 __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 	const int4 it = A.items[blockIdx.x];
 	const int lane = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
+	const int nimg = min(TG_NREAL, int(unsigned(it.y) >> PP_NEAR_SHIFT));
 	const int lg = ceil_log2_64(ns);
 	const int nsp = 1 << lg;
 	const int P = 64 >> lg;
@@ -354,19 +447,35 @@ __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 	const int ph = lane >> lg;
 	const int4 me = A.sinks[it.x + min(si, ns - 1)];
 	__shared__ int4 tile[64];
+	__shared__ float sfc[64], sfs[64];
+	constexpr float REV = 2.3283064365386963e-10f;  // 2^-32
+	// 1. structure factors, lane = Fourier vector
+	const int hx = lane < TG_NFOUR ? int(c_four[38 * lane]) : 0;
+	const int hy = lane < TG_NFOUR ? int(c_four[38 * lane + 1]) : 0;
+	const int hz = lane < TG_NFOUR ? int(c_four[38 * lane + 2]) : 0;
+	float C = 0.f, S = 0.f;
 	double dphi = 0.0, dg[3] = {0.0, 0.0, 0.0};
 	for (int base = it.z; base < it.w; base += 64) {
 		const int n = min(64, it.w - base);
 		__syncthreads();
 		if (lane < n) tile[lane] = A.srcs[A.list ? A.list[base + lane] : base + lane];
 		__syncthreads();
+		for (int k = 0; k < n; k++) {
+			const int4 s = tile[k];
+			const unsigned phs = unsigned(hx) * unsigned(s.x) + unsigned(hy) * unsigned(s.y) + unsigned(hz) * unsigned(s.z);
+			const float t = float(int(phs)) * REV;
+			const float w = __int_as_float(s.w);
+			C = fmaf(w, __builtin_amdgcn_cosf(t), C);
+			S = fmaf(w, __builtin_amdgcn_sinf(t), S);
+		}
+		// 2. real-space part of every pair of this tile
 		float phi = 0.f, g[3] = {0.f, 0.f, 0.f};
 		for (int k = ph; k < n; k += P) {
 			const int4 s = tile[k];
 			const float x[3] = {sepu<true>(me.x, s.x) * POS_INVF, sepu<true>(me.y, s.y) * POS_INVF,
 					sepu<true>(me.z, s.z) * POS_INVF};
 			float D0, D1[3];
-			green_ewald1(D0, D1, x);
+			green_ewald1_real(D0, D1, x, nimg);
 			const float w = __int_as_float(s.w);
 			phi = fmaf(w, D0, phi);
 			g[0] = fmaf(-w, D1[0], g[0]);
@@ -378,6 +487,9 @@ __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 		dg[1] += g[1];
 		dg[2] += g[2];
 	}
+	sfc[lane] = C;
+	sfs[lane] = S;
+	__syncthreads();
 	for (int off = nsp; off < 64; off <<= 1) {
 		dphi += wave_xor(dphi, off);
 		dg[0] += wave_xor(dg[0], off);
@@ -385,11 +497,26 @@ __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 		dg[2] += wave_xor(dg[2], off);
 	}
 	if (ph == 0 && si < ns) {
+		// 3. Fourier part at this sink
+		float fphi = 0.f, fg[3] = {0.f, 0.f, 0.f};
+		for (int h = 0; h < TG_NFOUR; h++) {
+			const float* H = c_four + 38 * h;
+			const unsigned phs = unsigned(int(H[0])) * unsigned(me.x) + unsigned(int(H[1])) * unsigned(me.y) +
+			                     unsigned(int(H[2])) * unsigned(me.z);
+			const float t = float(int(phs)) * REV;
+			const float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+			const float Ch = sfc[h], Sh = sfs[h];
+			fphi = fmaf(H[3], fmaf(cs, Ch, sn * Sh), fphi);
+			const float q = fmaf(sn, Ch, -cs * Sh);
+			fg[0] = fmaf(-H[4], q, fg[0]);
+			fg[1] = fmaf(-H[5], q, fg[1]);
+			fg[2] = fmaf(-H[6], q, fg[2]);
+		}
 		double* o = A.out + 4 * (size_t) (it.x + si);
-		o[0] += dphi;
-		o[1] += dg[0];
-		o[2] += dg[1];
-		o[3] += dg[2];
+		o[0] += dphi + double(fphi);
+		o[1] += dg[0] + double(fg[0]);
+		o[2] += dg[1] + double(fg[1]);
+		o[3] += dg[2] + double(fg[2]);
 	}
 }
 

@@ -2,7 +2,8 @@
 //
 // Work is described by int4 items.  Leaf kinds (PP/PC: sinks are particles):
 //   item = (first sink, sink count <= 64, list begin, list end)
-// The count field of every leaf item is read through PP_COUNT_MASK; for PP direct it also carries the length of the list's near prefix (sources that
+// The count field of every leaf item is read through PP_COUNT_MASK; for PP Ewald the upper bits
+// carry the number of real-space images the item needs (PP_NEAR_ALL: all); for PP direct they carry the length of the list's near prefix (sources that
 // may lie within the softening length): y = count | near_len << PP_NEAR_SHIFT, with
 // near_len == PP_NEAR_ALL meaning the whole list.
 // and results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
@@ -44,6 +45,8 @@ struct LeafArgs {
 	int uniform;             // all source masses equal to m (PP direct sums once, scales at the end)
 	float m;                 // the common mass when uniform
 	float h2u, hinvu, hinv3u;  // softening in position units (2^32 per box length)
+	int* queue;              // PP direct: work counter of the persistent form (NULL: one group per item)
+	int resident;            // PP direct persistent form: work groups to launch
 };
 
 struct NodeArgs {
@@ -64,6 +67,9 @@ enum Kind { PP_DIRECT = 0, PC_DIRECT, CC_DIRECT, CP_DIRECT, PP_EWALD, PC_EWALD, 
 // one-time per-device setup of the Ewald lattice tables; returns hipSuccess or the error
 hipError_t init_device_tables(int device);
 
+// number of real-space Ewald images (n != 0, |n|^2 <= 11) with |n| < radius: the prefix of the
+// |n|-sorted table a PP Ewald item needs when no pair of it is farther apart than radius - 2.6
+int ewald_images_within(double radius);
 hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s);
 hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s);
 hipError_t launch_pp_ewald(const LeafArgs& a, hipStream_t s);

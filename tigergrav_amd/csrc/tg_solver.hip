@@ -140,8 +140,42 @@ int Solver::init(int dev) {
 	SCHK(hipSetDevice(dev));
 	SCHK(init_device_tables(dev));
 	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+	SCHK(d_queue.alloc(256));
+	int cus = 0;
+	SCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+	resident_groups = cus * 4 * 6;  // six 64-lane work groups per SIMD
 	ev.resize(size_t(2) * TIMER_SPANS * TIMER_RING);
 	for (auto& e : ev) SCHK(hipEventCreate(&e));
+	return TG_OK;
+}
+
+void Solver::drop_graph() {
+	if (graph_exec) {
+		(void) hipGraphExecDestroy(graph_exec);
+		graph_exec = nullptr;
+	}
+}
+
+// The solve's launches captured once into a HIP graph (no timing events inside) and replayed:
+// one submission per solve instead of nine.  The capture holds no memset nodes: every buffer a
+// solve reads is written earlier in the same solve.
+int Solver::solve_graph() {
+	SCHK(hipSetDevice(device));
+	if (!graph_exec) {
+		SCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+		const int rc = enqueue_solve(false);
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(stream, &g);
+		if (rc != TG_OK) {
+			if (g) (void) hipGraphDestroy(g);
+			return rc;
+		}
+		SCHK(e);
+		const hipError_t ei = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+		(void) hipGraphDestroy(g);
+		SCHK(ei);
+	}
+	SCHK(hipGraphLaunch(graph_exec, stream));
 	return TG_OK;
 }
 
@@ -149,6 +183,7 @@ Solver::~Solver() {
 	if (stream) {
 		hipSetDevice(device);
 		hipStreamSynchronize(stream);
+		drop_graph();
 		for (auto& e : ev)
 			if (e) hipEventDestroy(e);
 		d_parts.release();
@@ -158,6 +193,7 @@ Solver::~Solver() {
 		d_partial.release();
 		d_node_slot_ptr.release();
 		d_dL.release();
+		d_queue.release();
 		d_leaf_items.release();
 		d_slot_node.release();
 		d_path.release();
@@ -172,6 +208,7 @@ Solver::~Solver() {
 }
 
 int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
+	drop_graph();  // the captured launches point at buffers that may be reallocated
 	n = count;
 	TreeParams P;
 	P.ewald = ewald;
@@ -351,6 +388,27 @@ int Solver::upload() {
 				}
 				near_len = int64_t(L.size()) - lb;
 				expand(L, far_ranges, 0, int64_t(far_ranges.size()));
+			} else if (s.kind == PP_EWALD) {
+				// the real-space images this leaf's pairs can need: |n| < 2.6 + the largest separation
+				double dmax2 = 0.0;
+				bool all = false;
+				for (int64_t k = (*s.ptr)[lf]; k < (*s.ptr)[lf + 1]; k += 2) {
+					const int32_t b = (*s.v)[k], e = (*s.v)[k + 1];
+					double mn[3], mx[3];
+					range_box(b, e, mn, mx);
+					double d2 = 0.0;
+					for (int d = 0; d < 3; d++) {
+						const double h1 = 0.5 * (nd.rmax[d] - nd.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
+						double dc = 0.5 * (nd.rmax[d] + nd.rmin[d]) - 0.5 * (mx[d] + mn[d]);
+						dc -= std::nearbyint(dc);
+						if (h1 + h2 > 0.25) all = true;
+						const double span = std::fabs(dc) + h1 + h2;
+						d2 += span * span;
+					}
+					dmax2 = std::max(dmax2, d2);
+				}
+				expand(L, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
+				near_len = all ? int64_t(PP_NEAR_ALL) : int64_t(ewald_images_within(2.6 + std::sqrt(dmax2) + 1e-6));
 			} else if (s.ranges) {
 				expand(L, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
 			} else {
@@ -360,7 +418,7 @@ int Solver::upload() {
 			if (le == lb) continue;
 			for (int32_t b = nd.pbegin; b < nd.pend; b += 64) {
 				const int cnt = std::min<int32_t>(64, nd.pend - b);
-				const int y = s.kind == PP_DIRECT ? pp_item_y(cnt, near_len) : cnt;
+				const int y = (s.kind == PP_DIRECT || s.kind == PP_EWALD) ? pp_item_y(cnt, near_len) : cnt;
 				items[s.kind].push_back(make_int4(b, y, int(lb), int(le)));
 			}
 		}
@@ -488,6 +546,8 @@ int Solver::enqueue_solve(bool timed) {
 	la.hinv3u = float(1.0 / (hu * hu * hu));
 	const int leaf_kinds[4] = {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD};
 	for (int k : leaf_kinds) {
+		la.queue = (k == PP_DIRECT && pp_persistent) ? d_queue.as<int>() : nullptr;
+		la.resident = resident_groups;
 		la.items = work[k].items.as<int4>();
 		la.list = work[k].list.as<int>();
 		la.nitems = work[k].nitems;
@@ -598,6 +658,31 @@ int tg_solver_set_particles(tg_solver* h, const int32_t* x, const float* mass, i
 int tg_solver_solve(tg_solver* h) {
 	if (!h || h->s.device < 0) return TG_EINVAL;
 	return fail(h, h->s.enqueue_solve(true));
+}
+
+int tg_solver_tune(tg_solver* h, const char* knob, int value) {
+	if (!h || !knob) return TG_EINVAL;
+	const std::string k(knob);
+	tg::Solver& S = h->s;
+	if (k == "pp_persistent") {
+		S.pp_persistent = value != 0;
+	} else if (k == "node_chunk") {
+		if (value < 1 || value > 64) return TG_EINVAL;
+		S.node_chunk = value;
+	} else if (k == "resident") {
+		if (value < 1) return TG_EINVAL;
+		S.resident_groups = value;
+	} else {
+		g_solver_err = "unknown tuning knob " + k;
+		return TG_EINVAL;
+	}
+	S.drop_graph();
+	return TG_OK;
+}
+
+int tg_solver_solve_graph(tg_solver* h) {
+	if (!h || h->s.device < 0) return TG_EINVAL;
+	return fail(h, h->s.solve_graph());
 }
 
 int tg_solver_synchronize(tg_solver* h) {

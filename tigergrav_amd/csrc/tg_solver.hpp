@@ -38,12 +38,14 @@ struct Solver {
 	double soft_len = 0.0, m = 0.0, theta = 0.7;
 	int parts_per_node = 64;
 	bool uniform_mass = true;
-	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64)
+	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64); applies from the next set_particles
+	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
+	int resident_groups = 1024;  // work groups of the persistent form (set from the CU count)
 	Tree tree;
 	int64_t n = 0;
 	// device state
 	DeviceArray d_parts, d_multis, d_nodepos, d_out, d_partial, d_node_slot_ptr;
-	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL;
+	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL, d_queue;
 	KindWork work[NKINDS];
 	// host-only solvers (device < 0) keep the PP-direct items and flattened source list
 	std::vector<int4> host_pp_items;
@@ -65,6 +67,9 @@ struct Solver {
 	int set_particles(const int32_t* x, const float* mass, int64_t count);
 	int upload();
 	int enqueue_solve(bool timed);
+	hipGraphExec_t graph_exec = nullptr;
+	int solve_graph();
+	void drop_graph();
 	int kernel_times(std::vector<float>& ms);       // last timed solve
 	int collect_times(double* ms_sum, int64_t* n);  // sums over the timed solves since the last collect
 	int fold_set(int64_t solve);

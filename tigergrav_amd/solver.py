@@ -37,6 +37,8 @@ def _bind():
     L.tg_solver_destroy.argtypes = [vp]
     L.tg_solver_set_particles.argtypes = [vp, vp, vp, i64]
     L.tg_solver_solve.argtypes = [vp]
+    L.tg_solver_solve_graph.argtypes = [vp]
+    L.tg_solver_tune.argtypes = [vp, C.c_char_p, ci]
     L.tg_solver_synchronize.argtypes = [vp]
     L.tg_solver_get_forces.argtypes = [vp, vp]
     L.tg_solver_kernel_ms.argtypes = [vp, vp]
@@ -45,7 +47,7 @@ def _bind():
     L.tg_solver_export.argtypes = [vp, C.c_char_p, vp, i64]
     L.tg_solver_export.restype = i64
     L.tg_solver_last_error.restype = C.c_char_p
-    for n in ("create", "destroy", "set_particles", "solve", "synchronize", "get_forces", "kernel_ms", "collect_ms", "info"):
+    for n in ("create", "destroy", "set_particles", "solve", "solve_graph", "tune", "synchronize", "get_forces", "kernel_ms", "collect_ms", "info"):
         getattr(L, "tg_solver_" + n).restype = ci
     _bound = True
     return L
@@ -90,6 +92,15 @@ class Solver:
 
     def solve(self):
         _check(lib().tg_solver_solve(self._h), "tg_solver_solve")
+
+    def tune(self, knob, value):
+        """Set a tuning knob: "pp_persistent" (0/1), "node_chunk" (1..64, from the next
+        set_particles), "resident" (work groups of the persistent PP kernel)."""
+        _check(lib().tg_solver_tune(self._h, knob.encode(), int(value)), "tg_solver_tune")
+
+    def solve_graph(self):
+        """The same solve replayed from a captured HIP graph (no per-kind timings)."""
+        _check(lib().tg_solver_solve_graph(self._h), "tg_solver_solve_graph")
 
     def synchronize(self):
         _check(lib().tg_solver_synchronize(self._h), "tg_solver_synchronize")
