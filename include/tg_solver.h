@@ -40,6 +40,19 @@ int tg_solver_destroy(tg_solver* s);
  * lists on the host (tree::refine, compute_multipoles, kick_fmm's list construction) and makes
  * them resident on the device */
 int tg_solver_set_particles(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
+/* new positions (and masses, or NULL to keep them) for the same particles, in input order, with
+ * the tree, the interaction lists and the expansion centres of the last tg_solver_set_particles
+ * kept: the moments are recomputed about the kept centres.  The next solve evaluates that frozen
+ * tree at the new positions -- the function tg_solver_adjoint differentiates (no reference
+ * counterpart; the reference rebuilds the tree every step). */
+int tg_solver_update_particles(tg_solver* s, const int32_t* x, const float* mass);
+/* Adjoint of the last solve's particle set (no reference counterpart): for the loss
+ * L = sum_i w_i . g_i (w: 3 floats per particle, input order), grad receives per particle
+ * (dL/dx, dL/dy, dL/dz, dL/dm) in input order, positions in box units -- the derivatives of the
+ * solve with its tree, lists and expansion centres held fixed (see tg_solver_update_particles).
+ * Interactions are recomputed from the stored lists.  Needs leaves of at most 64 particles.
+ * *ms, if not NULL, receives the device time of the adjoint kernels. */
+int tg_solver_adjoint(tg_solver* s, const float* w, double* grad, double* ms);
 /* enqueue one force solve on the solver's stream (replaces tree::kick_fmm's kernel calls) */
 int tg_solver_solve(tg_solver* s);
 /* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent

@@ -1,0 +1,70 @@
+"""The adjoint of the batched solve (tg_solver_adjoint) against finite differences of the same
+solve.
+
+The adjoint differentiates the solve with its tree, interaction lists and expansion centres held
+fixed -- exactly what tg_solver_update_particles re-evaluates.  For L = sum_i w_i . g_i and a random
+direction d (every particle's position, or every mass, moved at once), the central difference
+(L(x + d) - L(x - d)) / 2 of two frozen-tree solves must equal grad . d.  Moving every particle at
+once makes the change of L large against the solve's single-precision rounding, whose floor in
+these differences is about 1e-5 absolute (the tolerance carries it); the masses enter linearly, so
+that check is tight.
+"""
+import numpy as np
+import pytest
+
+from tigergrav_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (generator, ewald, theta, leaf, position step in 2^-32 box units)
+    "uniform_periodic": (lambda: ics.uniform(4096, 81), True, 0.7, 32, 2 ** 14),
+    "plummer_open": (lambda: ics.plummer_spheres(8192, 82), False, 0.7, 64, 2 ** 14),
+    "zeldovich_periodic": (lambda: ics.zeldovich(16, 83), True, 0.5, 64, 2 ** 18),  # near a lattice: small forces
+}
+
+
+def _loss(s, w, x, mass=None):
+    s.update_particles(x, mass)
+    s.solve()
+    s.synchronize()
+    return float(np.sum(w * s.forces()[:, 1:]))
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_adjoint_directional_derivatives(case):
+    from tigergrav_amd.solver import Solver
+
+    gen, ewald, theta, ppn, step_max = CASES[case]
+    x = gen()
+    n = x.shape[0]
+    h, m = ics.soft_len(n), 1.0 / n
+    print(f"adjoint {case}: N={n} ewald={ewald} theta={theta} leaf={ppn}", flush=True)
+    s = Solver(ewald=ewald, soft_len=h, m=m, theta=theta, parts_per_node=ppn, device=0)
+    s.set_particles(x)
+    rng = np.random.default_rng(len(case))
+    w = rng.standard_normal((n, 3)).astype(np.float32)
+    s.solve()
+    s.synchronize()
+    grad, ms = s.adjoint(w, with_time=True)
+    assert np.all(np.isfinite(grad))
+    # positions: every particle moved by up to step_max (2^-18 .. 2^-14 of the box, inside every leaf)
+    for k in range(3):
+        step = rng.integers(-step_max, step_max, size=x.shape).astype(np.int64)
+        xp = (x.astype(np.int64) + step).astype(np.int32)
+        xm = (x.astype(np.int64) - step).astype(np.int32)
+        fd = 0.5 * (_loss(s, w, xp) - _loss(s, w, xm))
+        pred = float(np.sum(grad[:, :3] * step * 2.0 ** -32))
+        print(f"  positions {k}: fd {fd:.6e} adjoint {pred:.6e} rel {abs(fd - pred) / abs(pred):.2e}", flush=True)
+        assert abs(fd - pred) <= 3e-3 * abs(pred) + 2e-5
+    # masses: L is linear in them
+    for k in range(2):
+        dm = (m * rng.uniform(-1e-2, 1e-2, n)).astype(np.float64)
+        mp = (m + dm).astype(np.float32)
+        mm = (m - dm).astype(np.float32)
+        dmf = 0.5 * (mp.astype(np.float64) - mm.astype(np.float64))  # the steps the float masses carry
+        fd = 0.5 * (_loss(s, w, x, mp) - _loss(s, w, x, mm))
+        pred = float(np.sum(grad[:, 3] * dmf))
+        print(f"  masses {k}: fd {fd:.6e} adjoint {pred:.6e} rel {abs(fd - pred) / abs(pred):.2e}", flush=True)
+        assert abs(fd - pred) <= 1e-3 * abs(pred)
+    print(f"  adjoint device time {ms:.3f} ms", flush=True)

@@ -238,3 +238,59 @@ def test_solve_accuracy_open():
     _, g64 = ob.direct64(0, h, T["pos"][idx], T["pos"], 1.0 / N)
     e = np.linalg.norm(f[idx, 1:] - g64, axis=1) / np.linalg.norm(g64, axis=1)
     assert math.sqrt(np.mean(e ** 2)) < 3e-3
+
+
+def test_update_particles_keeps_tree_and_recomputes_moments():
+    """Frozen-tree update (tg_solver_update_particles): the same positions give bit-identical
+    moments; moved particles keep the lists and expansion centres, and every leaf's moments are the
+    raw moments about its kept centre (independent numpy P2M, float tolerance), every interior
+    node's the M2M of its children's (tigergrav/multipole.hpp:414-433, restated here)."""
+    from tigergrav_amd import ics
+
+    n = 4096
+    x = ics.uniform(n, 71)
+    s = make_solver(x, True, ppn=32)
+    m0 = s.export("node_m").copy()
+    xc = s.export("node_x").copy()
+    lists = {k: s.export(k).copy() for k in ("pp_d", "cc_d", "pc_d")}
+    s.update_particles(x)
+    assert np.array_equal(s.export("node_m"), m0)
+    rng = np.random.default_rng(72)
+    x2 = (x.astype(np.int64) + rng.integers(-2 ** 20, 2 ** 20, size=x.shape)).astype(np.int32)
+    mass = rng.uniform(0.5, 1.5, n).astype(np.float32) / n
+    s.update_particles(x2, mass)
+    for k, v in lists.items():
+        assert np.array_equal(s.export(k), v)
+    assert np.array_equal(s.export("node_x"), xc)
+    M = s.export("node_m").astype(np.float64)
+    pos = s.export("pos").astype(np.float64) * 2.0 ** -32
+    order = s.export("order")
+    mt = mass[order].astype(np.float64)
+    nr, ch = s.export("node_range"), s.export("node_child")
+    X = xc.astype(np.float64) * 2.0 ** -32
+    i2 = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    i3 = [(0, 0, 0), (0, 0, 1), (0, 0, 2), (0, 1, 1), (0, 1, 2), (0, 2, 2), (1, 1, 1), (1, 1, 2), (1, 2, 2), (2, 2, 2)]
+    for nd in range(len(nr)):
+        b, e = nr[nd]
+        if e == b:
+            continue
+        if ch[nd, 0] < 0:
+            d = pos[b:e] - X[nd]
+            want = [mt[b:e].sum()] + [np.sum(mt[b:e] * d[:, p] * d[:, q]) for p, q in i2] + \
+                   [np.sum(mt[b:e] * d[:, p] * d[:, q] * d[:, r]) for p, q, r in i3]
+        else:
+            want = np.zeros(17)
+            for c in ch[nd]:
+                y = X[c] - X[nd]
+                mc = M[c]
+                m2 = {pq: mc[1 + i] for i, pq in enumerate(i2)}
+                s2 = lambda p, q: m2[(min(p, q), max(p, q))]
+                want[0] += mc[0]
+                for i, (p, q) in enumerate(i2):
+                    want[1 + i] += mc[1 + i] + mc[0] * y[p] * y[q]
+                for i, (p, q, r) in enumerate(i3):
+                    want[7 + i] += mc[7 + i] + mc[0] * y[p] * y[q] * y[r] + s2(p, q) * y[r] + s2(q, r) * y[p] + s2(r, p) * y[q]
+        want = np.asarray(want)
+        for lo, hi in ((0, 1), (1, 7), (7, 17)):  # each order against its own magnitude
+            scale = np.abs(want[lo:hi]).max() + 1e-30
+            assert np.max(np.abs(M[nd, lo:hi] - want[lo:hi])) <= 2e-5 * scale, (nd, lo, M[nd], want)

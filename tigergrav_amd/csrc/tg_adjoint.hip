@@ -1,0 +1,815 @@
+// tg_adjoint.hip -- the adjoint of the batched force solve.
+//
+// For a loss L = sum_i w_i . g_i of the accelerations of one solve, the gradients dL/dx_k and
+// dL/dm_k, with the tree, the interaction lists and the expansion centres of that solve held
+// fixed (the function tg_solver_update_particles evaluates).  The solve is linear in every
+// expansion and moment, so the adjoint runs the solve's operators transposed, recomputing every
+// interaction from the stored lists (nothing of the forward is kept but its expansions dL):
+//   1. sinks, over the forward's leaf work items:
+//      k_adj_l2p       g = -grad T_L(x - X):   dL/dx += -w^T Hess T_L,  lambda_leaf += dg/dL^T w;
+//                      Lambda_a += S(X_leaf - X_a)^T lambda_leaf for every node a on the root path
+//      k_adj_pp(_ewald) tidal tensor J(d) = dF/dd per pair; the lists are symmetric (leaf B is in
+//                      A's PP list iff A is in B's), so each particle's source-side terms are gathered
+//                      from its own list: dL/dx_k += J(d_kj) (m_j w_k - m_k w_j), dL/dm_k += -w_j . F(d_kj)
+//      k_adj_pc        dL/dx_i += w . dg/dx_i (fifth derivatives), mu_c += dg/dM_c^T w
+//   2. k_adj_node, per CC/CP list entry of node a:
+//      CC        mu_c += M2L(., X_a - X_c)^T Lambda_a
+//      CP        dL/dm_j += Lambda_a . D(X_a - x_j), dL/dx_j += -m_j Lambda_a . grad D (fifth derivatives)
+//   3. k_adj_moments, per leaf: mu down the root path through the transposed M2M shifts, then the
+//      transposed P2M onto the leaf's particles.
+// Ewald kinds use the periodic correction's derivatives in the same places.  Per-term arithmetic is
+// FP32 like the forward; sums are FP64.  Node-indexed adjoints (Lambda, mu) and the CP source
+// terms are accumulated with FP64 atomics, so their summation order varies between runs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tg_adjoint_math.hpp"
+#include "tg_gravity.h"
+#include "tg_local.hpp"
+#include "tg_solver.hpp"
+
+namespace tg {
+
+__constant__ double adj::c_adj_ser[6 * TG_ADJ_NSER];
+__constant__ float c_adj_real[TG_NREAL * 3];
+__constant__ float c_adj_four[TG_NFOUR * 38];
+
+namespace {
+
+using adj::add;
+using adj::merge;
+
+hipError_t init_adjoint_tables() {
+	// erf(2r)/r = sum_q c_q u^q, u = r^2; d_k = (r^-1 d/dr)^k = sum_q c_q (2q)(2q-2)...(2q-2k+2) u^(q-k)
+	std::vector<double> ser(6 * TG_ADJ_NSER, 0.0);
+	for (int k = 0; k <= 5; k++)
+		for (int j = 0; j < TG_ADJ_NSER; j++) {
+			const int q = j + k;
+			double c = 2.0 / std::sqrt(M_PI) * ((q & 1) ? -1.0 : 1.0) * std::ldexp(1.0, 2 * q + 1) / std::tgamma(q + 1.0) / (2 * q + 1);
+			for (int t = 0; t < k; t++) c *= double(2 * q - 2 * t);
+			ser[k * TG_ADJ_NSER + j] = c;
+		}
+	hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(adj::c_adj_ser), ser.data(), ser.size() * sizeof(double));
+	if (e != hipSuccess) return e;
+	std::vector<float> real, four;
+	ewald_tables(real, four);
+	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
+	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_adj_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;
+	return hipMemcpyToSymbol(HIP_SYMBOL(c_adj_four), four.data(), four.size() * sizeof(float));
+}
+
+// ---- compile-time contraction helpers ---------------------------------------------------------
+// multipole slot of expansion slot K of order 2 or 3 (the multipole has no order 1)
+constexpr int mslot(int K) { return K < 10 ? 1 + (K - 4) : 7 + (K - 10); }
+constexpr float mfac(int K) { return K < 10 ? ef2(K - 4) : -ef3(K - 10); }  // M2P/M2L sign and expansion factor
+
+template <int... S, class F>
+__device__ __forceinline__ void for_seq(std::integer_sequence<int, S...>, F&& f) {
+	(f(std::integral_constant<int, S>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void for_n(F&& f) {
+	for_seq(std::make_integer_sequence<int, N>{}, f);
+}
+
+// M2P acceleration at offset with derivative tensor D (the forward's m2p):
+//   g_a = -sum_K fac_K M_K D_{K u a}, K over {0} u order-2 u order-3 slots, fac_0 = 1
+// its derivative along b adds index b; its transpose in M contracts with w.
+// returns dg/dx_b contracted with w (3 values) -- needs D to order 5
+__device__ __forceinline__ void m2p_sink_adjoint(float* gx, const float* M, const float* D, const float* w) {
+	for_n<3>([&](auto b_) {
+		constexpr int b = decltype(b_)::value;
+		float acc = 0.0f;
+		for_n<3>([&](auto a_) {
+			constexpr int a = decltype(a_)::value;
+			float t = -M[0] * D[add(add(0, a), b)];
+			for_n<16>([&](auto q_) {
+				constexpr int K = 4 + decltype(q_)::value;  // order-2 and order-3 slots 4..19
+				t = fmaf(-mfac(K) * M[mslot(K)], D[add(add(K, a), b)], t);
+			});
+			acc = fmaf(w[a], t, acc);
+		});
+		gx[b] += acc;
+	});
+}
+// mu_K += dg/dM_K^T w
+__device__ __forceinline__ void m2p_source_adjoint(float* mu, const float* D, const float* w) {
+	float t0 = 0.0f;
+	for_n<3>([&](auto a_) {
+		constexpr int a = decltype(a_)::value;
+		t0 = fmaf(-w[a], D[add(0, a)], t0);
+	});
+	mu[0] += t0;
+	for_n<16>([&](auto q_) {
+		constexpr int K = 4 + decltype(q_)::value;
+		float t = 0.0f;
+		for_n<3>([&](auto a_) {
+			constexpr int a = decltype(a_)::value;
+			t = fmaf(w[a], D[add(K, a)], t);
+		});
+		mu[mslot(K)] += -mfac(K) * t;
+	});
+}
+// M2L transposed: mu_K += sum_t Lambda_t dL_t/dM_K for the forward's m2l (L_t += sum_K fac_K M_K D_{t u K}
+// over |t| + |K| <= 4, fac_0 = 1)
+__device__ __forceinline__ void m2l_adjoint(float* mu, const float* Lam, const float* D) {
+	float t0 = 0.0f;
+	for_n<35>([&](auto t_) {
+		constexpr int t = decltype(t_)::value;
+		t0 = fmaf(Lam[t], D[t], t0);
+	});
+	mu[0] += t0;
+	for_n<16>([&](auto q_) {
+		constexpr int K = 4 + decltype(q_)::value;
+		float acc = 0.0f;
+		for_n<35>([&](auto t_) {
+			constexpr int t = decltype(t_)::value;
+			if constexpr (adj::order_of(t) + adj::order_of(K) <= 4) acc = fmaf(Lam[t], D[merge(t, K)], acc);
+		});
+		mu[mslot(K)] += mfac(K) * acc;
+	});
+}
+
+// spline-softened pair (src/gravity.cpp:231-287) in box units: F(d) = -d f(r), and f'(r)/r
+__device__ __forceinline__ void spline_f(float r2, float h, float& f, float& fr) {
+	if (r2 > h * h) {
+		const float ri = __builtin_amdgcn_rsqf(r2);
+		const float ri2 = ri * ri;
+		f = ri2 * ri;
+		fr = -3.0f * f * ri2;
+		return;
+	}
+	const float r = sqrtf(r2), hi = 1.0f / h, q = r * hi;
+	if (q < 0.5f) {
+		const float hi3 = hi * hi * hi;
+		f = fmaf(q * q, fmaf(32.0f, q, -192.0f / 5.0f), 32.0f / 3.0f) * hi3;
+		fr = fmaf(96.0f, q, -384.0f / 5.0f) * hi3 * hi * hi;
+	} else {
+		const float q3 = q * q * q;
+		const float P = fmaf(fmaf(fmaf(fmaf(-32.0f / 3.0f, q, 192.0f / 5.0f), q, -48.0f), q, 64.0f / 3.0f), q3, -1.0f / 15.0f);
+		const float dP = q * q * fmaf(fmaf(fmaf(-64.0f, q, 192.0f), q, -192.0f), q, 64.0f);
+		const float ri = 1.0f / r, ri3 = ri * ri * ri;
+		f = P * ri3;
+		fr = (dP * hi - 3.0f * P * ri) * ri3 * ri;
+	}
+}
+
+struct AdjArgs {
+	const int4* parts;      // tree order (x, y, z, mass bits)
+	const float4* w;        // loss weights, tree order
+	const MultiSrc* multis;
+	const int4* nodepos;
+	const double* dL;       // the forward's per-node expansions
+	const int4* leaves;     // (node, pbegin, pend, path begin)
+	const int* path;
+	const int* level;
+	double* grad;           // 4 per particle, tree order: dL/dx, dL/dy, dL/dz, dL/dm
+	double* Lam;            // 35 per node
+	double* mu;             // 17 per node
+	float h;                // softening, box units
+	int wrap;
+	int maxdepth;
+};
+
+constexpr float PINV = 1.0f / 4294967296.0f;
+__device__ __forceinline__ float sepf(int a, int b, bool wrap) {
+	const unsigned d = unsigned(a) - unsigned(b);
+	if (wrap) return float(int(d)) * PINV;
+	return float(double(a) - double(b)) * PINV;
+}
+
+// ---- 1. leaves -------------------------------------------------------------------------------
+// L2P and the shifts: per leaf, writes each sink's gradient (the first term; later kernels add)
+__global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
+	const int4 lf = A.leaves[blockIdx.x];
+	const int t = threadIdx.x;
+	const int depth = A.level[lf.x] + 1;
+	const int ns = lf.z - lf.y;
+	extern __shared__ double lds[];  // [35][64] per-sink lambda, [35] L then lambda of the leaf, [35][maxdepth]
+	double* lam64 = lds;
+	double* Ls = lds + 35 * 64;
+	double* sh = Ls + 35;
+	__shared__ int anc[64];
+	const int4 c = A.nodepos[lf.x];
+	// the leaf's local expansion, as the forward forms it (sum over the path of shifted dL)
+	double off[3] = {0.0, 0.0, 0.0};
+	if (t < depth) {
+		const int a = A.path[lf.w + t];
+		anc[t] = a;
+		const int4 X = A.nodepos[a];
+		off[0] = double(int64_t(c.x) - int64_t(X.x)) * POS_INV;
+		off[1] = double(int64_t(c.y) - int64_t(X.y)) * POS_INV;
+		off[2] = double(int64_t(c.z) - int64_t(X.z)) * POS_INV;
+		double La[35], Lt[35];
+#pragma unroll
+		for (int q = 0; q < 35; q++) La[q] = A.dL[35 * int64_t(a) + q];
+		l2l(Lt, La, off);
+#pragma unroll
+		for (int q = 0; q < 35; q++) sh[q * A.maxdepth + t] = Lt[q];
+	}
+	__syncthreads();
+	if (t < 35) {
+		double acc = 0.0;
+		for (int k = 0; k < depth; k++) acc += sh[t * A.maxdepth + k];
+		Ls[t] = acc;
+	}
+	__syncthreads();
+	const bool sink = t < ns;
+	const int pi = lf.y + min(t, ns - 1);
+	const int4 me = A.parts[pi];
+	const float4 wv = A.w[pi];
+	const float wi[3] = {sink ? wv.x : 0.0f, sink ? wv.y : 0.0f, sink ? wv.z : 0.0f};
+	double gx[3] = {0.0, 0.0, 0.0};
+	{
+		const double d[3] = {double(float(double(int64_t(me.x) - int64_t(c.x)) * POS_INV)),
+				double(float(double(int64_t(me.y) - int64_t(c.y)) * POS_INV)),
+				double(float(double(int64_t(me.z) - int64_t(c.z)) * POS_INV))};
+		double W[35];
+		taylor_weights(W, d);
+		// g_a = -sum_{|s|<=3} L_{s u a} W_s  ->  dg_a/dd_b = -sum_{|s|<=2} L_{s u a u b} W_s
+		for_n<3>([&](auto a_) {
+			constexpr int a = decltype(a_)::value;
+			for_n<3>([&](auto b_) {
+				constexpr int b = decltype(b_)::value;
+				double hab = 0.0;
+				for_n<10>([&](auto s_) {
+					constexpr int s = decltype(s_)::value;
+					hab = fma(Ls[Merge<Merge<s, 1 + a>::value, 1 + b>::value], W[s], hab);
+				});
+				gx[b] -= double(wi[a]) * hab;
+			});
+		});
+		// lambda_t = dg/dL_t^T w = -sum_a w_a W_{t minus a}
+		double lam[35];
+#pragma unroll
+		for (int q = 0; q < 35; q++) lam[q] = 0.0;
+		for_n<20>([&](auto s_) {
+			constexpr int s = decltype(s_)::value;
+			for_n<3>([&](auto a_) {
+				constexpr int a = decltype(a_)::value;
+				lam[Merge<s, 1 + a>::value] -= double(wi[a]) * W[s];
+			});
+		});
+#pragma unroll
+		for (int q = 0; q < 35; q++) lam64[q * 64 + t] = lam[q];
+	}
+	__syncthreads();
+	if (t < 35) {
+		double acc = 0.0;
+		for (int k = 0; k < ns; k++) acc += lam64[t * 64 + k];
+		Ls[t] = acc;  // now lambda of the leaf
+	}
+	__syncthreads();
+	// Lambda_a += S(X_leaf - X_a)^T lambda for every node on the path: (S^T l)_u = sum_{t u s = u} l_t W_s(off)
+	if (t < depth) {
+		double W[35], out[35];
+		taylor_weights(W, off);
+#pragma unroll
+		for (int q = 0; q < 35; q++) out[q] = 0.0;
+		for_n<35>([&](auto t_) {
+			constexpr int tt = decltype(t_)::value;
+			const double lt = Ls[tt];
+			for_n<35>([&](auto s_) {
+				constexpr int s = decltype(s_)::value;
+				if constexpr (Merge<tt, s>::valid) out[Merge<tt, s>::value] = fma(lt, W[s], out[Merge<tt, s>::value]);
+			});
+		});
+		double* dst = A.Lam + 35 * int64_t(anc[t]);
+#pragma unroll
+		for (int q = 0; q < 35; q++) atomicAdd(dst + q, out[q]);
+	}
+	if (sink) {
+		double* o = A.grad + 4 * int64_t(pi);
+		o[0] = gx[0];
+		o[1] = gx[1];
+		o[2] = gx[2];
+		o[3] = 0.0;
+	}
+}
+
+// leaf work items of one kind (the forward's: first sink, count [| upper bits], list begin, end)
+struct AdjItems {
+	const int4* items;
+	const int* list;
+	int nitems;
+};
+
+// PP direct: one item per group, lane = sink, sources in LDS tiles of 64 with their weights.
+// The lists are symmetric, so particle k's own list carries its source-side terms as well:
+//   dL/dx_k += sum_j J(d_kj) (m_j w_k - m_k w_j),  J v = -f v - (f'/r)(d.v) d,
+//   dL/dm_k += sum_j f(d_kj) (d_kj . w_j)
+__global__ __launch_bounds__(64) void k_adj_pp(AdjArgs A, AdjItems I) {
+	const int4 it = I.items[blockIdx.x];
+	const int t = threadIdx.x;
+	const int ns = it.y & PP_COUNT_MASK;
+	const bool sink = t < ns;
+	const int pi = it.x + min(t, ns - 1);
+	const int4 me = A.parts[pi];
+	const float4 wv = A.w[pi];
+	const float wi[3] = {wv.x, wv.y, wv.z};
+	const float mi = __int_as_float(me.w);
+	__shared__ int4 tp[64];
+	__shared__ float4 tw[64];
+	double gx[3] = {0.0, 0.0, 0.0}, gm = 0.0;
+	for (int base = it.z; base < it.w; base += 64) {
+		const int n = min(64, it.w - base);
+		__syncthreads();
+		if (t < n) {
+			const int j = I.list[base + t];
+			tp[t] = A.parts[j];
+			tw[t] = A.w[j];
+		}
+		__syncthreads();
+		float fx[3] = {0.f, 0.f, 0.f}, fm = 0.f;
+		for (int k = 0; k < n; k++) {
+			const int4 s = tp[k];
+			const float4 ws = tw[k];
+			const float mj = __int_as_float(s.w);
+			const float d[3] = {sepf(me.x, s.x, A.wrap), sepf(me.y, s.y, A.wrap), sepf(me.z, s.z, A.wrap)};
+			const float v[3] = {mj * wi[0] - mi * ws.x, mj * wi[1] - mi * ws.y, mj * wi[2] - mi * ws.z};
+			float f, fr;
+			spline_f(fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0])), A.h, f, fr);
+			const float dv = fmaf(d[2], v[2], fmaf(d[1], v[1], d[0] * v[0]));
+			const float dw = fmaf(d[2], ws.z, fmaf(d[1], ws.y, d[0] * ws.x));
+			const float frdv = fr * dv;
+#pragma unroll
+			for (int b = 0; b < 3; b++) fx[b] = fmaf(-f, v[b], fmaf(-frdv, d[b], fx[b]));
+			fm = fmaf(f, dw, fm);
+		}
+#pragma unroll
+		for (int b = 0; b < 3; b++) gx[b] += fx[b];
+		gm += fm;
+	}
+	if (sink) {
+		double* o = A.grad + 4 * int64_t(pi);
+		o[0] += gx[0];
+		o[1] += gx[1];
+		o[2] += gx[2];
+		o[3] += gm;
+	}
+}
+
+// PP Ewald: the correction's F = -D1, J = -D2.  Real space per pair over the item's image prefix
+// (as the forward); the Fourier part through structure factors of the item's sources, mass- and
+// weight-weighted: C, S = sum_j m_j (cos, sin) 2 pi h.x_j and Cw, Sw = sum_j w_j (cos, sin), so
+//   sum_j D2F(d_kj) (m_j w_k - m_k w_j) = sum_h H2_h [(cos_k C + sin_k S) w_k - m_k (cos_k Cw + sin_k Sw)]
+//   sum_j w_j . D1F(d_kj)             = sum_h H1_h . (sin_k Cw - cos_k Sw)
+__global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
+	const int4 it = I.items[blockIdx.x];
+	const int t = threadIdx.x;
+	const int ns = it.y & PP_COUNT_MASK;
+	const int nimg = min(TG_NREAL, int(unsigned(it.y) >> PP_NEAR_SHIFT));
+	const bool sink = t < ns;
+	const int pi = it.x + min(t, ns - 1);
+	const int4 me = A.parts[pi];
+	const float4 wv = A.w[pi];
+	const float wi[3] = {wv.x, wv.y, wv.z};
+	const float mi = __int_as_float(me.w);
+	__shared__ int4 tp[64];
+	__shared__ float4 tw[64];
+	__shared__ float sf[8][64];  // C, S, Cw[3], Sw[3] per Fourier vector
+	constexpr float REV = 2.3283064365386963e-10f;
+	const int hx = t < TG_NFOUR ? int(c_adj_four[38 * t]) : 0;
+	const int hy = t < TG_NFOUR ? int(c_adj_four[38 * t + 1]) : 0;
+	const int hz = t < TG_NFOUR ? int(c_adj_four[38 * t + 2]) : 0;
+	float C = 0.f, S = 0.f, Cw[3] = {0.f, 0.f, 0.f}, Sw[3] = {0.f, 0.f, 0.f};
+	double gx[3] = {0.0, 0.0, 0.0}, gm = 0.0;
+	for (int base = it.z; base < it.w; base += 64) {
+		const int n = min(64, it.w - base);
+		__syncthreads();
+		if (t < n) {
+			const int j = I.list[base + t];
+			tp[t] = A.parts[j];
+			tw[t] = A.w[j];
+		}
+		__syncthreads();
+		for (int k = 0; k < n; k++) {
+			const int4 s = tp[k];
+			const float4 ws = tw[k];
+			const unsigned ph = unsigned(hx) * unsigned(s.x) + unsigned(hy) * unsigned(s.y) + unsigned(hz) * unsigned(s.z);
+			const float tt = float(int(ph)) * REV;
+			const float cs = __builtin_amdgcn_cosf(tt), sn = __builtin_amdgcn_sinf(tt);
+			const float mj = __int_as_float(s.w);
+			C = fmaf(mj, cs, C);
+			S = fmaf(mj, sn, S);
+			Cw[0] = fmaf(ws.x, cs, Cw[0]);
+			Cw[1] = fmaf(ws.y, cs, Cw[1]);
+			Cw[2] = fmaf(ws.z, cs, Cw[2]);
+			Sw[0] = fmaf(ws.x, sn, Sw[0]);
+			Sw[1] = fmaf(ws.y, sn, Sw[1]);
+			Sw[2] = fmaf(ws.z, sn, Sw[2]);
+		}
+		// real-space part of every pair of this tile
+		float fx[3] = {0.f, 0.f, 0.f}, fm = 0.f;
+		for (int k = 0; k < n; k++) {
+			const int4 s = tp[k];
+			const float4 ws = tw[k];
+			const float mj = __int_as_float(s.w);
+			const float x[3] = {sepf(me.x, s.x, true), sepf(me.y, s.y, true), sepf(me.z, s.z, true)};
+			const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+			if (!(r2 > 1e-12f)) continue;  // the reference's constant below r = 1e-6: no derivatives
+			float D[10];
+#pragma unroll
+			for (int q = 0; q < 10; q++) D[q] = 0.0f;
+			float dd[6];
+			adj::ewald_self_factors5(double(r2), dd);
+			adj::radial_add<10>(D, dd, x);
+			for (int i = 0; i < nimg; i++) {
+				const float y[3] = {x[0] - c_adj_real[3 * i], x[1] - c_adj_real[3 * i + 1], x[2] - c_adj_real[3 * i + 2]};
+				const float q2 = fmaf(y[2], y[2], fmaf(y[1], y[1], y[0] * y[0]));
+				adj::ewald_real_factors5(q2, dd);
+				adj::radial_add<10>(D, dd, y);
+			}
+			const float v[3] = {mj * wi[0] - mi * ws.x, mj * wi[1] - mi * ws.y, mj * wi[2] - mi * ws.z};
+			for_n<3>([&](auto b_) {
+				constexpr int b = decltype(b_)::value;
+				float jv = 0.0f;
+				for_n<3>([&](auto a_) {
+					constexpr int a = decltype(a_)::value;
+					jv = fmaf(-D[add(add(0, a), b)], v[a], jv);
+				});
+				fx[b] += jv;
+			});
+			fm = fmaf(ws.x, D[1], fmaf(ws.y, D[2], fmaf(ws.z, D[3], fm)));
+		}
+#pragma unroll
+		for (int b = 0; b < 3; b++) gx[b] += fx[b];
+		gm += fm;
+	}
+	sf[0][t] = C;
+	sf[1][t] = S;
+	sf[2][t] = Cw[0];
+	sf[3][t] = Cw[1];
+	sf[4][t] = Cw[2];
+	sf[5][t] = Sw[0];
+	sf[6][t] = Sw[1];
+	sf[7][t] = Sw[2];
+	__syncthreads();
+	if (!sink) return;
+	// Fourier part at this sink (rows: h, then D0, D1 (3), D2 (6) coefficients of the doubled pair)
+	float fg[3] = {0.f, 0.f, 0.f}, fgm = 0.f;
+	for (int h = 0; h < TG_NFOUR; h++) {
+		const float* H = c_adj_four + 38 * h;
+		const unsigned ph = unsigned(int(H[0])) * unsigned(me.x) + unsigned(int(H[1])) * unsigned(me.y) + unsigned(int(H[2])) * unsigned(me.z);
+		const float tt = float(int(ph)) * REV;
+		const float cs = __builtin_amdgcn_cosf(tt), sn = __builtin_amdgcn_sinf(tt);
+		const float a0 = fmaf(cs, sf[0][h], sn * sf[1][h]);             // sum_j m_j cos(theta_k - theta_j)
+		const float aw[3] = {fmaf(cs, sf[2][h], sn * sf[5][h]), fmaf(cs, sf[3][h], sn * sf[6][h]), fmaf(cs, sf[4][h], sn * sf[7][h])};
+		const float bw[3] = {fmaf(sn, sf[2][h], -cs * sf[5][h]), fmaf(sn, sf[3][h], -cs * sf[6][h]), fmaf(sn, sf[4][h], -cs * sf[7][h])};
+		// D2F_ab(d) = H[7 + s2(a,b)] cos(2 pi h.d), D1F_a(d) = H[4 + a] sin(2 pi h.d)
+		for_n<3>([&](auto b_) {
+			constexpr int b = decltype(b_)::value;
+			float jv = 0.0f;
+			for_n<3>([&](auto a_) {
+				constexpr int a = decltype(a_)::value;
+				jv = fmaf(-H[7 + s2(a, b)], fmaf(a0, wi[a], -mi * aw[a]), jv);
+			});
+			fg[b] += jv;
+			fgm = fmaf(H[4 + b], bw[b], fgm);
+		});
+	}
+	double* o = A.grad + 4 * int64_t(pi);
+	o[0] += gx[0] + double(fg[0]);
+	o[1] += gx[1] + double(fg[1]);
+	o[2] += gx[2] + double(fg[2]);
+	o[3] += gm + double(fgm);
+}
+
+// PC: lane = sink; per multipole of the list the sink side (fifth derivatives) and the moments'
+// adjoint summed over the group's sinks (one FP64 atomic per slot)
+template <bool EWALD>
+__global__ __launch_bounds__(64) void k_adj_pc(AdjArgs A, AdjItems I) {
+	const int4 it = I.items[blockIdx.x];
+	const int t = threadIdx.x;
+	const int ns = it.y & PP_COUNT_MASK;
+	const bool sink = t < ns;
+	const int pi = it.x + min(t, ns - 1);
+	const int4 me = A.parts[pi];
+	const float4 wv = A.w[pi];
+	const float wi[3] = {sink ? wv.x : 0.0f, sink ? wv.y : 0.0f, sink ? wv.z : 0.0f};
+	double gx[3] = {0.0, 0.0, 0.0};
+	const bool wrapx = EWALD || A.wrap;
+	for (int e = it.z; e < it.w; e++) {
+		const int cidx = I.list[e];
+		const MultiSrc& ms = A.multis[cidx];
+		float M[17];
+#pragma unroll
+		for (int q = 0; q < 17; q++) M[q] = ms.m[q];
+		const float d[3] = {sepf(me.x, ms.x[0], wrapx), sepf(me.y, ms.x[1], wrapx), sepf(me.z, ms.x[2], wrapx)};
+		float D[adj::NS5];
+		if (EWALD)
+			adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
+		else
+			adj::green_direct_n<adj::NS5>(D, d);
+		float g3[3] = {0.f, 0.f, 0.f};
+		m2p_sink_adjoint(g3, M, D, wi);
+#pragma unroll
+		for (int b = 0; b < 3; b++) gx[b] += g3[b];
+		float mu[17];
+#pragma unroll
+		for (int q = 0; q < 17; q++) mu[q] = 0.0f;
+		m2p_source_adjoint(mu, D, wi);
+#pragma unroll
+		for (int q = 0; q < 17; q++) {
+			float v = mu[q];
+#pragma unroll
+			for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+			mu[q] = v;
+		}
+		if (t == 0) {
+#pragma unroll
+			for (int q = 0; q < 17; q++) atomicAdd(A.mu + 17 * int64_t(cidx) + q, double(mu[q]));
+		}
+	}
+	if (sink) {
+		double* o = A.grad + 4 * int64_t(pi);
+		o[0] += gx[0];
+		o[1] += gx[1];
+		o[2] += gx[2];
+	}
+}
+
+// ---- 2. node lists -------------------------------------------------------------------------------
+struct AdjNodeArgs {
+	const int2* waves;
+	const int* list;
+	const int* owner;
+	const int* slot_node;
+	const int4* nodepos;
+	const int4* parts;
+	const MultiSrc* multis;
+	const double* Lam;
+	double* mu;
+	double* grad;
+	int nwaves;
+	int wrap;
+};
+
+template <bool EWALD, bool CC>
+__global__ __launch_bounds__(64) void k_adj_node(AdjNodeArgs A) {
+	const int2 wv = A.waves[blockIdx.x];
+	const int j = wv.x + threadIdx.x;
+	if (j >= wv.y) return;
+	const int a = A.slot_node[A.owner[j]];
+	const int src = A.list[j];
+	const int4 X = A.nodepos[a];
+	float Lam[35];
+#pragma unroll
+	for (int q = 0; q < 35; q++) Lam[q] = float(A.Lam[35 * int64_t(a) + q]);
+	const bool wrapx = EWALD || A.wrap;
+	if (CC) {
+		const MultiSrc& ms = A.multis[src];
+		const float d[3] = {sepf(X.x, ms.x[0], wrapx), sepf(X.y, ms.x[1], wrapx), sepf(X.z, ms.x[2], wrapx)};
+		float D[35];
+		if (EWALD)
+			adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
+		else
+			adj::green_direct_n<35>(D, d);
+		float mu[17];
+#pragma unroll
+		for (int q = 0; q < 17; q++) mu[q] = 0.0f;
+		m2l_adjoint(mu, Lam, D);
+#pragma unroll
+		for (int q = 0; q < 17; q++) atomicAdd(A.mu + 17 * int64_t(src) + q, double(mu[q]));
+	} else {
+		const int4 s = A.parts[src];
+		const float mj = __int_as_float(s.w);
+		const float d[3] = {sepf(X.x, s.x, wrapx), sepf(X.y, s.y, wrapx), sepf(X.z, s.z, wrapx)};
+		float D[adj::NS5];
+		if (EWALD)
+			adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
+		else
+			adj::green_direct_n<adj::NS5>(D, d);
+		// L_t += m D_t(X_a - x_j):  dm += Lambda . D,  dx_j = -m Lambda . grad D
+		float gm = 0.0f, g3[3] = {0.f, 0.f, 0.f};
+		for_n<35>([&](auto t_) {
+			constexpr int t = decltype(t_)::value;
+			gm = fmaf(Lam[t], D[t], gm);
+			for_n<3>([&](auto b_) {
+				constexpr int b = decltype(b_)::value;
+				g3[b] = fmaf(Lam[t], D[add(t, b)], g3[b]);
+			});
+		});
+		double* o = A.grad + 4 * int64_t(src);
+#pragma unroll
+		for (int b = 0; b < 3; b++) atomicAdd(o + b, double(-mj * g3[b]));
+		atomicAdd(o + 3, double(gm));
+	}
+}
+
+// ---- 3. moments ----------------------------------------------------------------------------------
+// multipole<float>::operator>>= transposed (tigergrav/multipole.hpp:414-433, restated in the host
+// tree): M3'_pql = M3 + M0 Y_p Y_q Y_l + M2_pq Y_l + M2_ql Y_p + M2_lp Y_q, M2'_pq = M2 + M0 Y_p Y_q
+__device__ __forceinline__ void m2m_adjoint(float* mu, const float* mup, const float* Y) {
+	mu[0] = mup[0];
+#pragma unroll
+	for (int s = 0; s < 6; s++) mu[1 + s] = mup[1 + s];
+#pragma unroll
+	for (int s = 0; s < 10; s++) mu[7 + s] = mup[7 + s];
+#pragma unroll
+	for (int s = 0; s < 6; s++) mu[0] = fmaf(mup[1 + s] * Y[T2[s].i[0]], Y[T2[s].i[1]], mu[0]);
+#pragma unroll
+	for (int s = 0; s < 10; s++) {
+		const int p = T3[s].i[0], q = T3[s].i[1], l = T3[s].i[2];
+		const float u = mup[7 + s];
+		mu[0] = fmaf(u * Y[p] * Y[q], Y[l], mu[0]);
+		mu[1 + s2(p, q)] = fmaf(u, Y[l], mu[1 + s2(p, q)]);
+		mu[1 + s2(q, l)] = fmaf(u, Y[p], mu[1 + s2(q, l)]);
+		mu[1 + s2(l, p)] = fmaf(u, Y[q], mu[1 + s2(l, p)]);
+	}
+}
+
+__global__ __launch_bounds__(64) void k_adj_moments(AdjArgs A) {
+	const int4 lf = A.leaves[blockIdx.x];
+	const int t = threadIdx.x;
+	const int depth = A.level[lf.x] + 1;
+	__shared__ float mus[17];
+	if (t == 0) {
+		float m[17], n[17];
+		int a = A.path[lf.w];
+#pragma unroll
+		for (int q = 0; q < 17; q++) m[q] = float(A.mu[17 * int64_t(a) + q]);
+		for (int k = 1; k < depth; k++) {
+			const int b = A.path[lf.w + k];
+			const int4 Xa = A.nodepos[a], Xb = A.nodepos[b];
+			const float Y[3] = {float(double(int64_t(Xb.x) - int64_t(Xa.x)) * POS_INV),
+					float(double(int64_t(Xb.y) - int64_t(Xa.y)) * POS_INV), float(double(int64_t(Xb.z) - int64_t(Xa.z)) * POS_INV)};
+			m2m_adjoint(n, m, Y);
+#pragma unroll
+			for (int q = 0; q < 17; q++) m[q] = n[q] + float(A.mu[17 * int64_t(b) + q]);
+			a = b;
+		}
+#pragma unroll
+		for (int q = 0; q < 17; q++) mus[q] = m[q];
+	}
+	__syncthreads();
+	const int ns = lf.z - lf.y;
+	if (t >= ns) return;
+	const int pi = lf.y + t;
+	const int4 me = A.parts[pi];
+	const int4 c = A.nodepos[lf.x];
+	const float mj = __int_as_float(me.w);
+	// P2M (part_vect_multipole_info): M2_pq += m d_p d_q, M3_pql += m d_p d_q d_l, d = float(x - X)
+	const float d[3] = {float(double(int64_t(me.x) - int64_t(c.x)) * POS_INV), float(double(int64_t(me.y) - int64_t(c.y)) * POS_INV),
+			float(double(int64_t(me.z) - int64_t(c.z)) * POS_INV)};
+	float gm = mus[0], g3[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+	for (int s = 0; s < 6; s++) {
+		const int p = T2[s].i[0], q = T2[s].i[1];
+		const float u = mus[1 + s];
+		gm = fmaf(u * d[p], d[q], gm);
+		g3[p] = fmaf(u, d[q], g3[p]);
+		g3[q] = fmaf(u, d[p], g3[q]);
+	}
+#pragma unroll
+	for (int s = 0; s < 10; s++) {
+		const int p = T3[s].i[0], q = T3[s].i[1], l = T3[s].i[2];
+		const float u = mus[7 + s];
+		gm = fmaf(u * d[p] * d[q], d[l], gm);
+		g3[p] = fmaf(u * d[q], d[l], g3[p]);
+		g3[q] = fmaf(u * d[p], d[l], g3[q]);
+		g3[l] = fmaf(u * d[p], d[q], g3[l]);
+	}
+	double* o = A.grad + 4 * int64_t(pi);
+	o[0] += double(mj * g3[0]);
+	o[1] += double(mj * g3[1]);
+	o[2] += double(mj * g3[2]);
+	o[3] += double(gm);
+}
+
+__global__ void k_adj_zero(double* p, int64_t n) {
+	const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0.0;
+}
+
+}  // namespace
+
+#define ACHK(call)                                                   \
+	do {                                                             \
+		hipError_t e_ = (call);                                      \
+		if (e_ != hipSuccess) {                                      \
+			err = std::string(#call) + ": " + hipGetErrorString(e_); \
+			return e_ == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;  \
+		}                                                            \
+	} while (0)
+
+int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
+	if (device < 0 || n == 0 || tree.nodes.empty()) {
+		err = "adjoint needs a device solver with particles";
+		return TG_EINVAL;
+	}
+	if (parts_per_node > 64) {
+		err = "the adjoint handles leaves of at most 64 particles";
+		return TG_EINVAL;
+	}
+	if (!adj_pp_symmetric) {
+		err = "the adjoint needs symmetric PP lists";
+		return TG_EINVAL;
+	}
+	ACHK(hipSetDevice(device));
+	static thread_local int tables_ready = -1;
+	if (tables_ready != device) {
+		ACHK(init_adjoint_tables());
+		tables_ready = device;
+	}
+	const int64_t nn = int64_t(tree.nodes.size());
+	std::vector<float4> w(n);
+	for (int64_t i = 0; i < n; i++) {
+		const int64_t o = tree.order[i];
+		w[i] = make_float4(w_in[3 * o], w_in[3 * o + 1], w_in[3 * o + 2], 0.0f);
+	}
+	ACHK(d_adj_w.alloc(std::max<size_t>(n * sizeof(float4), 16)));
+	ACHK(d_adj_grad.alloc(std::max<size_t>(n * 4 * sizeof(double), 16)));
+	ACHK(d_adj_lam.alloc(std::max<size_t>(nn * 35 * sizeof(double), 16)));
+	ACHK(d_adj_mu.alloc(std::max<size_t>(nn * 17 * sizeof(double), 16)));
+	hipStream_t s = stream;
+	ACHK(hipMemcpyAsync(d_adj_w.p, w.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+	if (!adj_ev[0]) {
+		ACHK(hipEventCreate(&adj_ev[0]));
+		ACHK(hipEventCreate(&adj_ev[1]));
+	}
+	hipEvent_t e0 = adj_ev[0], e1 = adj_ev[1];
+	if (ms) ACHK(hipEventRecord(e0, s));
+	hipLaunchKernelGGL(k_adj_zero, dim3(unsigned((nn * 35 + 255) / 256)), dim3(256), 0, s, d_adj_lam.as<double>(), nn * 35);
+	hipLaunchKernelGGL(k_adj_zero, dim3(unsigned((nn * 17 + 255) / 256)), dim3(256), 0, s, d_adj_mu.as<double>(), nn * 17);
+	ACHK(hipGetLastError());
+	AdjArgs a{};
+	a.parts = d_parts.as<int4>();
+	a.w = d_adj_w.as<float4>();
+	a.multis = d_multis.as<MultiSrc>();
+	a.nodepos = d_nodepos.as<int4>();
+	a.dL = d_dL.as<double>();
+	a.leaves = d_leaf_items.as<int4>();
+	a.path = d_path.as<int>();
+	a.level = d_level.as<int>();
+	a.grad = d_adj_grad.as<double>();
+	a.Lam = d_adj_lam.as<double>();
+	a.mu = d_adj_mu.as<double>();
+	a.h = float(soft_len);
+	a.wrap = ewald;
+	a.maxdepth = tree.max_level + 1;
+	const size_t lds = size_t(35 * 64 + 35 + 35 * a.maxdepth) * sizeof(double);
+	if (n_leaf_items > 0) {
+		hipLaunchKernelGGL(k_adj_l2p, dim3(n_leaf_items), dim3(64), lds, s, a);
+		ACHK(hipGetLastError());
+	}
+	for (int k : {PP_DIRECT, PP_EWALD, PC_DIRECT, PC_EWALD}) {
+		if (work[k].nitems <= 0) continue;
+		AdjItems I{work[k].items.as<int4>(), work[k].list.as<int>(), work[k].nitems};
+		const dim3 g(work[k].nitems), b(64);
+		if (k == PP_DIRECT)
+			hipLaunchKernelGGL(k_adj_pp, g, b, 0, s, a, I);
+		else if (k == PP_EWALD)
+			hipLaunchKernelGGL(k_adj_pp_ewald, g, b, 0, s, a, I);
+		else if (k == PC_DIRECT)
+			hipLaunchKernelGGL(k_adj_pc<false>, g, b, 0, s, a, I);
+		else
+			hipLaunchKernelGGL(k_adj_pc<true>, g, b, 0, s, a, I);
+		ACHK(hipGetLastError());
+	}
+	AdjNodeArgs na{};
+	na.slot_node = d_slot_node.as<int>();
+	na.nodepos = d_nodepos.as<int4>();
+	na.parts = d_parts.as<int4>();
+	na.multis = d_multis.as<MultiSrc>();
+	na.Lam = d_adj_lam.as<double>();
+	na.mu = d_adj_mu.as<double>();
+	na.grad = d_adj_grad.as<double>();
+	na.wrap = ewald;
+	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
+		if (work[k].nitems <= 0) continue;
+		na.waves = work[k].items.as<int2>();
+		na.list = work[k].list.as<int>();
+		na.owner = work[k].owner.as<int>();
+		na.nwaves = work[k].nitems;
+		const dim3 g(work[k].nitems), b(64);
+		if (k == CC_DIRECT)
+			hipLaunchKernelGGL((k_adj_node<false, true>), g, b, 0, s, na);
+		else if (k == CP_DIRECT)
+			hipLaunchKernelGGL((k_adj_node<false, false>), g, b, 0, s, na);
+		else if (k == CC_EWALD)
+			hipLaunchKernelGGL((k_adj_node<true, true>), g, b, 0, s, na);
+		else
+			hipLaunchKernelGGL((k_adj_node<true, false>), g, b, 0, s, na);
+		ACHK(hipGetLastError());
+	}
+	if (n_leaf_items > 0) {
+		hipLaunchKernelGGL(k_adj_moments, dim3(n_leaf_items), dim3(64), 0, s, a);
+		ACHK(hipGetLastError());
+	}
+	if (ms) ACHK(hipEventRecord(e1, s));
+	std::vector<double> g(n * 4);
+	ACHK(hipMemcpyAsync(g.data(), d_adj_grad.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+	ACHK(hipStreamSynchronize(s));
+	if (ms) ACHK(hipEventElapsedTime(ms, e0, e1));
+	for (int64_t i = 0; i < n; i++) {
+		const int64_t o = tree.order[i];
+		for (int c = 0; c < 4; c++) grad_out[4 * o + c] = g[4 * i + c];
+	}
+	return TG_OK;
+}
+
+}  // namespace tg

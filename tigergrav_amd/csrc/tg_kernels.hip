@@ -77,6 +77,11 @@ static void build_tables(std::vector<float>& real, std::vector<float>& four, std
 		}
 }
 
+void ewald_tables(std::vector<float>& real, std::vector<float>& four) {
+	std::vector<double> ser;
+	build_tables(real, four, ser);
+}
+
 int ewald_images_within(double radius) {
 	int n = 0;
 	for (int i = -4; i <= 4; i++)

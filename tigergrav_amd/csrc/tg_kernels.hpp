@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace tg {
 
 constexpr int PP_COUNT_MASK = 127;
@@ -70,6 +72,9 @@ hipError_t init_device_tables(int device);
 // number of real-space Ewald images (n != 0, |n|^2 <= 11) with |n| < radius: the prefix of the
 // |n|-sorted table a PP Ewald item needs when no pair of it is farther apart than radius - 2.6
 int ewald_images_within(double radius);
+// the Ewald lattice tables the kernels use (real-space vectors by |n|, 3 floats each; half-space
+// Fourier rows of 38 floats: h, then the 35 doubled coefficients), for other translation units
+void ewald_tables(std::vector<float>& real, std::vector<float>& four);
 hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s);
 hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s);
 hipError_t launch_pp_ewald(const LeafArgs& a, hipStream_t s);

@@ -186,6 +186,8 @@ Solver::~Solver() {
 		drop_graph();
 		for (auto& e : ev)
 			if (e) hipEventDestroy(e);
+		for (auto& e : adj_ev)
+			if (e) hipEventDestroy(e);
 		d_parts.release();
 		d_multis.release();
 		d_nodepos.release();
@@ -194,6 +196,11 @@ Solver::~Solver() {
 		d_node_slot_ptr.release();
 		d_dL.release();
 		d_queue.release();
+		d_leaf_lists.release();
+		d_adj_w.release();
+		d_adj_grad.release();
+		d_adj_lam.release();
+		d_adj_mu.release();
 		d_leaf_items.release();
 		d_slot_node.release();
 		d_path.release();
@@ -229,6 +236,79 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	int rc = upload();
 	t_upload = now() - t3;
 	return rc;
+}
+
+template <class T>
+static hipError_t put(DeviceArray& d, const std::vector<T>& h, hipStream_t s);
+
+int Solver::update_particles(const int32_t* x, const float* mass) {
+	if (tree.nodes.empty() || n == 0) {
+		err = "update_particles before set_particles";
+		return TG_EINVAL;
+	}
+	drop_graph();
+	std::vector<int32_t> pt(3 * size_t(n));
+	std::vector<float> mt;
+	for (int64_t i = 0; i < n; i++)
+		for (int d = 0; d < 3; d++) pt[3 * i + d] = x[3 * tree.order[i] + d];
+	if (mass) {
+		mt.resize(n);
+		for (int64_t i = 0; i < n; i++) mt[i] = mass[tree.order[i]];
+		uniform_mass = false;
+	}
+	tree.update_moments(pt.data(), mass ? mt.data() : nullptr);
+	if (device < 0) return TG_OK;
+	SCHK(hipSetDevice(device));
+	return upload_sources();
+}
+
+// particle records and multipoles of the current tree to the device
+// the adjoint gathers each particle's PP source-side terms from its own list, which needs leaf B in
+// leaf A's PP lists exactly when A is in B's (true of the walk's lists, checked here per tree)
+bool Solver::pp_lists_symmetric() const {
+	const Tree& T = tree;
+	std::vector<int32_t> leaf_at(size_t(n) + 1, -1);
+	for (int32_t lf : T.leaves)
+		if (!T.nodes[lf].empty()) leaf_at[T.nodes[lf].pbegin] = lf;
+	for (int kind = 0; kind < 2; kind++) {
+		const std::vector<int64_t>& ptr = kind ? T.lists.pp_e_ptr : T.lists.pp_d_ptr;
+		const std::vector<int32_t>& v = kind ? T.lists.pp_e : T.lists.pp_d;
+		std::vector<std::pair<int32_t, int32_t>> pairs;
+		for (int32_t lf : T.leaves) {
+			if (size_t(lf) + 1 >= ptr.size()) continue;
+			for (int64_t k = ptr[lf]; k < ptr[lf + 1]; k += 2) {
+				const int32_t src = leaf_at[v[k]];
+				if (src < 0 || T.nodes[src].pend != v[k + 1]) return false;  // not a whole leaf
+				pairs.emplace_back(lf, src);
+			}
+		}
+		std::vector<std::pair<int32_t, int32_t>> rev(pairs.size());
+		for (size_t i = 0; i < pairs.size(); i++) rev[i] = {pairs[i].second, pairs[i].first};
+		std::sort(pairs.begin(), pairs.end());
+		std::sort(rev.begin(), rev.end());
+		if (pairs != rev) return false;
+	}
+	return true;
+}
+
+int Solver::upload_sources() {
+	const Tree& T = tree;
+	const int64_t nn = int64_t(T.nodes.size());
+	std::vector<int4> parts(n);
+	for (int64_t i = 0; i < n; i++) {
+		int w;
+		std::memcpy(&w, &T.mass[i], 4);
+		parts[i] = make_int4(T.pos[3 * i], T.pos[3 * i + 1], T.pos[3 * i + 2], w);
+	}
+	std::vector<MultiSrc> multis(nn);
+	for (int64_t i = 0; i < nn; i++) {
+		std::memcpy(multis[i].m, T.nodes[i].m, sizeof(T.nodes[i].m));
+		std::memcpy(multis[i].x, T.nodes[i].x, sizeof(T.nodes[i].x));
+	}
+	SCHK(put(d_parts, parts, stream));
+	SCHK(put(d_multis, multis, stream));
+	SCHK(hipStreamSynchronize(stream));
+	return TG_OK;
 }
 
 template <class T>
@@ -357,11 +437,14 @@ int Solver::upload() {
 		return d2 <= hnear * hnear;
 	};
 	std::vector<int32_t> far_ranges;
-	std::vector<int4> leaf_items;
+	std::vector<int4> leaf_items, leaf_lists;
 	for (int32_t lf : T.leaves) {
 		const Node& nd = T.nodes[lf];
 		if (nd.empty()) continue;
 		leaf_items.push_back(make_int4(lf, nd.pbegin, nd.pend, 0));
+		leaf_lists.push_back(make_int4(0, 0, 0, 0));
+		leaf_lists.push_back(make_int4(0, 0, 0, 0));
+		int4* ll = &leaf_lists[leaf_lists.size() - 2];
 		struct {
 			int kind;
 			const std::vector<int64_t>* ptr;
@@ -415,6 +498,10 @@ int Solver::upload() {
 				L.insert(L.end(), s.v->begin() + (*s.ptr)[lf], s.v->begin() + (*s.ptr)[lf + 1]);
 			}
 			const int64_t le = int64_t(L.size());
+			if (s.kind == PP_DIRECT) { ll[0].x = int(lb); ll[0].y = int(le); }
+			else if (s.kind == PC_DIRECT) { ll[0].z = int(lb); ll[0].w = int(le); }
+			else if (s.kind == PP_EWALD) { ll[1].x = int(lb); ll[1].y = int(le); }
+			else { ll[1].z = int(lb); ll[1].w = int(le); }
 			if (le == lb) continue;
 			for (int32_t b = nd.pbegin; b < nd.pend; b += 64) {
 				const int cnt = std::min<int32_t>(64, nd.pend - b);
@@ -457,6 +544,8 @@ int Solver::upload() {
 	SCHK(put(d_nodepos, npos, s));
 	SCHK(put(d_node_slot_ptr, slot_ptr, s));
 	SCHK(put(d_leaf_items, leaf_items, s));
+	SCHK(put(d_leaf_lists, leaf_lists, s));
+	adj_checked = false;
 	SCHK(put(d_slot_node, slot_node, s));
 	SCHK(put(d_path, path, s));
 	SCHK(put(d_level, level, s));
@@ -678,6 +767,24 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 	}
 	S.drop_graph();
 	return TG_OK;
+}
+
+int tg_solver_adjoint(tg_solver* h, const float* w, double* grad, double* ms) {
+	if (!h || h->s.device < 0 || (h->s.n > 0 && (!w || !grad))) return TG_EINVAL;
+	tg::Solver& S = h->s;
+	if (!S.adj_checked) {
+		S.adj_pp_symmetric = S.pp_lists_symmetric();
+		S.adj_checked = true;
+	}
+	float t = 0.0f;
+	const int rc = S.adjoint(w, grad, ms ? &t : nullptr);
+	if (ms) *ms = t;
+	return fail(h, rc);
+}
+
+int tg_solver_update_particles(tg_solver* h, const int32_t* x, const float* mass) {
+	if (!h || (h->s.n > 0 && !x)) return TG_EINVAL;
+	return fail(h, h->s.update_particles(x, mass));
 }
 
 int tg_solver_solve_graph(tg_solver* h) {

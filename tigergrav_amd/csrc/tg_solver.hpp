@@ -46,6 +46,11 @@ struct Solver {
 	// device state
 	DeviceArray d_parts, d_multis, d_nodepos, d_out, d_partial, d_node_slot_ptr;
 	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL, d_queue;
+	// adjoint: per leaf item the list ranges of PP_d, PC_d (one int4) and PP_e, PC_e (another)
+	DeviceArray d_leaf_lists, d_adj_w, d_adj_grad, d_adj_lam, d_adj_mu;
+	bool adj_pp_symmetric = false;  // checked once per tree at the first adjoint call
+	bool adj_checked = false;
+	hipEvent_t adj_ev[2] = {nullptr, nullptr};
 	KindWork work[NKINDS];
 	// host-only solvers (device < 0) keep the PP-direct items and flattened source list
 	std::vector<int4> host_pp_items;
@@ -66,6 +71,12 @@ struct Solver {
 	int init(int dev);
 	int set_particles(const int32_t* x, const float* mass, int64_t count);
 	int upload();
+	int upload_sources();
+	int update_particles(const int32_t* x, const float* mass);
+	// gradients of sum_i w_i . g_i (w: 3 per particle, input order) with respect to positions and
+	// masses: grad 4 per particle (d/dx, d/dy, d/dz, d/dm), input order; ms: device time, or NULL
+	int adjoint(const float* w, double* grad, float* ms);
+	bool pp_lists_symmetric() const;
 	int enqueue_solve(bool timed);
 	hipGraphExec_t graph_exec = nullptr;
 	int solve_graph();

@@ -176,6 +176,63 @@ static void m2m_shift(float* me, const float* Y) {
 		for (int q = p; q < 3; q++) me[m2s(p, q)] += me[0] * Y[p] * Y[q];
 }
 
+void Tree::update_moments(const int32_t* pos_tree, const float* mass_tree) {
+	std::copy(pos_tree, pos_tree + pos.size(), pos.begin());
+	if (mass_tree) std::copy(mass_tree, mass_tree + mass.size(), mass.begin());
+	for (int ni = int(nodes.size()) - 1; ni >= 0; ni--) {
+		Node& nd = nodes[ni];
+		if (nd.empty()) continue;  // zero moments, as built
+		if (nd.leaf()) {
+			// the P2M of part_vect_multipole_info about the kept centre, 16 float lanes
+			double Xcom[3];
+			for (int d = 0; d < 3; d++) Xcom[d] = pos_to_double(nd.x[d]);
+			float M[17][16];
+			std::memset(M, 0, sizeof(M));
+			for (int64_t i = nd.pbegin; i < nd.pend; i += 16) {
+				for (int k = 0; k < 16; k++) {
+					double X[3];
+					float ms;
+					if (i + k < nd.pend) {
+						for (int d = 0; d < 3; d++) X[d] = pos_to_double(pos[3 * (i + k) + d]);
+						ms = mass[i + k];
+					} else {
+						X[0] = X[1] = X[2] = 0.0;
+						ms = 0.0f;
+					}
+					float dx[3];
+					for (int d = 0; d < 3; d++) dx[d] = float(X[d] - Xcom[d]);
+					M[0][k] += ms;
+					for (int j = 0; j < 3; j++)
+						for (int kk = 0; kk <= j; kk++) {
+							const float Xjk = dx[j] * dx[kk];
+							M[m2s(j, kk)][k] += ms * Xjk;
+							for (int l = 0; l <= kk; l++) M[m3s(j, kk, l)][k] += ms * Xjk * dx[l];
+						}
+				}
+			}
+			for (int q = 0; q < 17; q++) {
+				float s = 0.0f;
+				for (int k = 0; k < 16; k++) s += M[q][k];
+				nd.m[q] = s;
+			}
+			continue;
+		}
+		const Node& L = nodes[nd.child[0]];
+		const Node& R = nodes[nd.child[1]];
+		float ml[17], mr[17], yl[3], yr[3];
+		std::memcpy(ml, L.m, sizeof(ml));
+		std::memcpy(mr, R.m, sizeof(mr));
+		for (int d = 0; d < 3; d++) {
+			const double x = pos_to_double(nd.x[d]);
+			yl[d] = float(pos_to_double(L.x[d]) - x);
+			yr[d] = float(pos_to_double(R.x[d]) - x);
+		}
+		m2m_shift(ml, yl);
+		m2m_shift(mr, yr);
+		for (int q = 0; q < 17; q++) nd.m[q] = ml[q] + mr[q];
+	}
+}
+
 void Tree::compute_multipoles() {
 	// children come after their parents in `nodes`, so a reverse sweep is bottom-up
 	for (int i = int(nodes.size()) - 1; i >= 0; i--) multipoles(i);

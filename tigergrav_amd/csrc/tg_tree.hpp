@@ -75,6 +75,10 @@ public:
 	void build(const int32_t* x, const float* masses, int64_t n, const TreeParams& p, double particle_mass);
 	// P2M / M2M / radii, bottom up
 	void compute_multipoles();
+	// new positions / masses (tree order) with the tree, lists and expansion centres kept: the
+	// moments are recomputed about the existing centres (leaf P2M, then M2M upward) exactly as
+	// compute_multipoles forms them; radii, boxes and lists are left as they are
+	void update_moments(const int32_t* pos_tree, const float* mass_tree);
 	// dual-tree walk from the root with the reference's MAC; fills `lists` and `census`
 	void walk();
 

@@ -38,6 +38,8 @@ def _bind():
     L.tg_solver_set_particles.argtypes = [vp, vp, vp, i64]
     L.tg_solver_solve.argtypes = [vp]
     L.tg_solver_solve_graph.argtypes = [vp]
+    L.tg_solver_update_particles.argtypes = [vp, vp, vp]
+    L.tg_solver_adjoint.argtypes = [vp, vp, vp, vp]
     L.tg_solver_tune.argtypes = [vp, C.c_char_p, ci]
     L.tg_solver_synchronize.argtypes = [vp]
     L.tg_solver_get_forces.argtypes = [vp, vp]
@@ -47,7 +49,7 @@ def _bind():
     L.tg_solver_export.argtypes = [vp, C.c_char_p, vp, i64]
     L.tg_solver_export.restype = i64
     L.tg_solver_last_error.restype = C.c_char_p
-    for n in ("create", "destroy", "set_particles", "solve", "solve_graph", "tune", "synchronize", "get_forces", "kernel_ms", "collect_ms", "info"):
+    for n in ("create", "destroy", "set_particles", "solve", "solve_graph", "tune", "update_particles", "adjoint", "synchronize", "get_forces", "kernel_ms", "collect_ms", "info"):
         getattr(L, "tg_solver_" + n).restype = ci
     _bound = True
     return L
@@ -89,6 +91,30 @@ class Solver:
             mass = np.ascontiguousarray(mass, np.float32).reshape(-1)
             mp = ptr(mass)
         _check(lib().tg_solver_set_particles(self._h, ptr(x), mp, self.n), "tg_solver_set_particles")
+
+    def update_particles(self, x, mass=None):
+        """New positions (and masses) for the same particles, input order, evaluated on the kept
+        tree, lists and expansion centres (moments recomputed about those centres)."""
+        x = np.ascontiguousarray(x, np.int32).reshape(-1, 3)
+        if x.shape[0] != self.n:
+            raise ValueError("update_particles needs the same particle count")
+        mp = None
+        if mass is not None:
+            mass = np.ascontiguousarray(mass, np.float32).reshape(-1)
+            mp = ptr(mass)
+        _check(lib().tg_solver_update_particles(self._h, ptr(x), mp), "tg_solver_update_particles")
+
+    def adjoint(self, w, with_time=False):
+        """Gradients of L = sum_i w_i . g_i for the current particle set with its tree, lists and
+        expansion centres held fixed: (N, 4) array of (dL/dx, dL/dy, dL/dz, dL/dm), input order,
+        positions in box units.  With with_time, also the adjoint's device time in ms."""
+        w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+        if w.shape[0] != self.n:
+            raise ValueError("w needs one row per particle")
+        g = np.zeros((self.n, 4), np.float64)
+        ms = np.zeros(1, np.float64)
+        _check(lib().tg_solver_adjoint(self._h, ptr(w), ptr(g), ptr(ms)), "tg_solver_adjoint")
+        return (g, float(ms[0])) if with_time else g
 
     def solve(self):
         _check(lib().tg_solver_solve(self._h), "tg_solver_solve")
