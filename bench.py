@@ -46,6 +46,10 @@ def make_workload(name):
         return ics.zeldovich(128, seed=3, n_s=-2.0, rms_cells=0.3), True, (
             "zeldovich 128^3 = 2,097,152 particles, P(k)~k^-2 cut at grid Nyquist, rms displacement 0.3 cell, "
             "seed 3, periodic Ewald")
+    if name == "zeldovich64":
+        return ics.zeldovich(64, seed=4, n_s=-2.0, rms_cells=0.3), True, (
+            "zeldovich 64^3 = 262,144 particles, P(k)~k^-2 cut at grid Nyquist, rms displacement 0.3 cell, "
+            "seed 4, periodic Ewald")
     if name == "zeldovich256":
         return ics.zeldovich(256, seed=5, n_s=-1.5, rms_cells=0.5), True, (
             "zeldovich 256^3 = 16,777,216 particles, P(k)~k^-1.5 cut at grid Nyquist, rms displacement 0.5 cell, "
@@ -126,6 +130,44 @@ def _input_order(f_tree, order):
     return f
 
 
+def adjoint_leg(s, x, ewald, h, m, forward_ms, steps):
+    """The adjoint of the solve (tg_solver_adjoint) on the same particle set: warm device time per
+    call against the forward's, a directional central difference of the frozen-tree solve, and the
+    difference from the FP64 direct-sum gradient on a 4096-particle subset."""
+    from tigergrav_amd.accuracy import direct_gradient
+
+    N = x.shape[0]
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal((N, 3)).astype(np.float32)
+    s.adjoint(w)  # warm
+    t0 = time.perf_counter()
+    times = []
+    for _ in range(max(1, min(steps, 10))):
+        g, ms = s.adjoint(w, with_time=True)
+        times.append(ms)
+    wall = (time.perf_counter() - t0) * 1e3 / len(times)
+    adj_ms = float(np.mean(times))
+    # directional check on the frozen tree (every particle moved by up to 2^-18 of the box)
+    step = rng.integers(-2 ** 14, 2 ** 14, size=x.shape).astype(np.int64)
+
+    def loss(xx):
+        s.update_particles(xx)
+        s.solve()
+        s.synchronize()
+        return float(np.sum(w * s.forces()[:, 1:]))
+
+    fd = 0.5 * (loss((x.astype(np.int64) + step).astype(np.int32)) - loss((x.astype(np.int64) - step).astype(np.int32)))
+    pred = float(np.sum(g[:, :3] * step * 2.0 ** -32))
+    s.update_particles(x)
+    sub = np.sort(rng.choice(N, min(N, 4096), replace=False))
+    gd = direct_gradient(x, w, sub, ewald, h, m)
+    ex = np.linalg.norm(g[sub, :3] - gd[:, :3], axis=1) / np.linalg.norm(gd[:, :3], axis=1)
+    return {"ms": adj_ms, "wall_ms_per_call": wall, "forward_ms": forward_ms, "ratio_to_forward": adj_ms / forward_ms,
+            "directional_check": {"fd": fd, "adjoint": pred, "rel": abs(fd - pred) / abs(pred)},
+            "vs_direct_gradient_rms_rel": float(np.sqrt(np.mean(ex ** 2))), "subset": int(sub.size),
+            "loss": "sum_i w_i . g_i, w ~ N(0, I), seed 5"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +180,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--accuracy-sample", type=int, default=10000,
                     help="sinks checked against the FP64 direct sum (all of them if >= N)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the adjoint (gradients of sum_i w_i . g_i, w ~ N(0, I), seed 5) against the forward")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run only a plain device-to-device copy of this many bytes (5 times): the reference "
                          "point that scales the HBM byte counters of a profiler run")
@@ -257,6 +301,8 @@ def main():
     acc.update({"reference": "FP64 direct sum over all N sources" + (", converged Ewald sum" if ewald else ""),
                 "sample": f"{ns} sinks (seed 7)", "s": time.perf_counter() - t0})
     out["accuracy"] = acc
+    if args.adjoint:
+        out["adjoint"] = adjoint_leg(s, x, ewald, h, m, ms, args.steps)
     if not args.no_cpu:
         out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
         if f_cpu is not None:

@@ -21,6 +21,12 @@ extern "C" {
  * result for sinks[k]; FP64 throughout.  Returns TG_OK or a negative status. */
 int tg_direct_forces(const tg_params* p, const int32_t* x, const float* mass, int64_t n, const int64_t* sinks, int64_t ns,
 		tg_force* out);
+/* Gradient of the direct-sum loss L = sum_i w_i . g_i (all particles, w: 3 floats each, the same
+ * potential as tg_direct_forces) with respect to the positions and masses of particles sinks[k]:
+ * out[4k..4k+3] = (dL/dx, dL/dy, dL/dz, dL/dm), positions in box units; FP64.  No reference
+ * counterpart: the reference gradient the adjoint of a solve (tg_solver_adjoint) is compared with. */
+int tg_direct_gradient(const tg_params* p, const int32_t* x, const float* mass, int64_t n, const float* w, const int64_t* sinks,
+		int64_t ns, double* out);
 const char* tg_direct_last_error(void);
 
 #ifdef __cplusplus

@@ -68,3 +68,65 @@ def test_adjoint_directional_derivatives(case):
         print(f"  masses {k}: fd {fd:.6e} adjoint {pred:.6e} rel {abs(fd - pred) / abs(pred):.2e}", flush=True)
         assert abs(fd - pred) <= 1e-3 * abs(pred)
     print(f"  adjoint device time {ms:.3f} ms", flush=True)
+
+
+@pytest.mark.parametrize("theta", [0.7, 0.3])
+def test_adjoint_against_direct_sum_gradient(theta):
+    """The solve's adjoint against the FP64 direct-sum gradient of the same loss: they differ by
+    the tree's own approximation error, which shrinks with theta."""
+    from tigergrav_amd.accuracy import direct_gradient
+    from tigergrav_amd.solver import Solver
+
+    n = 4096
+    x = ics.plummer_spheres(n, 93)
+    h, m = ics.soft_len(n), 1.0 / n
+    rng = np.random.default_rng(94)
+    w = rng.standard_normal((n, 3)).astype(np.float32)
+    s = Solver(ewald=True, soft_len=h, m=m, theta=theta, parts_per_node=32, device=0)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    g = s.adjoint(w)
+    pick = np.sort(rng.choice(n, 512, replace=False))
+    gd = direct_gradient(x, w, pick, True, h, m)
+    ex = np.linalg.norm(g[pick, :3] - gd[:, :3], axis=1) / np.linalg.norm(gd[:, :3], axis=1)
+    em = np.abs(g[pick, 3] - gd[:, 3]) / np.abs(gd[:, 3])
+    rx, rm = float(np.sqrt(np.mean(ex ** 2))), float(np.sqrt(np.mean(em ** 2)))
+    print(f"adjoint vs direct gradient theta={theta}: rms rel d/dx {rx:.2e}, d/dm {rm:.2e}", flush=True)
+    assert rx < (0.05 if theta > 0.5 else 0.01)
+
+
+def test_torch_autograd_function():
+    """The torch.autograd.Function routes backward through the adjoint: pos.grad and mass.grad of
+    (w * acc).sum() equal the solver's adjoint, and a directional difference through the function
+    itself (positions moved along a random direction, tree rebuilt each time) agrees with them."""
+    import torch
+
+    from tigergrav_amd.autograd import tree_accelerations
+    from tigergrav_amd.solver import Solver
+
+    n = 2048
+    x = ics.uniform(n, 95)
+    pos0 = (x.astype(np.float64) * 2.0 ** -32 + 0.5)
+    h = ics.soft_len(n)
+    rng = np.random.default_rng(96)
+    w = torch.from_numpy(rng.standard_normal((n, 3)))
+    pos = torch.tensor(pos0, requires_grad=True)
+    mass = torch.full((n,), 1.0 / n, dtype=torch.float64, requires_grad=True)
+    acc = tree_accelerations(pos, mass, ewald=True, soft_len=h, theta=0.7, leaf=32)
+    (w * acc).sum().backward()
+    s = Solver(ewald=True, soft_len=h, m=1.0 / n, theta=0.7, parts_per_node=32, device=0)
+    s.set_particles(ics.double_to_pos(pos0), np.full(n, 1.0 / n, np.float32))
+    s.solve()
+    s.synchronize()
+    g = s.adjoint(w.numpy())
+    np.testing.assert_allclose(pos.grad.numpy(), g[:, :3], rtol=1e-6, atol=1e-9 * np.abs(g[:, :3]).max())
+    np.testing.assert_allclose(mass.grad.numpy(), g[:, 3], rtol=1e-6, atol=1e-9 * np.abs(g[:, 3]).max())
+    # directional check through the function (each call builds its own tree; the step keeps the lists)
+    d = rng.uniform(-1, 1, (n, 3)) * 2.0 ** -20
+    with torch.no_grad():
+        lp = float((w * tree_accelerations(torch.tensor(pos0 + d), mass, True, h, 0.7, 32)).sum())
+        lm = float((w * tree_accelerations(torch.tensor(pos0 - d), mass, True, h, 0.7, 32)).sum())
+    pred = float(np.sum(pos.grad.numpy() * d))
+    print(f"torch directional: fd {0.5 * (lp - lm):.6e} adjoint {pred:.6e}", flush=True)
+    assert abs(0.5 * (lp - lm) - pred) <= 3e-3 * abs(pred) + 2e-5

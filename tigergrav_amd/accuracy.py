@@ -21,6 +21,9 @@ def _bind():
         L.tg_direct_forces.argtypes = [C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                        C.c_void_p]
         L.tg_direct_forces.restype = C.c_int
+        L.tg_direct_gradient.argtypes = [C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                         C.c_int64, C.c_void_p]
+        L.tg_direct_gradient.restype = C.c_int
         L.tg_direct_last_error.restype = C.c_char_p
         _bound = True
     return L
@@ -41,6 +44,25 @@ def direct_forces(x, sinks, ewald, soft_len, m, mass=None, device=0):
     rc = L.tg_direct_forces(C.byref(prm), ptr(x), mp, x.shape[0], ptr(sinks), sinks.size, ptr(out))
     if rc != 0:
         raise TGError(f"tg_direct_forces failed with status {rc}: {L.tg_direct_last_error().decode(errors='replace')}")
+    return out
+
+
+def direct_gradient(x, w, sinks, ewald, soft_len, m, mass=None, device=0):
+    """(len(sinks), 4) array of (dL/dx, dL/dy, dL/dz, dL/dm) at particles `sinks` for the direct-sum
+    loss L = sum_i w_i . g_i over all particles (positions in box units), FP64 on the GPU."""
+    L = _bind()
+    x = np.ascontiguousarray(x, np.int32).reshape(-1, 3)
+    w = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+    sinks = np.ascontiguousarray(sinks, np.int64).reshape(-1)
+    out = np.zeros((sinks.size, 4), np.float64)
+    mp = None
+    if mass is not None:
+        mass = np.ascontiguousarray(mass, np.float32).reshape(-1)
+        mp = ptr(mass)
+    prm = Params(int(bool(ewald)), float(soft_len), float(m), int(device))
+    rc = L.tg_direct_gradient(C.byref(prm), ptr(x), mp, x.shape[0], ptr(w), ptr(sinks), sinks.size, ptr(out))
+    if rc != 0:
+        raise TGError(f"tg_direct_gradient failed with status {rc}: {L.tg_direct_last_error().decode(errors='replace')}")
     return out
 
 

@@ -11,10 +11,10 @@
 //      k_adj_pp(_ewald) tidal tensor J(d) = dF/dd per pair; the lists are symmetric (leaf B is in
 //                      A's PP list iff A is in B's), so each particle's source-side terms are gathered
 //                      from its own list: dL/dx_k += J(d_kj) (m_j w_k - m_k w_j), dL/dm_k += -w_j . F(d_kj)
-//      k_adj_pc        dL/dx_i += w . dg/dx_i (fifth derivatives), mu_c += dg/dM_c^T w
-//   2. k_adj_node, per CC/CP list entry of node a:
-//      CC        mu_c += M2L(., X_a - X_c)^T Lambda_a
-//      CP        dL/dm_j += Lambda_a . D(X_a - x_j), dL/dx_j += -m_j Lambda_a . grad D (fifth derivatives)
+//      k_adj_pc_sink   dL/dx_i += w . dg/dx_i (fifth derivatives); k_adj_pc_src mu_c += dg/dM_c^T w
+//   2. node lists:
+//      k_adj_cc_t  mu_c += M2L(., X_a - X_c)^T Lambda_a, over the CC entries sorted by source c
+//      k_adj_node  CP: dL/dm_j += Lambda_a . D(X_a - x_j), dL/dx_j += -m_j Lambda_a . grad D (fifth derivatives)
 //   3. k_adj_moments, per leaf: mu down the root path through the transposed M2M shifts, then the
 //      transposed P2M onto the leaf's particles.
 // Ewald kinds use the periodic correction's derivatives in the same places.  Per-term arithmetic is
@@ -22,6 +22,7 @@
 // terms are accumulated with FP64 atomics, so their summation order varies between runs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -66,6 +67,10 @@ hipError_t init_adjoint_tables() {
 constexpr int mslot(int K) { return K < 10 ? 1 + (K - 4) : 7 + (K - 10); }
 constexpr float mfac(int K) { return K < 10 ? ef2(K - 4) : -ef3(K - 10); }  // M2P/M2L sign and expansion factor
 
+// forces a constant index to be folded at compile time (the slot searches are loops)
+template <int V>
+constexpr int cx = V;
+
 template <int... S, class F>
 __device__ __forceinline__ void for_seq(std::integer_sequence<int, S...>, F&& f) {
 	(f(std::integral_constant<int, S>{}), ...);
@@ -85,10 +90,11 @@ __device__ __forceinline__ void m2p_sink_adjoint(float* gx, const float* M, cons
 		float acc = 0.0f;
 		for_n<3>([&](auto a_) {
 			constexpr int a = decltype(a_)::value;
-			float t = -M[0] * D[add(add(0, a), b)];
+			float t = -M[0] * D[cx<add(add(0, a), b)>];
 			for_n<16>([&](auto q_) {
 				constexpr int K = 4 + decltype(q_)::value;  // order-2 and order-3 slots 4..19
-				t = fmaf(-mfac(K) * M[mslot(K)], D[add(add(K, a), b)], t);
+				constexpr float f = -mfac(K);
+				t = fmaf(f * M[cx<mslot(K)>], D[cx<add(add(K, a), b)>], t);
 			});
 			acc = fmaf(w[a], t, acc);
 		});
@@ -100,7 +106,7 @@ __device__ __forceinline__ void m2p_source_adjoint(float* mu, const float* D, co
 	float t0 = 0.0f;
 	for_n<3>([&](auto a_) {
 		constexpr int a = decltype(a_)::value;
-		t0 = fmaf(-w[a], D[add(0, a)], t0);
+		t0 = fmaf(-w[a], D[cx<add(0, a)>], t0);
 	});
 	mu[0] += t0;
 	for_n<16>([&](auto q_) {
@@ -108,9 +114,10 @@ __device__ __forceinline__ void m2p_source_adjoint(float* mu, const float* D, co
 		float t = 0.0f;
 		for_n<3>([&](auto a_) {
 			constexpr int a = decltype(a_)::value;
-			t = fmaf(w[a], D[add(K, a)], t);
+			t = fmaf(w[a], D[cx<add(K, a)>], t);
 		});
-		mu[mslot(K)] += -mfac(K) * t;
+		constexpr float f = -mfac(K);
+		mu[cx<mslot(K)>] += f * t;
 	});
 }
 // M2L transposed: mu_K += sum_t Lambda_t dL_t/dM_K for the forward's m2l (L_t += sum_K fac_K M_K D_{t u K}
@@ -127,9 +134,10 @@ __device__ __forceinline__ void m2l_adjoint(float* mu, const float* Lam, const f
 		float acc = 0.0f;
 		for_n<35>([&](auto t_) {
 			constexpr int t = decltype(t_)::value;
-			if constexpr (adj::order_of(t) + adj::order_of(K) <= 4) acc = fmaf(Lam[t], D[merge(t, K)], acc);
+			if constexpr (adj::order_of(t) + adj::order_of(K) <= 4) acc = fmaf(Lam[t], D[cx<merge(t, K)>], acc);
 		});
-		mu[mslot(K)] += mfac(K) * acc;
+		constexpr float f = mfac(K);
+		mu[cx<mslot(K)>] += f * acc;
 	});
 }
 
@@ -414,13 +422,13 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 			float D[10];
 #pragma unroll
 			for (int q = 0; q < 10; q++) D[q] = 0.0f;
-			float dd[6];
-			adj::ewald_self_factors5(double(r2), dd);
+			float dd[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+			adj::ewald_self_factors5<2>(double(r2), dd);
 			adj::radial_add<10>(D, dd, x);
 			for (int i = 0; i < nimg; i++) {
 				const float y[3] = {x[0] - c_adj_real[3 * i], x[1] - c_adj_real[3 * i + 1], x[2] - c_adj_real[3 * i + 2]};
 				const float q2 = fmaf(y[2], y[2], fmaf(y[1], y[1], y[0] * y[0]));
-				adj::ewald_real_factors5(q2, dd);
+				adj::ewald_real_factors5<2>(q2, dd);
 				adj::radial_add<10>(D, dd, y);
 			}
 			const float v[3] = {mj * wi[0] - mi * ws.x, mj * wi[1] - mi * ws.y, mj * wi[2] - mi * ws.z};
@@ -429,7 +437,7 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 				float jv = 0.0f;
 				for_n<3>([&](auto a_) {
 					constexpr int a = decltype(a_)::value;
-					jv = fmaf(-D[add(add(0, a), b)], v[a], jv);
+					jv = fmaf(-D[cx<add(add(0, a), b)>], v[a], jv);
 				});
 				fx[b] += jv;
 			});
@@ -478,23 +486,21 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 	o[3] += gm + double(fgm);
 }
 
-// PC: lane = sink; per multipole of the list the sink side (fifth derivatives) and the moments'
-// adjoint summed over the group's sinks (one FP64 atomic per slot)
+// PC, sink side: lane = sink, over the item's multipoles, dL/dx_i += w . dg/dx_i (fifth derivatives)
 template <bool EWALD>
-__global__ __launch_bounds__(64) void k_adj_pc(AdjArgs A, AdjItems I) {
+__global__ __launch_bounds__(64) void k_adj_pc_sink(AdjArgs A, AdjItems I) {
 	const int4 it = I.items[blockIdx.x];
 	const int t = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
-	const bool sink = t < ns;
-	const int pi = it.x + min(t, ns - 1);
+	if (t >= ns) return;
+	const int pi = it.x + t;
 	const int4 me = A.parts[pi];
 	const float4 wv = A.w[pi];
-	const float wi[3] = {sink ? wv.x : 0.0f, sink ? wv.y : 0.0f, sink ? wv.z : 0.0f};
-	double gx[3] = {0.0, 0.0, 0.0};
+	const float wi[3] = {wv.x, wv.y, wv.z};
+	float gx[3] = {0.f, 0.f, 0.f};
 	const bool wrapx = EWALD || A.wrap;
 	for (int e = it.z; e < it.w; e++) {
-		const int cidx = I.list[e];
-		const MultiSrc& ms = A.multis[cidx];
+		const MultiSrc& ms = A.multis[I.list[e]];
 		float M[17];
 #pragma unroll
 		for (int q = 0; q < 17; q++) M[q] = ms.m[q];
@@ -504,31 +510,48 @@ __global__ __launch_bounds__(64) void k_adj_pc(AdjArgs A, AdjItems I) {
 			adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
 		else
 			adj::green_direct_n<adj::NS5>(D, d);
-		float g3[3] = {0.f, 0.f, 0.f};
-		m2p_sink_adjoint(g3, M, D, wi);
-#pragma unroll
-		for (int b = 0; b < 3; b++) gx[b] += g3[b];
+		m2p_sink_adjoint(gx, M, D, wi);
+	}
+	double* o = A.grad + 4 * int64_t(pi);
+	o[0] += double(gx[0]);
+	o[1] += double(gx[1]);
+	o[2] += double(gx[2]);
+}
+
+// PC, source side: lane = multipole of the item's list, over the item's sinks (staged in LDS):
+// mu_c += dg/dM_c^T w (fourth derivatives), one FP64 atomic per slot and entry
+template <bool EWALD>
+__global__ __launch_bounds__(64) void k_adj_pc_src(AdjArgs A, AdjItems I) {
+	const int4 it = I.items[blockIdx.x];
+	const int t = threadIdx.x;
+	const int ns = it.y & PP_COUNT_MASK;
+	__shared__ int4 sp[64];
+	__shared__ float4 sw[64];
+	if (t < ns) {
+		sp[t] = A.parts[it.x + t];
+		sw[t] = A.w[it.x + t];
+	}
+	__syncthreads();
+	const bool wrapx = EWALD || A.wrap;
+	for (int e = it.z + t; e < it.w; e += 64) {
+		const int cidx = I.list[e];
+		const MultiSrc& ms = A.multis[cidx];
 		float mu[17];
 #pragma unroll
 		for (int q = 0; q < 17; q++) mu[q] = 0.0f;
-		m2p_source_adjoint(mu, D, wi);
-#pragma unroll
-		for (int q = 0; q < 17; q++) {
-			float v = mu[q];
-#pragma unroll
-			for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-			mu[q] = v;
+		for (int k = 0; k < ns; k++) {
+			const int4 me = sp[k];
+			const float wi[3] = {sw[k].x, sw[k].y, sw[k].z};
+			const float d[3] = {sepf(me.x, ms.x[0], wrapx), sepf(me.y, ms.x[1], wrapx), sepf(me.z, ms.x[2], wrapx)};
+			float D[35];
+			if (EWALD)
+				adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
+			else
+				adj::green_direct_n<35>(D, d);
+			m2p_source_adjoint(mu, D, wi);
 		}
-		if (t == 0) {
 #pragma unroll
-			for (int q = 0; q < 17; q++) atomicAdd(A.mu + 17 * int64_t(cidx) + q, double(mu[q]));
-		}
-	}
-	if (sink) {
-		double* o = A.grad + 4 * int64_t(pi);
-		o[0] += gx[0];
-		o[1] += gx[1];
-		o[2] += gx[2];
+		for (int q = 0; q < 17; q++) atomicAdd(A.mu + 17 * int64_t(cidx) + q, double(mu[q]));
 	}
 }
 
@@ -590,13 +613,75 @@ __global__ __launch_bounds__(64) void k_adj_node(AdjNodeArgs A) {
 			gm = fmaf(Lam[t], D[t], gm);
 			for_n<3>([&](auto b_) {
 				constexpr int b = decltype(b_)::value;
-				g3[b] = fmaf(Lam[t], D[add(t, b)], g3[b]);
+				g3[b] = fmaf(Lam[t], D[cx<add(t, b)>], g3[b]);
 			});
 		});
 		double* o = A.grad + 4 * int64_t(src);
 #pragma unroll
 		for (int b = 0; b < 3; b++) atomicAdd(o + b, double(-mj * g3[b]));
 		atomicAdd(o + 3, double(gm));
+	}
+}
+
+// CC by source: the entries (source c, sink node a) sorted by source and cut into waves of <= 64; each
+// lane forms M2L(., X_a - X_c)^T Lambda_a, then 17 lanes add the wave's entries source by source and
+// issue one atomic per source segment and slot
+struct AdjCCArgs {
+	const int2* waves;
+	const int* src;
+	const int* snk;
+	const int4* nodepos;
+	const MultiSrc* multis;
+	const double* Lam;
+	double* mu;
+	int wrap;
+};
+
+template <bool EWALD>
+__global__ __launch_bounds__(64) void k_adj_cc_t(AdjCCArgs A) {
+	const int2 wv = A.waves[blockIdx.x];
+	const int t = threadIdx.x;
+	const int j = wv.x + t;
+	const int cnt = wv.y - wv.x;
+	__shared__ float red[17][65];
+	__shared__ int sc[64];
+	float mu[17];
+#pragma unroll
+	for (int q = 0; q < 17; q++) mu[q] = 0.0f;
+	int c = -1;
+	if (j < wv.y) {
+		c = A.src[j];
+		const int a = A.snk[j];
+		const int4 X = A.nodepos[a];
+		const MultiSrc& ms = A.multis[c];
+		float Lam[35];
+#pragma unroll
+		for (int q = 0; q < 35; q++) Lam[q] = float(A.Lam[35 * int64_t(a) + q]);
+		const bool wrapx = EWALD || A.wrap;
+		const float d[3] = {sepf(X.x, ms.x[0], wrapx), sepf(X.y, ms.x[1], wrapx), sepf(X.z, ms.x[2], wrapx)};
+		float D[35];
+		if (EWALD)
+			adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
+		else
+			adj::green_direct_n<35>(D, d);
+		m2l_adjoint(mu, Lam, D);
+	}
+#pragma unroll
+	for (int q = 0; q < 17; q++) red[q][t] = mu[q];
+	sc[t] = c;
+	__syncthreads();
+	if (t < 17) {
+		int cur = sc[0];
+		double acc = 0.0;
+		for (int k = 0; k < cnt; k++) {
+			if (sc[k] != cur) {
+				atomicAdd(A.mu + 17 * int64_t(cur) + t, acc);
+				cur = sc[k];
+				acc = 0.0;
+			}
+			acc += double(red[t][k]);
+		}
+		if (cnt > 0) atomicAdd(A.mu + 17 * int64_t(cur) + t, acc);
 	}
 }
 
@@ -710,6 +795,34 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 		return TG_EINVAL;
 	}
 	ACHK(hipSetDevice(device));
+	if (!adj_lists_ready) {
+		// CC entries by source: (source, sink node) pairs of the direct and Ewald lists, sorted by source
+		for (int e = 0; e < 2; e++) {
+			const std::vector<int64_t>& ptr = e ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
+			const std::vector<int32_t>& v = e ? tree.lists.cc_e : tree.lists.cc_d;
+			std::vector<std::pair<int32_t, int32_t>> pr;
+			for (size_t a = 0; a + 1 < ptr.size(); a++)
+				for (int64_t q = ptr[a]; q < ptr[a + 1]; q++) pr.emplace_back(v[q], int32_t(a));
+			std::sort(pr.begin(), pr.end());
+			std::vector<int> src(pr.size()), snk(pr.size());
+			std::vector<int2> waves;
+			for (size_t q = 0; q < pr.size(); q++) {
+				src[q] = pr[q].first;
+				snk[q] = pr[q].second;
+			}
+			for (size_t q = 0; q < pr.size(); q += 64) waves.push_back(make_int2(int(q), int(std::min(pr.size(), q + 64))));
+			auto put_v = [&](DeviceArray& d, const void* h, size_t bytes) -> hipError_t {
+				hipError_t r = d.alloc(std::max<size_t>(bytes, 16));
+				if (r != hipSuccess || bytes == 0) return r;
+				return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
+			};
+			ACHK(put_v(d_adj_cct_src[e], src.data(), src.size() * sizeof(int)));
+			ACHK(put_v(d_adj_cct_snk[e], snk.data(), snk.size() * sizeof(int)));
+			ACHK(put_v(d_adj_cct_waves[e], waves.data(), waves.size() * sizeof(int2)));
+			adj_cct_nwaves[e] = int(waves.size());
+		}
+		adj_lists_ready = true;
+	}
 	static thread_local int tables_ready = -1;
 	if (tables_ready != device) {
 		ACHK(init_adjoint_tables());
@@ -764,10 +877,13 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 			hipLaunchKernelGGL(k_adj_pp, g, b, 0, s, a, I);
 		else if (k == PP_EWALD)
 			hipLaunchKernelGGL(k_adj_pp_ewald, g, b, 0, s, a, I);
-		else if (k == PC_DIRECT)
-			hipLaunchKernelGGL(k_adj_pc<false>, g, b, 0, s, a, I);
-		else
-			hipLaunchKernelGGL(k_adj_pc<true>, g, b, 0, s, a, I);
+		else if (k == PC_DIRECT) {
+			hipLaunchKernelGGL(k_adj_pc_sink<false>, g, b, 0, s, a, I);
+			hipLaunchKernelGGL(k_adj_pc_src<false>, g, b, 0, s, a, I);
+		} else {
+			hipLaunchKernelGGL(k_adj_pc_sink<true>, g, b, 0, s, a, I);
+			hipLaunchKernelGGL(k_adj_pc_src<true>, g, b, 0, s, a, I);
+		}
 		ACHK(hipGetLastError());
 	}
 	AdjNodeArgs na{};
@@ -779,19 +895,25 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 	na.mu = d_adj_mu.as<double>();
 	na.grad = d_adj_grad.as<double>();
 	na.wrap = ewald;
-	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
+	for (int e = 0; e < 2; e++) {
+		if (adj_cct_nwaves[e] <= 0) continue;
+		AdjCCArgs ca{d_adj_cct_waves[e].as<int2>(), d_adj_cct_src[e].as<int>(), d_adj_cct_snk[e].as<int>(), d_nodepos.as<int4>(),
+				d_multis.as<MultiSrc>(), d_adj_lam.as<double>(), d_adj_mu.as<double>(), ewald};
+		if (e)
+			hipLaunchKernelGGL(k_adj_cc_t<true>, dim3(adj_cct_nwaves[e]), dim3(64), 0, s, ca);
+		else
+			hipLaunchKernelGGL(k_adj_cc_t<false>, dim3(adj_cct_nwaves[e]), dim3(64), 0, s, ca);
+		ACHK(hipGetLastError());
+	}
+	for (int k : {CP_DIRECT, CP_EWALD}) {
 		if (work[k].nitems <= 0) continue;
 		na.waves = work[k].items.as<int2>();
 		na.list = work[k].list.as<int>();
 		na.owner = work[k].owner.as<int>();
 		na.nwaves = work[k].nitems;
 		const dim3 g(work[k].nitems), b(64);
-		if (k == CC_DIRECT)
-			hipLaunchKernelGGL((k_adj_node<false, true>), g, b, 0, s, na);
-		else if (k == CP_DIRECT)
+		if (k == CP_DIRECT)
 			hipLaunchKernelGGL((k_adj_node<false, false>), g, b, 0, s, na);
-		else if (k == CC_EWALD)
-			hipLaunchKernelGGL((k_adj_node<true, true>), g, b, 0, s, na);
 		else
 			hipLaunchKernelGGL((k_adj_node<true, false>), g, b, 0, s, na);
 		ACHK(hipGetLastError());

@@ -127,6 +127,7 @@ __device__ __forceinline__ void green_direct_n(float* D, const float* x) {
 
 // radial factors d0..d5 of -erfc(2r)/r:
 //   d_k = -(-1)^k (2k-1)!! r^-(2k+1) [erfc(2r) + (4/sqrt(pi)) e^(-4r^2) sum_{j=1..k} 8^(j-1) r^(2j-1) / (2j-1)!!]
+template <int K = 5>
 __device__ __forceinline__ void ewald_real_factors5(float r2, float* d) {
 	const float ri = __builtin_amdgcn_rsqf(r2);
 	const float r = r2 * ri;
@@ -137,7 +138,7 @@ __device__ __forceinline__ void ewald_real_factors5(float r2, float* d) {
 	float sign = -1.0f;
 	d[0] = -ec * ri;
 #pragma unroll
-	for (int k = 1; k <= 5; k++) {
+	for (int k = 1; k <= K; k++) {
 		sum += term;
 		term *= 8.0f * r2 / float(2 * k + 1);
 		dfact *= float(2 * k - 1);
@@ -150,9 +151,10 @@ __device__ __forceinline__ void ewald_real_factors5(float r2, float* d) {
 // FP64 series of erf(2r)/r in u = r^2 and its radial factors to order 5 (host fills the table)
 #define TG_ADJ_NSER 32
 extern __constant__ double c_adj_ser[6 * TG_ADJ_NSER];
+template <int K = 5>
 __device__ __forceinline__ void ewald_self_factors5(double u, float* d) {
 #pragma unroll
-	for (int k = 0; k <= 5; k++) {
+	for (int k = 0; k <= K; k++) {
 		double acc = c_adj_ser[k * TG_ADJ_NSER + TG_ADJ_NSER - 1];
 #pragma unroll
 		for (int q = TG_ADJ_NSER - 2; q >= 0; q--) acc = fma(acc, u, c_adj_ser[k * TG_ADJ_NSER + q]);
@@ -195,13 +197,14 @@ __device__ __forceinline__ void green_ewald_n(float* D, const float* x, const fl
 		D[0] = 2.8372975f;
 		return;
 	}
-	float d[6];
-	ewald_self_factors5(double(r2), d);
+	constexpr int K = NS <= 1 ? 0 : (NS <= 4 ? 1 : (NS <= 10 ? 2 : (NS <= 20 ? 3 : (NS <= 35 ? 4 : 5))));
+	float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+	ewald_self_factors5<K>(double(r2), d);
 	radial_add<NS>(D, d, x);
 	for (int i = 0; i < TG_NREAL; i++) {
 		const float dx[3] = {x[0] - real[3 * i], x[1] - real[3 * i + 1], x[2] - real[3 * i + 2]};
 		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
-		ewald_real_factors5(q2, d);
+		ewald_real_factors5<K>(q2, d);
 		radial_add<NS>(D, d, dx);
 	}
 	for (int i = 0; i < TG_NFOUR; i++) {

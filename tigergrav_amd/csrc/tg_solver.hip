@@ -201,6 +201,11 @@ Solver::~Solver() {
 		d_adj_grad.release();
 		d_adj_lam.release();
 		d_adj_mu.release();
+		for (int e = 0; e < 2; e++) {
+			d_adj_cct_src[e].release();
+			d_adj_cct_snk[e].release();
+			d_adj_cct_waves[e].release();
+		}
 		d_leaf_items.release();
 		d_slot_node.release();
 		d_path.release();
@@ -546,6 +551,7 @@ int Solver::upload() {
 	SCHK(put(d_leaf_items, leaf_items, s));
 	SCHK(put(d_leaf_lists, leaf_lists, s));
 	adj_checked = false;
+	adj_lists_ready = false;
 	SCHK(put(d_slot_node, slot_node, s));
 	SCHK(put(d_path, path, s));
 	SCHK(put(d_level, level, s));

@@ -51,6 +51,10 @@ struct Solver {
 	bool adj_pp_symmetric = false;  // checked once per tree at the first adjoint call
 	bool adj_checked = false;
 	hipEvent_t adj_ev[2] = {nullptr, nullptr};
+	// CC entries by source for the adjoint (direct, Ewald): built at the first adjoint call per tree
+	DeviceArray d_adj_cct_src[2], d_adj_cct_snk[2], d_adj_cct_waves[2];
+	int adj_cct_nwaves[2] = {0, 0};
+	bool adj_lists_ready = false;
 	KindWork work[NKINDS];
 	// host-only solvers (device < 0) keep the PP-direct items and flattened source list
 	std::vector<int4> host_pp_items;
