@@ -130,6 +130,17 @@ def _input_order(f_tree, order):
     return f
 
 
+def device_free_bytes():
+    """Free device memory on device 0 (hipMemGetInfo)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        return 0
+    return int(free.value)
+
+
 def adjoint_leg(s, x, ewald, h, m, forward_ms, steps):
     """The adjoint of the solve (tg_solver_adjoint) on the same particle set: warm device time per
     call against the forward's, a directional central difference of the frozen-tree solve, and the
@@ -216,6 +227,7 @@ def main():
 
         best, tuning = autotune(x, ewald, h, m, args.theta)
         args.leaf = best["leaf"]
+    free0 = device_free_bytes()
     s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
     if best is not None:
         from tigergrav_amd.autotune import apply
@@ -301,8 +313,14 @@ def main():
     acc.update({"reference": "FP64 direct sum over all N sources" + (", converged Ewald sum" if ewald else ""),
                 "sample": f"{ns} sinks (seed 7)", "s": time.perf_counter() - t0})
     out["accuracy"] = acc
+    # device memory held by the solver (all of it stays allocated between solves): resident data
+    # after the forward, and after the adjoint's buffers are added
+    fp = {"forward_bytes": free0 - device_free_bytes()}
     if args.adjoint:
         out["adjoint"] = adjoint_leg(s, x, ewald, h, m, ms, args.steps)
+        fp["forward_plus_adjoint_bytes"] = free0 - device_free_bytes()
+    fp["method"] = "hipMemGetInfo difference from before the solver was created (the accuracy checks free their buffers)"
+    out["hbm_footprint"] = fp
     if not args.no_cpu:
         out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
         if f_cpu is not None:

@@ -16,11 +16,23 @@ from tigergrav_amd import ics
 
 pytestmark = pytest.mark.gpu
 
+def _edge(x, k=64, seed=85):
+    """k particles moved to within 2^12 position units of the periodic box edge, so that the
+    directional steps carry some of them across it (the frozen tree must stay continuous there)."""
+    x = x.copy()
+    r = np.random.default_rng(seed)
+    for d in range(3):
+        sel = r.choice(x.shape[0], k, replace=False)
+        x[sel, d] = (2 ** 31 - 2 ** 12 + r.integers(0, 2 ** 13, k)).astype(np.int64).astype(np.int32)
+    return x
+
+
 CASES = {
     # name: (generator, ewald, theta, leaf, position step in 2^-32 box units)
     "uniform_periodic": (lambda: ics.uniform(4096, 81), True, 0.7, 32, 2 ** 14),
     "plummer_open": (lambda: ics.plummer_spheres(8192, 82), False, 0.7, 64, 2 ** 14),
     "zeldovich_periodic": (lambda: ics.zeldovich(16, 83), True, 0.5, 64, 2 ** 18),  # near a lattice: small forces
+    "box_edge_periodic": (lambda: _edge(ics.uniform(4096, 84)), True, 0.7, 32, 2 ** 14),
 }
 
 

@@ -257,13 +257,15 @@ def test_update_particles_keeps_tree_and_recomputes_moments():
     assert np.array_equal(s.export("node_m"), m0)
     rng = np.random.default_rng(72)
     x2 = (x.astype(np.int64) + rng.integers(-2 ** 20, 2 ** 20, size=x.shape)).astype(np.int32)
+    x2[:8, 0] = np.int32(2 ** 31 - 1)  # at the box edge ...
+    x2[8:16, 0] = np.int32(-2 ** 31)   # ... and across it
     mass = rng.uniform(0.5, 1.5, n).astype(np.float32) / n
     s.update_particles(x2, mass)
     for k, v in lists.items():
         assert np.array_equal(s.export(k), v)
     assert np.array_equal(s.export("node_x"), xc)
     M = s.export("node_m").astype(np.float64)
-    pos = s.export("pos").astype(np.float64) * 2.0 ** -32
+    posi = s.export("pos").astype(np.int64)
     order = s.export("order")
     mt = mass[order].astype(np.float64)
     nr, ch = s.export("node_range"), s.export("node_child")
@@ -275,7 +277,8 @@ def test_update_particles_keeps_tree_and_recomputes_moments():
         if e == b:
             continue
         if ch[nd, 0] < 0:
-            d = pos[b:e] - X[nd]
+            # periodic: offsets from the kept centre wrap around the box (some particles crossed its edge)
+            d = (((posi[b:e] - xc[nd].astype(np.int64) + 2 ** 31) % 2 ** 32) - 2 ** 31) * 2.0 ** -32
             want = [mt[b:e].sum()] + [np.sum(mt[b:e] * d[:, p] * d[:, q]) for p, q in i2] + \
                    [np.sum(mt[b:e] * d[:, p] * d[:, q] * d[:, r]) for p, q, r in i3]
         else:

@@ -6,8 +6,9 @@
 // expansion and moment, so the adjoint runs the solve's operators transposed, recomputing every
 // interaction from the stored lists (nothing of the forward is kept but its expansions dL):
 //   1. sinks, over the forward's leaf work items:
-//      k_adj_l2p       g = -grad T_L(x - X):   dL/dx += -w^T Hess T_L,  lambda_leaf += dg/dL^T w;
-//                      Lambda_a += S(X_leaf - X_a)^T lambda_leaf for every node a on the root path
+//      k_adj_l2p       g = -grad T_L(x - X):   dL/dx += -w^T Hess T_L,  lambda_leaf = dg/dL^T w;
+//      k_adj_up        Lambda_a = sum over leaves B below a of S(X_B - X_a)^T lambda_B (L2L transposed),
+//                      level by level from the children's
 //      k_adj_pp(_ewald) tidal tensor J(d) = dF/dd per pair; the lists are symmetric (leaf B is in
 //                      A's PP list iff A is in B's), so each particle's source-side terms are gathered
 //                      from its own list: dL/dx_k += J(d_kj) (m_j w_k - m_k w_j), dL/dm_k += -w_j . F(d_kj)
@@ -18,8 +19,8 @@
 //   3. k_adj_moments, per leaf: mu down the root path through the transposed M2M shifts, then the
 //      transposed P2M onto the leaf's particles.
 // Ewald kinds use the periodic correction's derivatives in the same places.  Per-term arithmetic is
-// FP32 like the forward; sums are FP64.  Node-indexed adjoints (Lambda, mu) and the CP source
-// terms are accumulated with FP64 atomics, so their summation order varies between runs.
+// FP32 like the forward; sums are FP64.  The multipole adjoints mu and the PC/CP source terms are
+// accumulated with FP64 atomics, so their summation order varies between runs; Lambda is not.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -200,13 +201,11 @@ __global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
 	double* lam64 = lds;
 	double* Ls = lds + 35 * 64;
 	double* sh = Ls + 35;
-	__shared__ int anc[64];
 	const int4 c = A.nodepos[lf.x];
 	// the leaf's local expansion, as the forward forms it (sum over the path of shifted dL)
 	double off[3] = {0.0, 0.0, 0.0};
 	if (t < depth) {
 		const int a = A.path[lf.w + t];
-		anc[t] = a;
 		const int4 X = A.nodepos[a];
 		off[0] = double(int64_t(c.x) - int64_t(X.x)) * POS_INV;
 		off[1] = double(int64_t(c.y) - int64_t(X.y)) * POS_INV;
@@ -232,9 +231,9 @@ __global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
 	const float wi[3] = {sink ? wv.x : 0.0f, sink ? wv.y : 0.0f, sink ? wv.z : 0.0f};
 	double gx[3] = {0.0, 0.0, 0.0};
 	{
-		const double d[3] = {double(float(double(int64_t(me.x) - int64_t(c.x)) * POS_INV)),
-				double(float(double(int64_t(me.y) - int64_t(c.y)) * POS_INV)),
-				double(float(double(int64_t(me.z) - int64_t(c.z)) * POS_INV))};
+		const double d[3] = {double(float(double(centre_offset(me.x, c.x, A.wrap)) * POS_INV)),
+				double(float(double(centre_offset(me.y, c.y, A.wrap)) * POS_INV)),
+				double(float(double(centre_offset(me.z, c.z, A.wrap)) * POS_INV))};
 		double W[35];
 		taylor_weights(W, d);
 		// g_a = -sum_{|s|<=3} L_{s u a} W_s  ->  dg_a/dd_b = -sum_{|s|<=2} L_{s u a u b} W_s
@@ -265,29 +264,11 @@ __global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
 		for (int q = 0; q < 35; q++) lam64[q * 64 + t] = lam[q];
 	}
 	__syncthreads();
+	// lambda of the leaf; k_adj_up carries it to the ancestors
 	if (t < 35) {
 		double acc = 0.0;
 		for (int k = 0; k < ns; k++) acc += lam64[t * 64 + k];
-		Ls[t] = acc;  // now lambda of the leaf
-	}
-	__syncthreads();
-	// Lambda_a += S(X_leaf - X_a)^T lambda for every node on the path: (S^T l)_u = sum_{t u s = u} l_t W_s(off)
-	if (t < depth) {
-		double W[35], out[35];
-		taylor_weights(W, off);
-#pragma unroll
-		for (int q = 0; q < 35; q++) out[q] = 0.0;
-		for_n<35>([&](auto t_) {
-			constexpr int tt = decltype(t_)::value;
-			const double lt = Ls[tt];
-			for_n<35>([&](auto s_) {
-				constexpr int s = decltype(s_)::value;
-				if constexpr (Merge<tt, s>::valid) out[Merge<tt, s>::value] = fma(lt, W[s], out[Merge<tt, s>::value]);
-			});
-		});
-		double* dst = A.Lam + 35 * int64_t(anc[t]);
-#pragma unroll
-		for (int q = 0; q < 35; q++) atomicAdd(dst + q, out[q]);
+		A.Lam[35 * int64_t(lf.x) + t] = acc;
 	}
 	if (sink) {
 		double* o = A.grad + 4 * int64_t(pi);
@@ -685,6 +666,46 @@ __global__ __launch_bounds__(64) void k_adj_cc_t(AdjCCArgs A) {
 	}
 }
 
+// ---- up-sweep of Lambda ------------------------------------------------------------------------------
+// The forward gives leaf B the expansion sum_a S(X_B - X_a) dL_a over its path, so dL/d(dL_a) =
+// sum over leaves B below a of S(X_B - X_a)^T lambda_B.  Shifts compose exactly
+// (S(X_B - X_a) = S(X_B - X_c) S(X_c - X_a) for a child c of a on the way), hence level by level,
+// deepest first:  Lambda_a = sum_children S(X_c - X_a)^T Lambda_c,  with Lambda_B = lambda_B at the
+// leaves.  One lane per (node, child); the two children of a node sit on adjacent lanes and are
+// added with one exchange.  No atomics: every Lambda is written once.
+__global__ __launch_bounds__(64) void k_adj_up(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ nodepos,
+		double* __restrict__ Lam) {
+	const int j = blockIdx.x * 64 + threadIdx.x;
+	const int2 pc = j < npairs ? pairs[j] : make_int2(-1, -1);
+	double out[35];
+#pragma unroll
+	for (int q = 0; q < 35; q++) out[q] = 0.0;
+	if (pc.y >= 0) {
+		const int4 Xa = nodepos[pc.x], Xc = nodepos[pc.y];
+		// (S^T l)_u = sum_{t u s = u} l_t W_s(X_c - X_a), W the Taylor weights of the offset
+		const double off[3] = {double(int64_t(Xc.x) - int64_t(Xa.x)) * POS_INV, double(int64_t(Xc.y) - int64_t(Xa.y)) * POS_INV,
+				double(int64_t(Xc.z) - int64_t(Xa.z)) * POS_INV};
+		double W[35];
+		taylor_weights(W, off);
+		const double* lc = Lam + 35 * int64_t(pc.y);
+		for_n<35>([&](auto t_) {
+			constexpr int tt = decltype(t_)::value;
+			const double lt = lc[tt];
+			for_n<35>([&](auto s_) {
+				constexpr int s = decltype(s_)::value;
+				if constexpr (Merge<tt, s>::valid) out[Merge<tt, s>::value] = fma(lt, W[s], out[Merge<tt, s>::value]);
+			});
+		});
+	}
+#pragma unroll
+	for (int q = 0; q < 35; q++) out[q] += __shfl_xor(out[q], 1, 64);
+	if ((j & 1) == 0 && j < npairs) {
+		double* dst = Lam + 35 * int64_t(pc.x);
+#pragma unroll
+		for (int q = 0; q < 35; q++) dst[q] = out[q];
+	}
+}
+
 // ---- 3. moments ----------------------------------------------------------------------------------
 // multipole<float>::operator>>= transposed (tigergrav/multipole.hpp:414-433, restated in the host
 // tree): M3'_pql = M3 + M0 Y_p Y_q Y_l + M2_pq Y_l + M2_ql Y_p + M2_lp Y_q, M2'_pq = M2 + M0 Y_p Y_q
@@ -738,8 +759,8 @@ __global__ __launch_bounds__(64) void k_adj_moments(AdjArgs A) {
 	const int4 c = A.nodepos[lf.x];
 	const float mj = __int_as_float(me.w);
 	// P2M (part_vect_multipole_info): M2_pq += m d_p d_q, M3_pql += m d_p d_q d_l, d = float(x - X)
-	const float d[3] = {float(double(int64_t(me.x) - int64_t(c.x)) * POS_INV), float(double(int64_t(me.y) - int64_t(c.y)) * POS_INV),
-			float(double(int64_t(me.z) - int64_t(c.z)) * POS_INV)};
+	const float d[3] = {float(double(centre_offset(me.x, c.x, A.wrap)) * POS_INV), float(double(centre_offset(me.y, c.y, A.wrap)) * POS_INV),
+			float(double(centre_offset(me.z, c.z, A.wrap)) * POS_INV)};
 	float gm = mus[0], g3[3] = {0.f, 0.f, 0.f};
 #pragma unroll
 	for (int s = 0; s < 6; s++) {
@@ -821,6 +842,24 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 			ACHK(put_v(d_adj_cct_waves[e], waves.data(), waves.size() * sizeof(int2)));
 			adj_cct_nwaves[e] = int(waves.size());
 		}
+		// (node, child) pairs of the internal nodes, deepest level first; an empty child gets -1
+		std::vector<std::vector<int2>> by_level(size_t(tree.max_level) + 1);
+		for (size_t i = 0; i < tree.nodes.size(); i++) {
+			const Node& nd = tree.nodes[i];
+			if (nd.leaf()) continue;
+			for (int c = 0; c < 2; c++) {
+				const int ch = nd.child[c];
+				by_level[size_t(nd.level)].push_back(make_int2(int(i), ch >= 0 && !tree.nodes[size_t(ch)].empty() ? ch : -1));
+			}
+		}
+		std::vector<int2> pairs;
+		adj_up_ptr.assign(1, 0);
+		for (int l = tree.max_level; l >= 0; l--) {
+			pairs.insert(pairs.end(), by_level[size_t(l)].begin(), by_level[size_t(l)].end());
+			if (int(pairs.size()) > adj_up_ptr.back()) adj_up_ptr.push_back(int(pairs.size()));
+		}
+		ACHK(d_adj_up.alloc(std::max<size_t>(pairs.size() * sizeof(int2), 16)));
+		if (!pairs.empty()) ACHK(hipMemcpy(d_adj_up.p, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
 		adj_lists_ready = true;
 	}
 	static thread_local int tables_ready = -1;
@@ -867,6 +906,12 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 	const size_t lds = size_t(35 * 64 + 35 + 35 * a.maxdepth) * sizeof(double);
 	if (n_leaf_items > 0) {
 		hipLaunchKernelGGL(k_adj_l2p, dim3(n_leaf_items), dim3(64), lds, s, a);
+		ACHK(hipGetLastError());
+	}
+	for (size_t l = 0; l + 1 < adj_up_ptr.size(); l++) {
+		const int b = adj_up_ptr[l], np = adj_up_ptr[l + 1] - b;
+		hipLaunchKernelGGL(k_adj_up, dim3(unsigned((np + 63) / 64)), dim3(64), 0, s, d_adj_up.as<int2>() + b, np, d_nodepos.as<int4>(),
+				d_adj_lam.as<double>());
 		ACHK(hipGetLastError());
 	}
 	for (int k : {PP_DIRECT, PP_EWALD, PC_DIRECT, PC_EWALD}) {

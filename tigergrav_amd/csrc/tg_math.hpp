@@ -36,6 +36,13 @@ __host__ __device__ constexpr int s3(int a, int b, int c) {
 __host__ __device__ constexpr int s4sorted(int a, int b, int c, int d) {
 	return a == 0 ? s3sorted(b, c, d) : (a == 1 ? (b == 1 ? (c == 1 ? (d == 1 ? 10 : 11) : 12) : 13) : 14);
 }
+// a - b in position units (2^32 per box) between a particle and an expansion centre: the int32
+// wrap-around difference when the box is periodic, else the exact difference.  The two agree for a
+// particle inside its node; a particle moved across the box edge with the tree held fixed keeps a
+// continuous offset only through the first.
+__host__ __device__ inline int64_t centre_offset(int32_t a, int32_t b, bool wrap) {
+	return wrap ? int64_t(int32_t(uint32_t(a) - uint32_t(b))) : int64_t(a) - int64_t(b);
+}
 __host__ __device__ constexpr int min2(int a, int b) { return a < b ? a : b; }
 __host__ __device__ constexpr int max2(int a, int b) { return a < b ? b : a; }
 __host__ __device__ constexpr int s4(int a, int b, int c, int d) {

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_node_sum(const int64_t* __restrict__ sl
 static_assert(TG_MAXDEPTH <= 64, "one lane per path entry");
 __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, const int* __restrict__ path,
 		const int* __restrict__ level, const int4* __restrict__ pos, const int4* __restrict__ parts,
-		const double* __restrict__ dL, double* __restrict__ out, double phi_self, int maxdepth) {
+		const double* __restrict__ dL, double* __restrict__ out, double phi_self, int maxdepth, int wrap) {
 	const int4 lf = leaves[blockIdx.x];
 	const int t = threadIdx.x;
 	const int depth = level[lf.x] + 1;
@@ -96,9 +96,9 @@ __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, c
 	for (int i = lf.y + t; i < lf.z; i += 64) {
 		const int4 q = parts[i];
 		// the reference rounds the offset from the leaf centre to float (src/tree.cpp:607)
-		const double d[3] = {double(float(double(int64_t(q.x) - int64_t(c.x)) * POS_INV)),
-				double(float(double(int64_t(q.y) - int64_t(c.y)) * POS_INV)),
-				double(float(double(int64_t(q.z) - int64_t(c.z)) * POS_INV))};
+		const double d[3] = {double(float(double(centre_offset(q.x, c.x, wrap)) * POS_INV)),
+				double(float(double(centre_offset(q.y, c.y, wrap)) * POS_INV)),
+				double(float(double(centre_offset(q.z, c.z, wrap)) * POS_INV))};
 		double phi, g[3];
 		l2p(phi, g, Ls, d);
 		double* o = out + 4 * size_t(i);
@@ -201,6 +201,7 @@ Solver::~Solver() {
 		d_adj_grad.release();
 		d_adj_lam.release();
 		d_adj_mu.release();
+		d_adj_up.release();
 		for (int e = 0; e < 2; e++) {
 			d_adj_cct_src[e].release();
 			d_adj_cct_snk[e].release();
@@ -623,7 +624,7 @@ int Solver::enqueue_solve(bool timed) {
 		SCHK(hipGetLastError());
 		hipLaunchKernelGGL(k_local, dim3(n_leaf_items), dim3(64), local_lds_bytes(maxdepth), s, d_leaf_items.as<int4>(),
 				d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(), d_dL.as<double>(),
-				d_out.as<double>(), phi_self, maxdepth);
+				d_out.as<double>(), phi_self, maxdepth, ewald);
 		SCHK(hipGetLastError());
 	}
 	SCHK(mark(e + 1));

@@ -54,6 +54,10 @@ struct Solver {
 	// CC entries by source for the adjoint (direct, Ewald): built at the first adjoint call per tree
 	DeviceArray d_adj_cct_src[2], d_adj_cct_snk[2], d_adj_cct_waves[2];
 	int adj_cct_nwaves[2] = {0, 0};
+	// (node, child) pairs for the Lambda up-sweep, grouped by level deepest first: batch l is
+	// [adj_up_ptr[l], adj_up_ptr[l + 1])
+	DeviceArray d_adj_up;
+	std::vector<int> adj_up_ptr;
 	bool adj_lists_ready = false;
 	KindWork work[NKINDS];
 	// host-only solvers (device < 0) keep the PP-direct items and flattened source list

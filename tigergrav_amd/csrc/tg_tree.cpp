@@ -184,23 +184,19 @@ void Tree::update_moments(const int32_t* pos_tree, const float* mass_tree) {
 		if (nd.empty()) continue;  // zero moments, as built
 		if (nd.leaf()) {
 			// the P2M of part_vect_multipole_info about the kept centre, 16 float lanes
-			double Xcom[3];
-			for (int d = 0; d < 3; d++) Xcom[d] = pos_to_double(nd.x[d]);
 			float M[17][16];
 			std::memset(M, 0, sizeof(M));
 			for (int64_t i = nd.pbegin; i < nd.pend; i += 16) {
 				for (int k = 0; k < 16; k++) {
-					double X[3];
-					float ms;
+					// offset from the kept centre; periodic: the wrap-around difference, so a particle
+					// moved across the box edge stays next to its node
+					float dx[3] = {0.0f, 0.0f, 0.0f};
+					float ms = 0.0f;
 					if (i + k < nd.pend) {
-						for (int d = 0; d < 3; d++) X[d] = pos_to_double(pos[3 * (i + k) + d]);
+						for (int d = 0; d < 3; d++)
+							dx[d] = float(double(centre_offset_host(pos[3 * (i + k) + d], nd.x[d], P.ewald != 0)) * (1.0 / 4294967296.0));
 						ms = mass[i + k];
-					} else {
-						X[0] = X[1] = X[2] = 0.0;
-						ms = 0.0f;
 					}
-					float dx[3];
-					for (int d = 0; d < 3; d++) dx[d] = float(X[d] - Xcom[d]);
 					M[0][k] += ms;
 					for (int j = 0; j < 3; j++)
 						for (int kk = 0; kk <= j; kk++) {

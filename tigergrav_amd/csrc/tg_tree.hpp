@@ -91,6 +91,11 @@ private:
 
 // reference conversions (tigergrav/position.hpp:32-51)
 inline double pos_to_double(int32_t x) { return double(x) * (1.0 / 4294967296.0) + 0.5; }
+// a - b in position units between a particle and a node centre: the int32 wrap-around difference
+// when periodic, else the exact difference (equal for a particle inside the node)
+inline int64_t centre_offset_host(int32_t a, int32_t b, bool wrap) {
+	return wrap ? int64_t(int32_t(uint32_t(a) - uint32_t(b))) : int64_t(a) - int64_t(b);
+}
 int32_t double_to_pos(double x);
 // box_id_to_range (src/range.cpp:134-169)
 void box_id_to_range(boxid_t id, double* mn, double* mx);
