@@ -1,15 +1,19 @@
 """Benchmark: periodic force solve of 2,097,152 particles on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload zeldovich128] [--theta 0.7] [--leaf 64]
+    python bench.py [--steps K] [--warmup W] [--workload zeldovich128] [--theta 0.7] [--leaf 0] [--no-graph]
 
 Workload (default): 128^3 particles displaced by a Zel'dovich field of a seeded Gaussian random
 field with P(k) ~ k^-2 (cut at the grid Nyquist), rms displacement 0.3 cell, periodic (Ewald),
-theta = 0.7, leaf size 64, softening 0.02 N^(-1/3), equal masses 1/N.
+theta = 0.7, softening 0.02 N^(-1/3), equal masses 1/N.  By default the leaf size (32/64/128), the
+CC/CP chunk and the PP kernel form are autotuned on this workload's own device time (--leaf N
+fixes the leaf size instead), and the timed steps replay the solve from a captured HIP graph.
 
 One step = one force solve on the device over the resident tree and interaction lists: the CC/CP
 node kernels, the local-expansion pass (reduce + L2L + L2P) and the PC/PP leaf kernels.  The tree,
 multipoles and lists are built once on the host before the timed region (their time is reported
-as the end-to-end part of `s_per_force_solve_end_to_end`).
+as the end-to-end part of `s_per_force_solve_end_to_end`).  Per-kind kernel times (and the
+roofline) come from HIP events on the solver's stream around each kind's kernels, over K eager
+solves run right after the graph-replayed ones (a captured graph carries no timers).
 
 Prints one JSON line.  `value` is interactions per second (the reference's interaction census,
 src/tree.cpp:483-494,618-628, divided by the device time of one solve).
@@ -186,9 +190,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="zeldovich128")
     ap.add_argument("--theta", type=float, default=0.7)
-    ap.add_argument("--leaf", type=int, default=64,
+    ap.add_argument("--leaf", type=int, default=0,
                     help="particles per leaf; 0 autotunes leaf (32, 64, 128), CC/CP chunk and the PP kernel form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--pp-persistent", type=int, default=None, choices=(0, 1),
+                    help="with a fixed --leaf: the persistent (1) or one-group-per-item (0) PP kernel")
+    ap.add_argument("--node-chunk", type=int, default=None, help="with a fixed --leaf: CC/CP list entries per partial slot")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager solves (with per-kind event timers) instead of HIP-graph replays")
     ap.add_argument("--accuracy-sample", type=int, default=10000,
                     help="sinks checked against the FP64 direct sum (all of them if >= N)")
     ap.add_argument("--adjoint", action="store_true",
@@ -225,7 +234,8 @@ def main():
         # autotune leaf size, CC/CP chunk and PP kernel form on this very workload's device time
         from tigergrav_amd.autotune import autotune
 
-        best, tuning = autotune(x, ewald, h, m, args.theta)
+        # the adjoint handles leaves of at most 64 particles
+        best, tuning = autotune(x, ewald, h, m, args.theta, leaves=(32, 64) if args.adjoint else (32, 64, 128))
         args.leaf = best["leaf"]
     free0 = device_free_bytes()
     s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
@@ -233,6 +243,11 @@ def main():
         from tigergrav_amd.autotune import apply
 
         apply(s, best)
+    else:
+        if args.pp_persistent is not None:
+            s.tune("pp_persistent", args.pp_persistent)
+        if args.node_chunk is not None:
+            s.tune("node_chunk", args.node_chunk)
     t0 = time.time()
     s.set_particles(x)
     t_host = time.time() - t0
@@ -240,20 +255,28 @@ def main():
     count = info["count"]
     total = sum(count.values())
 
+    step = s.solve if args.no_graph else s.solve_graph
     for _ in range(args.warmup):
-        s.solve()
+        step()
     s.synchronize()
     s.collect_ms()  # drop the warmup solves' event times
-    # timed region: K solves back to back on the resident data; every solve records HIP events
-    # around each kind's kernels on the solver's stream
+    # timed region: K solves back to back on the resident data (graph replays by default)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        s.solve()
+        step()
     s.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     value = total / (ms / 1e3)
-    kms, nsolves = s.collect_ms()  # mean per-solve device time of each kind over the timed solves
+    ms_eager = None
+    if not args.no_graph:
+        # the same K solves eagerly, each recording HIP events around each kind's kernels
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s.solve()
+        s.synchronize()
+        ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
+    kms, nsolves = s.collect_ms()  # mean per-solve device time of each kind over the eager solves
     assert nsolves == args.steps
 
     dom = max(KINDS, key=lambda k: kms[k])
@@ -280,7 +303,7 @@ def main():
     if os.path.exists(traffic_file):
         try:
             tr = json.load(open(traffic_file))
-            key = f"{args.workload}:{dom}"
+            key = f"{args.workload}:leaf{args.leaf}:{dom}"
             if key in tr:
                 roofline["traffic"] = tr[key]
         except Exception:
@@ -291,8 +314,12 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "N": N, "theta": args.theta, "leaf": args.leaf, "ewald": ewald, "softening": h,
+                   "pp_persistent": best["pp_persistent"] if best else args.pp_persistent,
+                   "node_chunk": best["node_chunk"] if best else args.node_chunk,
                    "accumulation": "f64", "parallelism": "single GPU"},
         "s_per_force_solve": ms / 1e3,
+        "step": "eager solves" if args.no_graph else "HIP-graph replays",
+        "ms_per_step_eager": ms_eager,
         "s_per_force_solve_end_to_end": t_host + ms / 1e3,
         "host_build_s": {"tree": info["t_tree"], "multipoles": info["t_multipoles"], "lists": info["t_walk"],
                          "upload": info["t_upload"]},
@@ -317,7 +344,8 @@ def main():
     # after the forward, and after the adjoint's buffers are added
     fp = {"forward_bytes": free0 - device_free_bytes()}
     if args.adjoint:
-        out["adjoint"] = adjoint_leg(s, x, ewald, h, m, ms, args.steps)
+        out["adjoint"] = adjoint_leg(s, x, ewald, h, m, ms, args.steps) if args.leaf <= 64 else {
+            "skipped": "the adjoint handles leaves of at most 64 particles"}
         fp["forward_plus_adjoint_bytes"] = free0 - device_free_bytes()
     fp["method"] = "hipMemGetInfo difference from before the solver was created (the accuracy checks free their buffers)"
     out["hbm_footprint"] = fp

@@ -294,8 +294,16 @@ __global__ __launch_bounds__(64) void k_adj_pp(AdjArgs A, AdjItems I) {
 	const int4 it = I.items[blockIdx.x];
 	const int t = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
-	const bool sink = t < ns;
-	const int pi = it.x + min(t, ns - 1);
+	// sinks padded to a power of two nsp; the P = 64 / nsp lane groups (phases) split each tile's
+	// sources, so leaves of 32 or fewer particles keep every lane busy
+	const int lg = ns <= 1 ? 0 : 32 - __clz(ns - 1);
+	const int nsp = 1 << lg;
+	const int P = 64 >> lg;
+	const int si = t & (nsp - 1);
+	const int ph = t >> lg;
+	const int near_len = int(unsigned(it.y) >> PP_NEAR_SHIFT);
+	const int near_end = near_len == PP_NEAR_ALL ? INT_MAX : it.z + near_len;
+	const int pi = it.x + min(si, ns - 1);
 	const int4 me = A.parts[pi];
 	const float4 wv = A.w[pi];
 	const float wi[3] = {wv.x, wv.y, wv.z};
@@ -313,14 +321,25 @@ __global__ __launch_bounds__(64) void k_adj_pp(AdjArgs A, AdjItems I) {
 		}
 		__syncthreads();
 		float fx[3] = {0.f, 0.f, 0.f}, fm = 0.f;
-		for (int k = 0; k < n; k++) {
+		// tiles inside the list's near prefix may hold pairs closer than h (spline); later tiles
+		// hold only pairs at least h apart (the forward's split)
+		const bool near = base < near_end;
+		for (int k = ph; k < n; k += P) {
 			const int4 s = tp[k];
 			const float4 ws = tw[k];
 			const float mj = __int_as_float(s.w);
 			const float d[3] = {sepf(me.x, s.x, A.wrap), sepf(me.y, s.y, A.wrap), sepf(me.z, s.z, A.wrap)};
 			const float v[3] = {mj * wi[0] - mi * ws.x, mj * wi[1] - mi * ws.y, mj * wi[2] - mi * ws.z};
+			const float r2 = fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]));
 			float f, fr;
-			spline_f(fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0])), A.h, f, fr);
+			if (near) {
+				spline_f(r2, A.h, f, fr);
+			} else {
+				const float ri = __builtin_amdgcn_rsqf(r2);
+				const float ri2 = ri * ri;
+				f = ri2 * ri;
+				fr = -3.0f * f * ri2;
+			}
 			const float dv = fmaf(d[2], v[2], fmaf(d[1], v[1], d[0] * v[0]));
 			const float dw = fmaf(d[2], ws.z, fmaf(d[1], ws.y, d[0] * ws.x));
 			const float frdv = fr * dv;
@@ -332,7 +351,12 @@ __global__ __launch_bounds__(64) void k_adj_pp(AdjArgs A, AdjItems I) {
 		for (int b = 0; b < 3; b++) gx[b] += fx[b];
 		gm += fm;
 	}
-	if (sink) {
+	for (int off = nsp; off < 64; off <<= 1) {
+#pragma unroll
+		for (int b = 0; b < 3; b++) gx[b] += __shfl_xor(gx[b], off, 64);
+		gm += __shfl_xor(gm, off, 64);
+	}
+	if (ph == 0 && si < ns) {
 		double* o = A.grad + 4 * int64_t(pi);
 		o[0] += gx[0];
 		o[1] += gx[1];
