@@ -12,15 +12,18 @@
 //      k_adj_pp(_ewald) tidal tensor J(d) = dF/dd per pair; the lists are symmetric (leaf B is in
 //                      A's PP list iff A is in B's), so each particle's source-side terms are gathered
 //                      from its own list: dL/dx_k += J(d_kj) (m_j w_k - m_k w_j), dL/dm_k += -w_j . F(d_kj)
-//      k_adj_pc_sink   dL/dx_i += w . dg/dx_i (fifth derivatives); k_adj_pc_src mu_c += dg/dM_c^T w
+//      k_adj_pc_sink   dL/dx_i += w . dg/dx_i (fifth derivatives); k_adj_pc_t (by source node c)
+//                      mu_c += dg/dM_c^T w
 //   2. node lists:
 //      k_adj_cc_t  mu_c += M2L(., X_a - X_c)^T Lambda_a, over the CC entries sorted by source c
-//      k_adj_node  CP: dL/dm_j += Lambda_a . D(X_a - x_j), dL/dx_j += -m_j Lambda_a . grad D (fifth derivatives)
-//   3. k_adj_moments, per leaf: mu down the root path through the transposed M2M shifts, then the
-//      transposed P2M onto the leaf's particles.
+//      k_adj_cp_t  CP, by source leaf: dL/dm_j += Lambda_a . D(X_a - x_j), dL/dx_j += -m_j Lambda_a . grad D
+//                  (fifth derivatives), no atomics
+//   3. k_adj_down, level by level from the root: mu~_c = mu_c + M2M^T mu~_parent; then k_adj_moments,
+//      per leaf: the transposed P2M onto the leaf's particles.
 // Ewald kinds use the periodic correction's derivatives in the same places.  Per-term arithmetic is
-// FP32 like the forward; sums are FP64.  The multipole adjoints mu and the PC/CP source terms are
-// accumulated with FP64 atomics, so their summation order varies between runs; Lambda is not.
+// FP32 like the forward; sums are FP64.  The multipole adjoints mu are accumulated with FP64
+// atomics (one per source node and item), so their summation order varies between runs; Lambda and
+// the particle gradients are not.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -525,106 +528,139 @@ __global__ __launch_bounds__(64) void k_adj_pc_sink(AdjArgs A, AdjItems I) {
 
 // PC, source side: lane = multipole of the item's list, over the item's sinks (staged in LDS):
 // mu_c += dg/dM_c^T w (fourth derivatives), one FP64 atomic per slot and entry
-template <bool EWALD>
-__global__ __launch_bounds__(64) void k_adj_pc_src(AdjArgs A, AdjItems I) {
-	const int4 it = I.items[blockIdx.x];
-	const int t = threadIdx.x;
-	const int ns = it.y & PP_COUNT_MASK;
-	__shared__ int4 sp[64];
-	__shared__ float4 sw[64];
-	if (t < ns) {
-		sp[t] = A.parts[it.x + t];
-		sw[t] = A.w[it.x + t];
-	}
-	__syncthreads();
-	const bool wrapx = EWALD || A.wrap;
-	for (int e = it.z + t; e < it.w; e += 64) {
-		const int cidx = I.list[e];
-		const MultiSrc& ms = A.multis[cidx];
-		float mu[17];
-#pragma unroll
-		for (int q = 0; q < 17; q++) mu[q] = 0.0f;
-		for (int k = 0; k < ns; k++) {
-			const int4 me = sp[k];
-			const float wi[3] = {sw[k].x, sw[k].y, sw[k].z};
-			const float d[3] = {sepf(me.x, ms.x[0], wrapx), sepf(me.y, ms.x[1], wrapx), sepf(me.z, ms.x[2], wrapx)};
-			float D[35];
-			if (EWALD)
-				adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
-			else
-				adj::green_direct_n<35>(D, d);
-			m2p_source_adjoint(mu, D, wi);
-		}
-#pragma unroll
-		for (int q = 0; q < 17; q++) atomicAdd(A.mu + 17 * int64_t(cidx) + q, double(mu[q]));
-	}
-}
-
-// ---- 2. node lists -------------------------------------------------------------------------------
-struct AdjNodeArgs {
-	const int2* waves;
-	const int* list;
-	const int* owner;
-	const int* slot_node;
-	const int4* nodepos;
+// PC source side by source node: mu_c += sum over the sinks i of every leaf whose PC list holds c of
+// dg_i/dM_c^T w_i.  One work group per item (node c, a run of at most 16 sink leaves); the lanes
+// cover 64 / L leaves at a time, L = 2^lgL lanes per leaf (L >= the largest leaf), one sink each;
+// the 17 sums are reduced over the wave once per item and added with one atomic each.
+struct AdjPCArgs {
+	const int4* items;   // (node c, first sink leaf, end, 0)
+	const int2* leaves;  // sink leaves: (first particle, count)
 	const int4* parts;
+	const float4* w;
 	const MultiSrc* multis;
-	const double* Lam;
 	double* mu;
-	double* grad;
-	int nwaves;
+	int lgL;
 	int wrap;
 };
 
-template <bool EWALD, bool CC>
-__global__ __launch_bounds__(64) void k_adj_node(AdjNodeArgs A) {
-	const int2 wv = A.waves[blockIdx.x];
-	const int j = wv.x + threadIdx.x;
-	if (j >= wv.y) return;
-	const int a = A.slot_node[A.owner[j]];
-	const int src = A.list[j];
-	const int4 X = A.nodepos[a];
-	float Lam[35];
-#pragma unroll
-	for (int q = 0; q < 35; q++) Lam[q] = float(A.Lam[35 * int64_t(a) + q]);
+template <bool EWALD>
+__global__ __launch_bounds__(64) void k_adj_pc_t(AdjPCArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int t = threadIdx.x;
+	const int Q = 64 >> A.lgL;         // leaves per pass
+	const int qq = t >> A.lgL;         // this lane's leaf within the pass
+	const int o = t & ((1 << A.lgL) - 1);  // and its sink within the leaf
+	const MultiSrc& ms = A.multis[it.x];
+	const int cx0 = ms.x[0], cx1 = ms.x[1], cx2 = ms.x[2];
 	const bool wrapx = EWALD || A.wrap;
-	if (CC) {
-		const MultiSrc& ms = A.multis[src];
-		const float d[3] = {sepf(X.x, ms.x[0], wrapx), sepf(X.y, ms.x[1], wrapx), sepf(X.z, ms.x[2], wrapx)};
+	float mu[17];
+#pragma unroll
+	for (int q = 0; q < 17; q++) mu[q] = 0.0f;
+	for (int k = it.y + qq; k - qq < it.z; k += Q) {
+		if (k >= it.z) continue;
+		const int2 lf = A.leaves[k];
+		if (o >= lf.y) continue;
+		const int pi = lf.x + o;
+		const int4 me = A.parts[pi];
+		const float4 wv = A.w[pi];
+		const float wi[3] = {wv.x, wv.y, wv.z};
+		const float d[3] = {sepf(me.x, cx0, wrapx), sepf(me.y, cx1, wrapx), sepf(me.z, cx2, wrapx)};
 		float D[35];
 		if (EWALD)
 			adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
 		else
 			adj::green_direct_n<35>(D, d);
-		float mu[17];
+		m2p_source_adjoint(mu, D, wi);
+	}
+	__shared__ float red[17][65];
 #pragma unroll
-		for (int q = 0; q < 17; q++) mu[q] = 0.0f;
-		m2l_adjoint(mu, Lam, D);
+	for (int q = 0; q < 17; q++) red[q][t] = mu[q];
+	__syncthreads();
+	if (t < 17) {
+		double acc = 0.0;
+		for (int k = 0; k < 64; k++) acc += double(red[t][k]);
+		atomicAdd(A.mu + 17 * int64_t(it.x) + t, acc);
+	}
+}
+
+// ---- 2. node lists -------------------------------------------------------------------------------
+// CP by source leaf: the forward adds m_j D(X_a - x_j) to the expansion of every node a whose CP
+// list holds leaf B's range, so  dL/dm_j = sum_a Lambda_a . D(X_a - x_j)  and
+// dL/dx_j = -m_j sum_a Lambda_a . grad D(X_a - x_j)  (fifth derivatives).  One work group per source
+// leaf (item: first particle, count, sink-node list range); lanes are the leaf's particles, and the
+// P = 64 / nsp lane groups split the sink nodes, staged 64 at a time in LDS.  Every particle belongs
+// to one item, so the gradient is added without atomics.
+struct AdjCPArgs {
+	const int4* items;
+	const int* sinks;
+	const int4* nodepos;
+	const int4* parts;
+	const double* Lam;
+	double* grad;
+	int wrap;
+};
+
+template <bool EWALD>
+__global__ __launch_bounds__(64) void k_adj_cp_t(AdjCPArgs A) {
+	const int4 it = A.items[blockIdx.x];
+	const int t = threadIdx.x;
+	const int ns = it.y;
+	const int lg = ns <= 1 ? 0 : 32 - __clz(ns - 1);
+	const int nsp = 1 << lg;
+	const int P = 64 >> lg;
+	const int si = t & (nsp - 1);
+	const int ph = t >> lg;
+	const int pi = it.x + min(si, ns - 1);
+	const int4 me = A.parts[pi];
+	const bool wrapx = EWALD || A.wrap;
+	__shared__ float lam[64][36];
+	__shared__ int4 xs[64];
+	double gm = 0.0, g3[3] = {0.0, 0.0, 0.0};
+	for (int base = it.z; base < it.w; base += 64) {
+		const int n = min(64, it.w - base);
+		__syncthreads();
+		if (t < n) {
+			const int a = A.sinks[base + t];
+			xs[t] = A.nodepos[a];
 #pragma unroll
-		for (int q = 0; q < 17; q++) atomicAdd(A.mu + 17 * int64_t(src) + q, double(mu[q]));
-	} else {
-		const int4 s = A.parts[src];
-		const float mj = __int_as_float(s.w);
-		const float d[3] = {sepf(X.x, s.x, wrapx), sepf(X.y, s.y, wrapx), sepf(X.z, s.z, wrapx)};
-		float D[adj::NS5];
-		if (EWALD)
-			adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
-		else
-			adj::green_direct_n<adj::NS5>(D, d);
-		// L_t += m D_t(X_a - x_j):  dm += Lambda . D,  dx_j = -m Lambda . grad D
-		float gm = 0.0f, g3[3] = {0.f, 0.f, 0.f};
-		for_n<35>([&](auto t_) {
-			constexpr int t = decltype(t_)::value;
-			gm = fmaf(Lam[t], D[t], gm);
-			for_n<3>([&](auto b_) {
-				constexpr int b = decltype(b_)::value;
-				g3[b] = fmaf(Lam[t], D[cx<add(t, b)>], g3[b]);
+			for (int q = 0; q < 35; q++) lam[t][q] = float(A.Lam[35 * int64_t(a) + q]);
+		}
+		__syncthreads();
+		float fm = 0.0f, f3[3] = {0.f, 0.f, 0.f};
+		for (int k = ph; k < n; k += P) {
+			const int4 X = xs[k];
+			const float d[3] = {sepf(X.x, me.x, wrapx), sepf(X.y, me.y, wrapx), sepf(X.z, me.z, wrapx)};
+			float D[adj::NS5];
+			if (EWALD)
+				adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
+			else
+				adj::green_direct_n<adj::NS5>(D, d);
+			for_n<35>([&](auto t_) {
+				constexpr int tt = decltype(t_)::value;
+				const float l = lam[k][tt];
+				fm = fmaf(l, D[tt], fm);
+				for_n<3>([&](auto b_) {
+					constexpr int bb = decltype(b_)::value;
+					f3[bb] = fmaf(l, D[cx<add(tt, bb)>], f3[bb]);
+				});
 			});
-		});
-		double* o = A.grad + 4 * int64_t(src);
+		}
+		gm += fm;
 #pragma unroll
-		for (int b = 0; b < 3; b++) atomicAdd(o + b, double(-mj * g3[b]));
-		atomicAdd(o + 3, double(gm));
+		for (int q = 0; q < 3; q++) g3[q] += f3[q];
+	}
+	for (int off = nsp; off < 64; off <<= 1) {
+		gm += __shfl_xor(gm, off, 64);
+#pragma unroll
+		for (int q = 0; q < 3; q++) g3[q] += __shfl_xor(g3[q], off, 64);
+	}
+	if (ph == 0 && si < ns) {
+		const double mj = double(__int_as_float(me.w));
+		double* o = A.grad + 4 * int64_t(pi);
+		o[0] -= mj * g3[0];
+		o[1] -= mj * g3[1];
+		o[2] -= mj * g3[2];
+		o[3] += gm;
 	}
 }
 
@@ -752,29 +788,33 @@ __device__ __forceinline__ void m2m_adjoint(float* mu, const float* mup, const f
 	}
 }
 
+// The forward forms a parent's moments from its children's (M2M), so the multipole adjoint of a
+// node is its own mu plus the M2M transpose of its parent's:  mu~_c = mu_c + M2M(X_c - X_a)^T mu~_a,
+// level by level from the root (the up-sweep's (node, child) pairs in reverse level order); in place.
+__global__ __launch_bounds__(64) void k_adj_down(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ nodepos,
+		double* __restrict__ mu) {
+	const int j = blockIdx.x * 64 + threadIdx.x;
+	if (j >= npairs) return;
+	const int2 pc = pairs[j];
+	if (pc.y < 0) return;
+	const int4 Xa = nodepos[pc.x], Xc = nodepos[pc.y];
+	const float Y[3] = {float(double(int64_t(Xc.x) - int64_t(Xa.x)) * POS_INV), float(double(int64_t(Xc.y) - int64_t(Xa.y)) * POS_INV),
+			float(double(int64_t(Xc.z) - int64_t(Xa.z)) * POS_INV)};
+	float m[17], n[17];
+#pragma unroll
+	for (int q = 0; q < 17; q++) m[q] = float(mu[17 * int64_t(pc.x) + q]);
+	m2m_adjoint(n, m, Y);
+	double* dst = mu + 17 * int64_t(pc.y);
+#pragma unroll
+	for (int q = 0; q < 17; q++) dst[q] += double(n[q]);
+}
+
+// P2M transposed onto a leaf's particles, from the leaf's swept multipole adjoint
 __global__ __launch_bounds__(64) void k_adj_moments(AdjArgs A) {
 	const int4 lf = A.leaves[blockIdx.x];
 	const int t = threadIdx.x;
-	const int depth = A.level[lf.x] + 1;
 	__shared__ float mus[17];
-	if (t == 0) {
-		float m[17], n[17];
-		int a = A.path[lf.w];
-#pragma unroll
-		for (int q = 0; q < 17; q++) m[q] = float(A.mu[17 * int64_t(a) + q]);
-		for (int k = 1; k < depth; k++) {
-			const int b = A.path[lf.w + k];
-			const int4 Xa = A.nodepos[a], Xb = A.nodepos[b];
-			const float Y[3] = {float(double(int64_t(Xb.x) - int64_t(Xa.x)) * POS_INV),
-					float(double(int64_t(Xb.y) - int64_t(Xa.y)) * POS_INV), float(double(int64_t(Xb.z) - int64_t(Xa.z)) * POS_INV)};
-			m2m_adjoint(n, m, Y);
-#pragma unroll
-			for (int q = 0; q < 17; q++) m[q] = n[q] + float(A.mu[17 * int64_t(b) + q]);
-			a = b;
-		}
-#pragma unroll
-		for (int q = 0; q < 17; q++) mus[q] = m[q];
-	}
+	if (t < 17) mus[t] = float(A.mu[17 * int64_t(lf.x) + t]);
 	__syncthreads();
 	const int ns = lf.z - lf.y;
 	if (t >= ns) return;
@@ -842,6 +882,11 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 	ACHK(hipSetDevice(device));
 	if (!adj_lists_ready) {
 		// CC entries by source: (source, sink node) pairs of the direct and Ewald lists, sorted by source
+		auto put_v = [&](DeviceArray& d, const void* h, size_t bytes) -> hipError_t {
+			hipError_t r = d.alloc(std::max<size_t>(bytes, 16));
+			if (r != hipSuccess || bytes == 0) return r;
+			return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
+		};
 		for (int e = 0; e < 2; e++) {
 			const std::vector<int64_t>& ptr = e ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
 			const std::vector<int32_t>& v = e ? tree.lists.cc_e : tree.lists.cc_d;
@@ -856,15 +901,75 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 				snk[q] = pr[q].second;
 			}
 			for (size_t q = 0; q < pr.size(); q += 64) waves.push_back(make_int2(int(q), int(std::min(pr.size(), q + 64))));
-			auto put_v = [&](DeviceArray& d, const void* h, size_t bytes) -> hipError_t {
-				hipError_t r = d.alloc(std::max<size_t>(bytes, 16));
-				if (r != hipSuccess || bytes == 0) return r;
-				return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
-			};
 			ACHK(put_v(d_adj_cct_src[e], src.data(), src.size() * sizeof(int)));
 			ACHK(put_v(d_adj_cct_snk[e], snk.data(), snk.size() * sizeof(int)));
 			ACHK(put_v(d_adj_cct_waves[e], waves.data(), waves.size() * sizeof(int2)));
 			adj_cct_nwaves[e] = int(waves.size());
+		}
+		// CP entries by source leaf (direct, Ewald): every CP range is a whole leaf's particles
+		std::vector<int32_t> leaf_at(size_t(n) + 1, -1);
+		for (int32_t lf : tree.leaves)
+			if (!tree.nodes[lf].empty()) leaf_at[size_t(tree.nodes[lf].pbegin)] = lf;
+		for (int e = 0; e < 2; e++) {
+			const std::vector<int64_t>& ptr = e ? tree.lists.cp_e_ptr : tree.lists.cp_d_ptr;
+			const std::vector<int32_t>& v = e ? tree.lists.cp_e : tree.lists.cp_d;
+			std::vector<std::pair<int32_t, int32_t>> pr;  // (source leaf first particle, sink node)
+			for (size_t a = 0; a + 1 < ptr.size(); a++)
+				for (int64_t q = ptr[a]; q < ptr[a + 1]; q += 2) {
+					const int32_t b = v[q], en = v[q + 1];
+					if (en <= b) continue;
+					const int32_t lf = leaf_at[size_t(b)];
+					if (lf < 0 || tree.nodes[lf].pend != en || en - b > 64) {
+						err = "the adjoint needs CP sources that are whole leaves of at most 64 particles";
+						return TG_EINVAL;
+					}
+					pr.emplace_back(b, int32_t(a));
+				}
+			std::sort(pr.begin(), pr.end());
+			std::vector<int> sinks(pr.size());
+			std::vector<int4> items;
+			for (size_t q = 0; q < pr.size(); q++) {
+				sinks[q] = pr[q].second;
+				if (q == 0 || pr[q].first != pr[q - 1].first) {
+					const int32_t b = pr[q].first, lf = leaf_at[size_t(b)];
+					items.push_back(make_int4(b, tree.nodes[lf].pend - b, int(q), int(q)));
+				}
+				items.back().w = int(q + 1);
+			}
+			ACHK(put_v(d_adj_cpt_sinks[e], sinks.data(), sinks.size() * sizeof(int)));
+			ACHK(put_v(d_adj_cpt_items[e], items.data(), items.size() * sizeof(int4)));
+			adj_cpt_nitems[e] = int(items.size());
+		}
+		// PC entries by source node (direct, Ewald): runs of at most 16 sink leaves per item
+		int maxleaf = 1;
+		for (int32_t lf : tree.leaves) maxleaf = std::max(maxleaf, int(tree.nodes[lf].pend - tree.nodes[lf].pbegin));
+		adj_pc_lgL = 0;
+		while ((1 << adj_pc_lgL) < maxleaf) adj_pc_lgL++;
+		if (adj_pc_lgL > 6) {
+			err = "the adjoint handles leaves of at most 64 particles";
+			return TG_EINVAL;
+		}
+		for (int e = 0; e < 2; e++) {
+			const std::vector<int64_t>& ptr = e ? tree.lists.pc_e_ptr : tree.lists.pc_d_ptr;
+			const std::vector<int32_t>& v = e ? tree.lists.pc_e : tree.lists.pc_d;
+			std::vector<std::pair<int32_t, int32_t>> pr;  // (source node, sink leaf)
+			for (int32_t lf : tree.leaves) {
+				if (tree.nodes[lf].empty() || size_t(lf) + 1 >= ptr.size()) continue;
+				for (int64_t q = ptr[lf]; q < ptr[lf + 1]; q++) pr.emplace_back(v[q], lf);
+			}
+			std::sort(pr.begin(), pr.end());
+			std::vector<int2> sl(pr.size());
+			std::vector<int4> items;
+			for (size_t q = 0; q < pr.size(); q++) {
+				const Node& nd = tree.nodes[pr[q].second];
+				sl[q] = make_int2(nd.pbegin, nd.pend - nd.pbegin);
+				if (q == 0 || pr[q].first != pr[q - 1].first || items.back().z - items.back().y >= 16)
+					items.push_back(make_int4(pr[q].first, int(q), int(q), 0));
+				items.back().z = int(q + 1);
+			}
+			ACHK(put_v(d_adj_pct_leaves[e], sl.data(), sl.size() * sizeof(int2)));
+			ACHK(put_v(d_adj_pct_items[e], items.data(), items.size() * sizeof(int4)));
+			adj_pct_nitems[e] = int(items.size());
 		}
 		// (node, child) pairs of the internal nodes, deepest level first; an empty child gets -1
 		std::vector<std::vector<int2>> by_level(size_t(tree.max_level) + 1);
@@ -946,24 +1051,12 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 			hipLaunchKernelGGL(k_adj_pp, g, b, 0, s, a, I);
 		else if (k == PP_EWALD)
 			hipLaunchKernelGGL(k_adj_pp_ewald, g, b, 0, s, a, I);
-		else if (k == PC_DIRECT) {
+		else if (k == PC_DIRECT)
 			hipLaunchKernelGGL(k_adj_pc_sink<false>, g, b, 0, s, a, I);
-			hipLaunchKernelGGL(k_adj_pc_src<false>, g, b, 0, s, a, I);
-		} else {
+		else
 			hipLaunchKernelGGL(k_adj_pc_sink<true>, g, b, 0, s, a, I);
-			hipLaunchKernelGGL(k_adj_pc_src<true>, g, b, 0, s, a, I);
-		}
 		ACHK(hipGetLastError());
 	}
-	AdjNodeArgs na{};
-	na.slot_node = d_slot_node.as<int>();
-	na.nodepos = d_nodepos.as<int4>();
-	na.parts = d_parts.as<int4>();
-	na.multis = d_multis.as<MultiSrc>();
-	na.Lam = d_adj_lam.as<double>();
-	na.mu = d_adj_mu.as<double>();
-	na.grad = d_adj_grad.as<double>();
-	na.wrap = ewald;
 	for (int e = 0; e < 2; e++) {
 		if (adj_cct_nwaves[e] <= 0) continue;
 		AdjCCArgs ca{d_adj_cct_waves[e].as<int2>(), d_adj_cct_src[e].as<int>(), d_adj_cct_snk[e].as<int>(), d_nodepos.as<int4>(),
@@ -974,17 +1067,30 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 			hipLaunchKernelGGL(k_adj_cc_t<false>, dim3(adj_cct_nwaves[e]), dim3(64), 0, s, ca);
 		ACHK(hipGetLastError());
 	}
-	for (int k : {CP_DIRECT, CP_EWALD}) {
-		if (work[k].nitems <= 0) continue;
-		na.waves = work[k].items.as<int2>();
-		na.list = work[k].list.as<int>();
-		na.owner = work[k].owner.as<int>();
-		na.nwaves = work[k].nitems;
-		const dim3 g(work[k].nitems), b(64);
-		if (k == CP_DIRECT)
-			hipLaunchKernelGGL((k_adj_node<false, false>), g, b, 0, s, na);
+	for (int e = 0; e < 2; e++) {
+		if (adj_pct_nitems[e] <= 0) continue;
+		AdjPCArgs pa{d_adj_pct_items[e].as<int4>(), d_adj_pct_leaves[e].as<int2>(), d_parts.as<int4>(), d_adj_w.as<float4>(),
+				d_multis.as<MultiSrc>(), d_adj_mu.as<double>(), adj_pc_lgL, ewald};
+		if (e)
+			hipLaunchKernelGGL(k_adj_pc_t<true>, dim3(adj_pct_nitems[e]), dim3(64), 0, s, pa);
 		else
-			hipLaunchKernelGGL((k_adj_node<true, false>), g, b, 0, s, na);
+			hipLaunchKernelGGL(k_adj_pc_t<false>, dim3(adj_pct_nitems[e]), dim3(64), 0, s, pa);
+		ACHK(hipGetLastError());
+	}
+	for (int e = 0; e < 2; e++) {
+		if (adj_cpt_nitems[e] <= 0) continue;
+		AdjCPArgs pa{d_adj_cpt_items[e].as<int4>(), d_adj_cpt_sinks[e].as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(),
+				d_adj_lam.as<double>(), d_adj_grad.as<double>(), ewald};
+		if (e)
+			hipLaunchKernelGGL(k_adj_cp_t<true>, dim3(adj_cpt_nitems[e]), dim3(64), 0, s, pa);
+		else
+			hipLaunchKernelGGL(k_adj_cp_t<false>, dim3(adj_cpt_nitems[e]), dim3(64), 0, s, pa);
+		ACHK(hipGetLastError());
+	}
+	for (size_t l = adj_up_ptr.size(); l-- > 1;) {
+		const int b = adj_up_ptr[l - 1], np = adj_up_ptr[l] - b;
+		hipLaunchKernelGGL(k_adj_down, dim3(unsigned((np + 63) / 64)), dim3(64), 0, s, d_adj_up.as<int2>() + b, np, d_nodepos.as<int4>(),
+				d_adj_mu.as<double>());
 		ACHK(hipGetLastError());
 	}
 	if (n_leaf_items > 0) {

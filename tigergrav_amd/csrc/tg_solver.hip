@@ -196,12 +196,17 @@ Solver::~Solver() {
 		d_node_slot_ptr.release();
 		d_dL.release();
 		d_queue.release();
-		d_leaf_lists.release();
 		d_adj_w.release();
 		d_adj_grad.release();
 		d_adj_lam.release();
 		d_adj_mu.release();
 		d_adj_up.release();
+		for (int e = 0; e < 2; e++) {
+			d_adj_cpt_items[e].release();
+			d_adj_cpt_sinks[e].release();
+			d_adj_pct_items[e].release();
+			d_adj_pct_leaves[e].release();
+		}
 		for (int e = 0; e < 2; e++) {
 			d_adj_cct_src[e].release();
 			d_adj_cct_snk[e].release();
@@ -317,6 +322,17 @@ int Solver::upload_sources() {
 	return TG_OK;
 }
 
+// list expansion: work group g writes runs [64 g, 64 g + 64) one after another, the lanes covering
+// a run's entries: out[dst + l] = value + step * l
+__global__ __launch_bounds__(64) void k_expand_runs(const ListRun* __restrict__ runs, int64_t nruns, int step, int* __restrict__ out) {
+	const int64_t r0 = int64_t(blockIdx.x) * 64;
+	const int64_t r1 = r0 + 64 < nruns ? r0 + 64 : nruns;
+	for (int64_t r = r0; r < r1; r++) {
+		const ListRun g = runs[r];
+		for (int l = threadIdx.x; l < g.len; l += 64) out[g.dst + l] = g.value + step * l;
+	}
+}
+
 template <class T>
 static hipError_t put(DeviceArray& d, const std::vector<T>& h, hipStream_t s) {
 	hipError_t e = d.alloc(std::max<size_t>(h.size() * sizeof(T), 16));
@@ -345,17 +361,28 @@ int Solver::upload() {
 		npos[i] = make_int4(nd.x[0], nd.x[1], nd.x[2], 0);
 	}
 	// node kinds: each node's list is cut into chunks of <= node_chunk entries, one partial slot per
-	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64
+	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64.
+	// Source ranges (CP, PP) are not flattened here: each becomes a run that the device expands
+	// (k_expand_runs), as do the partial-slot owners of the node kinds' entries.
 	const Lists& Ls = T.lists;
 	std::vector<int4> items[NKINDS];
-	std::vector<int> lists[NKINDS], owner[NKINDS];
+	std::vector<int> lists[NKINDS];  // host-built lists (PC kinds)
+	std::vector<ListRun> runs[NKINDS], owner_runs[NKINDS];
+	const std::vector<int32_t>* walk_list[NKINDS] = {};  // CC kinds: the walk's flat list, node order
+	walk_list[CC_DIRECT] = &Ls.cc_d;
+	walk_list[CC_EWALD] = &Ls.cc_e;
+	int64_t list_len[NKINDS] = {};
 	std::vector<int2> chunks[NKINDS];
 	std::vector<int> slot_node;
 	std::vector<int64_t> slot_ptr(nn + 1, 0);
 	int64_t slot = 0;
-	auto expand = [](std::vector<int>& dst, const std::vector<int32_t>& ranges, int64_t b, int64_t e) {
-		for (int64_t k = b; k < e; k += 2)
-			for (int32_t p = ranges[k]; p < ranges[k + 1]; p++) dst.push_back(p);
+	auto add_ranges = [&](int kind, const std::vector<int32_t>& ranges, int64_t b, int64_t e) {
+		for (int64_t k = b; k < e; k += 2) {
+			const int32_t len = ranges[k + 1] - ranges[k];
+			if (len <= 0) continue;
+			runs[kind].push_back(ListRun{ranges[k], len, list_len[kind]});
+			list_len[kind] += len;
+		}
 	};
 	const int chunk = std::min(64, std::max(1, node_chunk));
 	for (int64_t nd = 0; nd < nn; nd++) {
@@ -366,18 +393,22 @@ int Solver::upload() {
 			bool ranges;
 		} src[4] = {{CC_DIRECT, &Ls.cc_d_ptr, &Ls.cc_d, false}, {CP_DIRECT, &Ls.cp_d_ptr, &Ls.cp_d, true},
 				{CC_EWALD, &Ls.cc_e_ptr, &Ls.cc_e, false}, {CP_EWALD, &Ls.cp_e_ptr, &Ls.cp_e, true}};
-		for (auto& s : src) {
-			std::vector<int>& L = lists[s.kind];
-			const int64_t lb = int64_t(L.size());
-			if (s.ranges)
-				expand(L, *s.v, (*s.ptr)[nd], (*s.ptr)[nd + 1]);
-			else
-				L.insert(L.end(), s.v->begin() + (*s.ptr)[nd], s.v->begin() + (*s.ptr)[nd + 1]);
-			const int64_t le = int64_t(L.size());
+		for (auto& q : src) {
+			const int64_t lb = list_len[q.kind];
+			if (q.ranges) {
+				add_ranges(q.kind, *q.v, (*q.ptr)[nd], (*q.ptr)[nd + 1]);
+			} else {
+				if ((*q.ptr)[nd] != lb) {
+					err = "node interaction list not in node order";
+					return TG_EINVAL;
+				}
+				list_len[q.kind] = (*q.ptr)[nd + 1];
+			}
+			const int64_t le = list_len[q.kind];
 			for (int64_t b = lb; b < le; b += chunk) {
 				const int64_t e = std::min<int64_t>(le, b + chunk);
-				chunks[s.kind].push_back(make_int2(int(b), int(e)));
-				owner[s.kind].insert(owner[s.kind].end(), size_t(e - b), int(slot));
+				chunks[q.kind].push_back(make_int2(int(b), int(e)));
+				owner_runs[q.kind].push_back(ListRun{int32_t(slot), int32_t(e - b), b});
 				slot_node.push_back(int(nd));
 				slot++;
 			}
@@ -442,15 +473,25 @@ int Solver::upload() {
 		}
 		return d2 <= hnear * hnear;
 	};
+	// near classification of every PP-direct source range, in parallel over the leaves
+	std::vector<uint8_t> pp_near(Ls.pp_d.size() / 2, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+	for (int64_t i = 0; i < int64_t(T.leaves.size()); i++) {
+		const int32_t lf = T.leaves[size_t(i)];
+		const Node& nd = T.nodes[lf];
+		if (nd.empty() || size_t(lf) + 1 >= Ls.pp_d_ptr.size()) continue;
+		for (int64_t k = Ls.pp_d_ptr[lf]; k < Ls.pp_d_ptr[lf + 1]; k += 2) {
+			double mn[3], mx[3];
+			range_box(Ls.pp_d[k], Ls.pp_d[k + 1], mn, mx);
+			pp_near[size_t(k / 2)] = may_be_near(nd, mn, mx) ? 1 : 0;
+		}
+	}
 	std::vector<int32_t> far_ranges;
-	std::vector<int4> leaf_items, leaf_lists;
+	std::vector<int4> leaf_items;
 	for (int32_t lf : T.leaves) {
 		const Node& nd = T.nodes[lf];
 		if (nd.empty()) continue;
 		leaf_items.push_back(make_int4(lf, nd.pbegin, nd.pend, 0));
-		leaf_lists.push_back(make_int4(0, 0, 0, 0));
-		leaf_lists.push_back(make_int4(0, 0, 0, 0));
-		int4* ll = &leaf_lists[leaf_lists.size() - 2];
 		struct {
 			int kind;
 			const std::vector<int64_t>* ptr;
@@ -459,24 +500,24 @@ int Solver::upload() {
 		} src[4] = {{PC_DIRECT, &Ls.pc_d_ptr, &Ls.pc_d, false}, {PP_DIRECT, &Ls.pp_d_ptr, &Ls.pp_d, true},
 				{PC_EWALD, &Ls.pc_e_ptr, &Ls.pc_e, false}, {PP_EWALD, &Ls.pp_e_ptr, &Ls.pp_e, true}};
 		for (auto& s : src) {
-			std::vector<int>& L = lists[s.kind];
-			const int64_t lb = int64_t(L.size());
+			const int64_t lb = list_len[s.kind];
 			int64_t near_len = 0;
 			if (s.kind == PP_DIRECT) {
 				far_ranges.clear();
 				for (int64_t k = (*s.ptr)[lf]; k < (*s.ptr)[lf + 1]; k += 2) {
 					const int32_t b = (*s.v)[k], e = (*s.v)[k + 1];
-					double mn[3], mx[3];
-					range_box(b, e, mn, mx);
-					if (may_be_near(nd, mn, mx)) {
-						for (int32_t q = b; q < e; q++) L.push_back(q);
+					if (pp_near[size_t(k / 2)]) {
+						if (e > b) {
+							runs[s.kind].push_back(ListRun{b, e - b, list_len[s.kind]});
+							list_len[s.kind] += e - b;
+						}
 					} else {
 						far_ranges.push_back(b);
 						far_ranges.push_back(e);
 					}
 				}
-				near_len = int64_t(L.size()) - lb;
-				expand(L, far_ranges, 0, int64_t(far_ranges.size()));
+				near_len = list_len[s.kind] - lb;
+				add_ranges(s.kind, far_ranges, 0, int64_t(far_ranges.size()));
 			} else if (s.kind == PP_EWALD) {
 				// the real-space images this leaf's pairs can need: |n| < 2.6 + the largest separation
 				double dmax2 = 0.0;
@@ -496,18 +537,16 @@ int Solver::upload() {
 					}
 					dmax2 = std::max(dmax2, d2);
 				}
-				expand(L, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
+				add_ranges(s.kind, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
 				near_len = all ? int64_t(PP_NEAR_ALL) : int64_t(ewald_images_within(2.6 + std::sqrt(dmax2) + 1e-6));
 			} else if (s.ranges) {
-				expand(L, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
+				add_ranges(s.kind, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
 			} else {
+				std::vector<int>& L = lists[s.kind];
 				L.insert(L.end(), s.v->begin() + (*s.ptr)[lf], s.v->begin() + (*s.ptr)[lf + 1]);
+				list_len[s.kind] = int64_t(L.size());
 			}
-			const int64_t le = int64_t(L.size());
-			if (s.kind == PP_DIRECT) { ll[0].x = int(lb); ll[0].y = int(le); }
-			else if (s.kind == PC_DIRECT) { ll[0].z = int(lb); ll[0].w = int(le); }
-			else if (s.kind == PP_EWALD) { ll[1].x = int(lb); ll[1].y = int(le); }
-			else { ll[1].z = int(lb); ll[1].w = int(le); }
+			const int64_t le = list_len[s.kind];
 			if (le == lb) continue;
 			for (int32_t b = nd.pbegin; b < nd.pend; b += 64) {
 				const int cnt = std::min<int32_t>(64, nd.pend - b);
@@ -526,6 +565,11 @@ int Solver::upload() {
 		});
 	}
 	n_leaf_items = int(leaf_items.size());
+	for (int k = 0; k < NKINDS; k++)
+		if (list_len[k] > int64_t(INT32_MAX)) {
+			err = "an interaction list exceeds 2^31 entries (32-bit list offsets)";
+			return TG_EINVAL;
+		}
 	// root-to-leaf paths for the local-expansion pass
 	std::vector<int> path, level(nn);
 	for (int64_t i = 0; i < nn; i++) level[i] = T.nodes[i].level;
@@ -539,9 +583,11 @@ int Solver::upload() {
 		for (int a = li.x; a >= 0; a = T.nodes[a].parent) up.push_back(a);
 		path.insert(path.end(), up.rbegin(), up.rend());
 	}
-	if (device < 0) {  // host-only: keep the PP-direct work for inspection
+	if (device < 0) {  // host-only: keep the PP-direct work for inspection, its list expanded here
 		host_pp_items = std::move(items[PP_DIRECT]);
-		host_pp_list = std::move(lists[PP_DIRECT]);
+		host_pp_list.assign(size_t(list_len[PP_DIRECT]), 0);
+		for (const ListRun& g : runs[PP_DIRECT])
+			for (int32_t l = 0; l < g.len; l++) host_pp_list[size_t(g.dst + l)] = g.value + l;
 		return TG_OK;
 	}
 	hipStream_t s = stream;
@@ -550,35 +596,66 @@ int Solver::upload() {
 	SCHK(put(d_nodepos, npos, s));
 	SCHK(put(d_node_slot_ptr, slot_ptr, s));
 	SCHK(put(d_leaf_items, leaf_items, s));
-	SCHK(put(d_leaf_lists, leaf_lists, s));
 	adj_checked = false;
 	adj_lists_ready = false;
 	SCHK(put(d_slot_node, slot_node, s));
 	SCHK(put(d_path, path, s));
 	SCHK(put(d_level, level, s));
 	list_bytes = 0;
-	for (int k = 0; k < NKINDS; k++) {
+	// run tables stay allocated until the stream has drained below
+	std::vector<DeviceArray> run_tables;
+	auto expand_runs = [&](const std::vector<ListRun>& r, int step, DeviceArray& out) -> hipError_t {
+		if (r.empty()) return hipSuccess;
+		run_tables.emplace_back();
+		hipError_t e = put(run_tables.back(), r, s);
+		if (e != hipSuccess) return e;
+		hipLaunchKernelGGL(k_expand_runs, dim3(unsigned((r.size() + 63) / 64)), dim3(64), 0, s, run_tables.back().as<ListRun>(),
+				int64_t(r.size()), step, out.as<int>());
+		return hipGetLastError();
+	};
+	int rc = TG_OK;
+	for (int k = 0; k < NKINDS && rc == TG_OK; k++) {
 		const bool node_kind = (k == CC_DIRECT || k == CP_DIRECT || k == CC_EWALD || k == CP_EWALD);
+		const size_t lbytes = size_t(list_len[k]) * sizeof(int);
+		hipError_t e = hipSuccess;
 		if (node_kind) {
-			SCHK(put(work[k].items, waves[k], s));
-			SCHK(put(work[k].owner, owner[k], s));
+			e = put(work[k].items, waves[k], s);
+			if (e == hipSuccess) e = work[k].owner.alloc(std::max<size_t>(lbytes, 16));
+			if (e == hipSuccess) e = expand_runs(owner_runs[k], 0, work[k].owner);
 			work[k].nitems = int(waves[k].size());
-			list_bytes += int64_t(waves[k].size() * sizeof(int2) + owner[k].size() * sizeof(int));
+			list_bytes += int64_t(waves[k].size() * sizeof(int2) + lbytes);
 		} else {
-			SCHK(put(work[k].items, items[k], s));
+			e = put(work[k].items, items[k], s);
 			work[k].nitems = int(items[k].size());
 			list_bytes += int64_t(items[k].size() * sizeof(int4));
 		}
-		SCHK(put(work[k].list, lists[k], s));
-		work[k].nlist = int64_t(lists[k].size());
-		list_bytes += int64_t(lists[k].size() * sizeof(int));
+		if (e == hipSuccess) e = work[k].list.alloc(std::max<size_t>(lbytes, 16));
+		if (e == hipSuccess && lbytes > 0) {
+			if (walk_list[k])
+				e = hipMemcpyAsync(work[k].list.p, walk_list[k]->data(), lbytes, hipMemcpyHostToDevice, s);
+			else if (!lists[k].empty())
+				e = hipMemcpyAsync(work[k].list.p, lists[k].data(), lbytes, hipMemcpyHostToDevice, s);
+			else
+				e = expand_runs(runs[k], 1, work[k].list);
+		}
+		work[k].nlist = list_len[k];
+		list_bytes += int64_t(lbytes);
+		if (e != hipSuccess) {
+			err = hipGetErrorString(e);
+			rc = TG_EHIP;
+		}
 	}
+	const hipError_t es = hipStreamSynchronize(s);
+	for (auto& t : run_tables) t.release();
+	if (rc != TG_OK) return rc;
+	SCHK(es);
 	SCHK(d_partial.alloc(std::max<size_t>(n_slots * 35 * sizeof(double), 16)));
 	SCHK(d_dL.alloc(std::max<size_t>(T.nodes.size() * 35 * sizeof(double), 16)));
 	SCHK(d_out.alloc(std::max<size_t>(n * 4 * sizeof(double), 16)));
 	SCHK(hipStreamSynchronize(s));
 	return TG_OK;
 }
+
 
 int Solver::enqueue_solve(bool timed) {
 	SCHK(hipSetDevice(device));

@@ -22,6 +22,13 @@ struct DeviceArray {
 	}
 };
 
+// a run of interaction-list entries expanded on the device: list[dst + l] = value + step * l, l < len
+struct ListRun {
+	int32_t value;
+	int32_t len;
+	int64_t dst;
+};
+
 // one kind's work on the device
 struct KindWork {
 	DeviceArray items;  // int4 leaf items (PP/PC) or int2 waves (CC/CP)
@@ -46,14 +53,21 @@ struct Solver {
 	// device state
 	DeviceArray d_parts, d_multis, d_nodepos, d_out, d_partial, d_node_slot_ptr;
 	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL, d_queue;
-	// adjoint: per leaf item the list ranges of PP_d, PC_d (one int4) and PP_e, PC_e (another)
-	DeviceArray d_leaf_lists, d_adj_w, d_adj_grad, d_adj_lam, d_adj_mu;
+	// adjoint buffers: loss weights, gradients, expansion and multipole adjoints
+	DeviceArray d_adj_w, d_adj_grad, d_adj_lam, d_adj_mu;
 	bool adj_pp_symmetric = false;  // checked once per tree at the first adjoint call
 	bool adj_checked = false;
 	hipEvent_t adj_ev[2] = {nullptr, nullptr};
 	// CC entries by source for the adjoint (direct, Ewald): built at the first adjoint call per tree
 	DeviceArray d_adj_cct_src[2], d_adj_cct_snk[2], d_adj_cct_waves[2];
 	int adj_cct_nwaves[2] = {0, 0};
+	// CP entries by source leaf (direct, Ewald): items (first particle, count, sink range) and sink nodes
+	DeviceArray d_adj_cpt_items[2], d_adj_cpt_sinks[2];
+	int adj_cpt_nitems[2] = {0, 0};
+	// PC entries by source node (direct, Ewald): items (node, sink-leaf run) and sink leaves
+	DeviceArray d_adj_pct_items[2], d_adj_pct_leaves[2];
+	int adj_pct_nitems[2] = {0, 0};
+	int adj_pc_lgL = 6;  // lanes per sink leaf in k_adj_pc_t: 2^adj_pc_lgL >= the largest leaf
 	// (node, child) pairs for the Lambda up-sweep, grouped by level deepest first: batch l is
 	// [adj_up_ptr[l], adj_up_ptr[l + 1])
 	DeviceArray d_adj_up;
