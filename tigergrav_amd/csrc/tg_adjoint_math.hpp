@@ -186,7 +186,7 @@ __device__ __forceinline__ void fourier_add_seq(float* D, const float* ph, const
 
 // the periodic correction (tigergrav/green.hpp:189-286) and its derivatives (first NS slots), in the
 // forward's decomposition: the n = 0 image as erf(2r)/r (FP64 series), the real-space images of the
-// forward's table (|n|^2 <= 11), the Fourier vectors of one half-space with doubled coefficients,
+// forward's table (|n|^2 <= 11) within the forward's prefix |n| < 2.6 + r, the Fourier vectors of one half-space with doubled coefficients,
 // and +pi/4 on the potential.  Below r = 1e-6 the reference takes 2.8372975 and zero derivatives.
 template <int NS>
 __device__ __forceinline__ void green_ewald_n(float* D, const float* x, const float* real, const float* four) {
@@ -201,7 +201,8 @@ __device__ __forceinline__ void green_ewald_n(float* D, const float* x, const fl
 	float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 	ewald_self_factors5<K>(double(r2), d);
 	radial_add<NS>(D, d, x);
-	for (int i = 0; i < TG_NREAL; i++) {
+	const int nimg = ewald_images_for(r2);  // the forward's image prefix
+	for (int i = 0; i < nimg; i++) {
 		const float dx[3] = {x[0] - real[3 * i], x[1] - real[3 * i + 1], x[2] - real[3 * i + 2]};
 		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
 		ewald_real_factors5<K>(q2, d);

@@ -10,7 +10,7 @@
 //  * node kinds (CC, CP): lane = source; a wave packs the short lists of several expansion
 //    centres (<= 64 entries), 35 float accumulators per lane, merged per centre in FP64 through
 //    LDS by a segmented scan.
-//  * Ewald kinds use tg::green_ewald / green_ewald1 (tg_math.hpp).
+//  * Ewald kinds use tg::green_ewald (tg_math.hpp) and, for PP, green_ewald1_real below.
 #include "tg_kernels.hpp"
 #include "tg_math.hpp"
 
@@ -103,6 +103,11 @@ hipError_t init_device_tables(int device) {
 	std::vector<double> ser;
 	build_tables(real, four, ser);
 	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
+	// the image prefixes (ewald_images_upto) assume the table sorted by |n|^2
+	for (int i = 0; i < TG_NREAL; i++) {
+		const int n2 = int(std::lround(real[3 * i] * real[3 * i] + real[3 * i + 1] * real[3 * i + 1] + real[3 * i + 2] * real[3 * i + 2]));
+		if (i >= ewald_images_upto(n2) || i < ewald_images_upto(n2 - 1)) return hipErrorInvalidValue;
+	}
 	hipError_t e;
 	if ((e = hipSetDevice(device)) != hipSuccess) return e;
 	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;

@@ -351,6 +351,20 @@ __device__ __forceinline__ void acc_assemble(const EwaldAcc& s, float* D) {
 // Full periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x.
 // The n = 0 image is combined with the subtracted Newtonian term analytically (erf(2r)/r), so
 // nothing cancels; at |x| <= 1e-6 the reference's convention D = (2.8372975, 0, ..., 0) applies.
+// real-space images in the table (sorted by |n|^2: 6, 12, 8, 6, 24, 24, 0, 12, 30, 24, 24 with
+// |n|^2 = 1..11) with 0 < |n|^2 <= k
+__host__ __device__ constexpr int ewald_images_upto(int k) {
+	return k <= 0 ? 0 : k == 1 ? 6 : k == 2 ? 18 : k == 3 ? 26 : k == 4 ? 32 : k == 5 ? 56 : k <= 7 ? 80 : k == 8 ? 92 : k == 9 ? 122 : k == 10 ? 146 : 170;
+}
+// the images a separation x (minimum image, r = |x|) can need: |n| < 2.6 + r.  Every other image is
+// at least 2.6 away, where erfc(2 r') / r' and its derivatives to fourth order are below 1e-7 of
+// the correction's own (the forward's PP Ewald prefix uses the same bound)
+__device__ __forceinline__ int ewald_images_for(float r2) {
+	const float R = 2.6f + sqrtf(r2) + 1e-6f;
+	const float R2 = R * R;
+	return R2 > 11.0f ? TG_NREAL : ewald_images_upto(int(ceilf(R2)) - 1);
+}
+
 __device__ __forceinline__ void green_ewald(float* D, const float* x) {
 	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
 	EwaldAcc s;
@@ -360,8 +374,9 @@ __device__ __forceinline__ void green_ewald(float* D, const float* x) {
 		ewald_self_factors<4>(double(r2), d);
 		acc_add(s, x, d);
 	}
+	const int nimg = ewald_images_for(r2);
 #pragma unroll 2
-	for (int i = 0; i < TG_NREAL; i++) {
+	for (int i = 0; i < nimg; i++) {
 		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
 		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
 		float d[5];
@@ -390,40 +405,6 @@ __device__ __forceinline__ void green_ewald(float* D, const float* x) {
 #pragma unroll
 		for (int t = 1; t < 35; t++) D[t] = 0.0f;
 	}
-}
-
-// potential and first derivatives of the periodic correction only (PP_ewald)
-__device__ __forceinline__ void green_ewald1(float& D0, float* D1, const float* x) {
-	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
-	float d[2];
-	ewald_self_factors<1>(double(r2), d);
-	float a0 = d[0];
-	float a1[3] = {x[0] * d[1], x[1] * d[1], x[2] * d[1]};
-#pragma unroll 4
-	for (int i = 0; i < TG_NREAL; i++) {
-		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
-		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
-		float e[2];
-		ewald_real_factors<1>(q2, e);
-		a0 += e[0];
-#pragma unroll
-		for (int a = 0; a < 3; a++) a1[a] = fmaf(dx[a], e[1], a1[a]);
-	}
-#pragma unroll 4
-	for (int i = 0; i < TG_NFOUR; i++) {
-		const float* H = c_four + 38 * i;
-		const float w = fmaf(H[2], x[2], fmaf(H[1], x[1], H[0] * x[0]));
-		const float sn = __builtin_amdgcn_sinf(w);
-		const float cs = __builtin_amdgcn_cosf(w);
-		a0 = fmaf(H[3], cs, a0);
-#pragma unroll
-		for (int a = 0; a < 3; a++) a1[a] = fmaf(H[4 + a], sn, a1[a]);
-	}
-	a0 += 0.78539816339744831f;
-	const bool z = !(r2 > 1e-12f);
-	D0 = z ? 2.8372975f : a0;
-#pragma unroll
-	for (int a = 0; a < 3; a++) D1[a] = z ? 0.0f : a1[a];
 }
 
 }  // namespace tg
