@@ -711,18 +711,23 @@ __global__ __launch_bounds__(64) void k_adj_cc_t(AdjCCArgs A) {
 	for (int q = 0; q < 17; q++) red[q][t] = mu[q];
 	sc[t] = c;
 	__syncthreads();
+	// segments of one source from a ballot of their first entries; lanes 0..16 add one slot each
+	const bool first = t < cnt && (t == 0 || sc[t] != sc[t - 1]);
+	unsigned long long starts = __ballot(first);
 	if (t < 17) {
-		int cur = sc[0];
-		double acc = 0.0;
-		for (int k = 0; k < cnt; k++) {
-			if (sc[k] != cur) {
-				atomicAdd(A.mu + 17 * int64_t(cur) + t, acc);
-				cur = sc[k];
-				acc = 0.0;
+		while (starts) {
+			const int b = __ffsll(starts) - 1;
+			starts &= starts - 1;
+			const int e = starts ? __ffsll(starts) - 1 : cnt;
+			double a0 = 0.0, a1 = 0.0;
+			int k = b;
+			for (; k + 2 <= e; k += 2) {
+				a0 += double(red[t][k]);
+				a1 += double(red[t][k + 1]);
 			}
-			acc += double(red[t][k]);
+			if (k < e) a0 += double(red[t][k]);
+			atomicAdd(A.mu + 17 * int64_t(sc[b]) + t, a0 + a1);
 		}
-		if (cnt > 0) atomicAdd(A.mu + 17 * int64_t(cur) + t, acc);
 	}
 }
 

@@ -548,19 +548,26 @@ __device__ __forceinline__ void node_segment_store(const float* L, int slot, int
 	for (int c = 0; c < 35; c++) red[c][lane] = L[c];
 	sl[lane] = slot;
 	__syncthreads();
+	// the wave's segments (runs of one slot) from a ballot of their first entries; lanes 0..34 then
+	// add one coefficient each over every segment, four FP64 partial sums per segment
+	const bool first = lane < cnt && (lane == 0 || sl[lane] != sl[lane - 1]);
+	unsigned long long starts = __ballot(first);
 	if (lane < 35) {
-		int cur = sl[0];
-		double acc = 0.0;
-		for (int j = 0; j < cnt; j++) {
-			const int sj = sl[j];
-			if (sj != cur) {
-				partial[35 * (size_t) cur + lane] = acc;
-				cur = sj;
-				acc = 0.0;
+		while (starts) {
+			const int b = __ffsll(starts) - 1;
+			starts &= starts - 1;
+			const int e = starts ? __ffsll(starts) - 1 : cnt;
+			double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+			int j = b;
+			for (; j + 4 <= e; j += 4) {
+				a0 += double(red[lane][j]);
+				a1 += double(red[lane][j + 1]);
+				a2 += double(red[lane][j + 2]);
+				a3 += double(red[lane][j + 3]);
 			}
-			acc += double(red[lane][j]);
+			for (; j < e; j++) a0 += double(red[lane][j]);
+			partial[35 * (size_t) sl[b] + lane] = (a0 + a1) + (a2 + a3);
 		}
-		partial[35 * (size_t) cur + lane] = acc;
 	}
 }
 
