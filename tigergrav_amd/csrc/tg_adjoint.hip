@@ -193,6 +193,20 @@ __device__ __forceinline__ float sepf(int a, int b, bool wrap) {
 	return float(double(a) - double(b)) * PINV;
 }
 
+// sum of row[0..n) with four partial sums (short dependency chains)
+__device__ __forceinline__ double sum4(const double* row, int n) {
+	double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+	int k = 0;
+	for (; k + 4 <= n; k += 4) {
+		a0 += row[k];
+		a1 += row[k + 1];
+		a2 += row[k + 2];
+		a3 += row[k + 3];
+	}
+	for (; k < n; k++) a0 += row[k];
+	return (a0 + a1) + (a2 + a3);
+}
+
 // ---- 1. leaves -------------------------------------------------------------------------------
 // L2P and the shifts: per leaf, writes each sink's gradient (the first term; later kernels add)
 __global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
@@ -221,11 +235,7 @@ __global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
 		for (int q = 0; q < 35; q++) sh[q * A.maxdepth + t] = Lt[q];
 	}
 	__syncthreads();
-	if (t < 35) {
-		double acc = 0.0;
-		for (int k = 0; k < depth; k++) acc += sh[t * A.maxdepth + k];
-		Ls[t] = acc;
-	}
+	if (t < 35) Ls[t] = sum4(sh + t * A.maxdepth, depth);
 	__syncthreads();
 	const bool sink = t < ns;
 	const int pi = lf.y + min(t, ns - 1);
@@ -268,11 +278,7 @@ __global__ __launch_bounds__(64) void k_adj_l2p(AdjArgs A) {
 	}
 	__syncthreads();
 	// lambda of the leaf; k_adj_up carries it to the ancestors
-	if (t < 35) {
-		double acc = 0.0;
-		for (int k = 0; k < ns; k++) acc += lam64[t * 64 + k];
-		A.Lam[35 * int64_t(lf.x) + t] = acc;
-	}
+	if (t < 35) A.Lam[35 * int64_t(lf.x) + t] = sum4(lam64 + t * 64, ns);
 	if (sink) {
 		double* o = A.grad + 4 * int64_t(pi);
 		o[0] = gx[0];

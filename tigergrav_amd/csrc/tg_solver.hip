@@ -87,9 +87,18 @@ __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, c
 	}
 	__syncthreads();
 	if (t < 35) {
-		double acc = 0.0;
-		for (int k = 0; k < depth; k++) acc += lds[t * maxdepth + k];
-		Ls[t] = acc;
+		// four partial sums over the path (root first within each), short dependency chains
+		const double* row = lds + t * maxdepth;
+		double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+		int k = 0;
+		for (; k + 4 <= depth; k += 4) {
+			a0 += row[k];
+			a1 += row[k + 1];
+			a2 += row[k + 2];
+			a3 += row[k + 3];
+		}
+		for (; k < depth; k++) a0 += row[k];
+		Ls[t] = (a0 + a1) + (a2 + a3);
 	}
 	__syncthreads();
 	// evaluate at the leaf's particles
