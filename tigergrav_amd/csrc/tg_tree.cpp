@@ -371,11 +371,44 @@ inline int32_t nid(int32_t c) { return c < 0 ? ~c : c; }
 
 struct WalkCtx {
 	std::vector<NodeLists>* out;
+	const std::vector<float>* rmax;  // per node: the span of its particles' bounding box
 	double theta_inv;
 	float r_ewald;
 };
 
 static void walk_rec(Tree& T, const WalkCtx& W, int ni, std::vector<int32_t> dlist, std::vector<int32_t> elist);
+
+// per node, the span of its particles' bounding box (src/tree.cpp:406, part_vect_range): leaves
+// from their particles, interior nodes as the union of their non-empty children's boxes (min and
+// max are exact, so this equals a scan of the node's particles)
+std::vector<float> Tree::sink_spans() const {
+	const size_t nn = nodes.size();
+	std::vector<double> mn(3 * nn, 1.0), mx(3 * nn, 0.0);
+	std::vector<float> span(nn, 0.0f);
+	for (size_t k = nn; k-- > 0;) {
+		const Node& nd = nodes[k];
+		if (nd.empty()) continue;
+		if (nd.leaf()) {
+			for (int64_t i = nd.pbegin; i < nd.pend; i++)
+				for (int d = 0; d < 3; d++) {
+					const double x = pos_to_double(pos[3 * i + d]);
+					mx[3 * k + d] = std::max(mx[3 * k + d], x);
+					mn[3 * k + d] = std::min(mn[3 * k + d], x);
+				}
+		} else {
+			for (int c = 0; c < 2; c++) {
+				const size_t ch = size_t(nd.child[c]);
+				if (nodes[ch].empty()) continue;
+				for (int d = 0; d < 3; d++) {
+					mx[3 * k + d] = std::max(mx[3 * k + d], mx[3 * ch + d]);
+					mn[3 * k + d] = std::min(mn[3 * k + d], mn[3 * ch + d]);
+				}
+			}
+		}
+		span[k] = float(range_max_span(&mn[3 * k], &mx[3 * k]));
+	}
+	return span;
+}
 
 void Tree::walk() {
 	std::vector<NodeLists> per(nodes.size());
@@ -383,6 +416,8 @@ void Tree::walk() {
 	W.out = &per;
 	W.theta_inv = 1.0 / P.theta;
 	W.r_ewald = float(2.0 * P.theta * std::pow(1.0e-3 / 8.0, 1.0 / 3.0));  // src/tree.cpp:403-404
+	const std::vector<float> rmax = sink_spans();
+	W.rmax = &rmax;
 	std::vector<int32_t> root = {0};
 #pragma omp parallel
 #pragma omp single nowait
@@ -434,19 +469,8 @@ static void walk_rec(Tree& T, const WalkCtx& W, int ni, std::vector<int32_t> dli
 	const double h = T.P.soft_len;
 	double X[3];
 	for (int d = 0; d < 3; d++) X[d] = pos_to_double(n.x[d]);
-	// rmax of the sink's particle range (src/tree.cpp:406): a leaf's own bbox; an interior node's
-	// range is that of its particles
-	float rmax;
-	{
-		double mn[3] = {1.0, 1.0, 1.0}, mx[3] = {0.0, 0.0, 0.0};
-		for (int64_t i = n.pbegin; i < n.pend; i++)
-			for (int d = 0; d < 3; d++) {
-				const double x = pos_to_double(T.pos[3 * i + d]);
-				mx[d] = std::max(mx[d], x);
-				mn[d] = std::min(mn[d], x);
-			}
-		rmax = float(range_max_span(mn, mx));
-	}
+	// rmax of the sink's particle range (src/tree.cpp:406)
+	const float rmax = (*W.rmax)[size_t(ni)];
 	auto sep = [&](const Node& c, double* dX) {
 		for (int d = 0; d < 3; d++) dX[d] = X[d] - pos_to_double(c.x[d]);
 	};

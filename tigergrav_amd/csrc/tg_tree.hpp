@@ -81,6 +81,8 @@ public:
 	void update_moments(const int32_t* pos_tree, const float* mass_tree);
 	// dual-tree walk from the root with the reference's MAC; fills `lists` and `census`
 	void walk();
+	// per node, the span of its particles' bounding box (the walk's sink rmax)
+	std::vector<float> sink_spans() const;
 
 private:
 	void refine(int node);
