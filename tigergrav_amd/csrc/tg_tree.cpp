@@ -445,19 +445,32 @@ void Tree::walk() {
 	flat(Ls.pp_d_ptr, Ls.pp_d, &NodeLists::pp_d);
 	flat(Ls.pc_e_ptr, Ls.pc_e, &NodeLists::pc_e);
 	flat(Ls.pp_e_ptr, Ls.pp_e, &NodeLists::pp_e);
-	// census (src/tree.cpp:483-494,618-628): node kinds count sources, leaf kinds sinks x sources
+	count_census();
+}
+
+// total length of the [begin, end) ranges v[b..e) (2 ints each)
+static uint64_t range_total(const std::vector<int32_t>& v, int64_t b, int64_t e) {
+	uint64_t t = 0;
+	for (int64_t k = b; k < e; k += 2) t += uint64_t(int64_t(v[size_t(k) + 1]) - int64_t(v[size_t(k)]));
+	return t;
+}
+
+// census (src/tree.cpp:483-494,618-628) of the flat lists: node kinds count sources, leaf kinds
+// sinks x sources
+void Tree::count_census() {
+	const Lists& Ls = lists;
 	Census C;
-	for (size_t i = 0; i < nodes.size(); i++) {
-		const auto& q = per[i];
-		C.CC_direct += q.cc_d.size();
-		C.CC_ewald += q.cc_e.size();
-		for (size_t k = 0; k < q.cp_d.size(); k += 2) C.CP_direct += uint64_t(q.cp_d[k + 1] - q.cp_d[k]);
-		for (size_t k = 0; k < q.cp_e.size(); k += 2) C.CP_ewald += uint64_t(q.cp_e[k + 1] - q.cp_e[k]);
+	const size_t nn = Ls.cc_d_ptr.empty() ? 0 : Ls.cc_d_ptr.size() - 1;
+	for (size_t i = 0; i < nn; i++) {
 		const uint64_t ns = uint64_t(nodes[i].pend - nodes[i].pbegin);
-		C.PC_direct += ns * q.pc_d.size();
-		C.PC_ewald += ns * q.pc_e.size();
-		for (size_t k = 0; k < q.pp_d.size(); k += 2) C.PP_direct += ns * uint64_t(q.pp_d[k + 1] - q.pp_d[k]);
-		for (size_t k = 0; k < q.pp_e.size(); k += 2) C.PP_ewald += ns * uint64_t(q.pp_e[k + 1] - q.pp_e[k]);
+		C.CC_direct += uint64_t(Ls.cc_d_ptr[i + 1] - Ls.cc_d_ptr[i]);
+		C.CC_ewald += uint64_t(Ls.cc_e_ptr[i + 1] - Ls.cc_e_ptr[i]);
+		C.CP_direct += range_total(Ls.cp_d, Ls.cp_d_ptr[i], Ls.cp_d_ptr[i + 1]);
+		C.CP_ewald += range_total(Ls.cp_e, Ls.cp_e_ptr[i], Ls.cp_e_ptr[i + 1]);
+		C.PC_direct += ns * uint64_t(Ls.pc_d_ptr[i + 1] - Ls.pc_d_ptr[i]);
+		C.PC_ewald += ns * uint64_t(Ls.pc_e_ptr[i + 1] - Ls.pc_e_ptr[i]);
+		C.PP_direct += ns * range_total(Ls.pp_d, Ls.pp_d_ptr[i], Ls.pp_d_ptr[i + 1]);
+		C.PP_ewald += ns * range_total(Ls.pp_e, Ls.pp_e_ptr[i], Ls.pp_e_ptr[i + 1]);
 	}
 	census = C;
 }

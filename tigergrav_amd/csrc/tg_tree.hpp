@@ -83,6 +83,8 @@ public:
 	void walk();
 	// per node, the span of its particles' bounding box (the walk's sink rmax)
 	std::vector<float> sink_spans() const;
+	// the interaction census of `lists`
+	void count_census();
 
 private:
 	void refine(int node);
