@@ -57,7 +57,9 @@ int tg_solver_adjoint(tg_solver* s, const float* w, double* grad, double* ms);
 int tg_solver_solve(tg_solver* s);
 /* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent
  * queue-driven kernel; "node_chunk" 1..64 sources per CC/CP partial slot (applies from the next
- * tg_solver_set_particles); "resident" work groups of the persistent form */
+ * tg_solver_set_particles); "resident" work groups of the persistent form; "gpu_walk" 0/1 builds
+ * the interaction lists on the device (the same lists as the host walk of kick_fmm,
+ * src/tree.cpp:378-635; from the next tg_solver_set_particles) */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);
 /* the same solve replayed from a HIP graph captured on its first call after set_particles (one
  * submission instead of nine launches; records no per-kind timings) */

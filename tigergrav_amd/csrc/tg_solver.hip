@@ -17,6 +17,7 @@
 #include "tg_gravity.h"
 #include "tg_local.hpp"
 #include "tg_solver.hpp"
+#include "tg_walk.hpp"
 #include "tg_solver.h"
 
 namespace tg {
@@ -248,7 +249,12 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	double t1 = now();
 	tree.compute_multipoles();
 	double t2 = now();
-	tree.walk();
+	if (gpu_walk) {
+		const hipError_t e = device_walk(tree, device, stream, err);
+		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
+	} else {
+		tree.walk();
+	}
 	double t3 = now();
 	t_tree = t1 - t0;
 	t_multipoles = t2 - t1;
@@ -854,6 +860,9 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 	} else if (k == "resident") {
 		if (value < 1) return TG_EINVAL;
 		S.resident_groups = value;
+	} else if (k == "gpu_walk") {
+		if (S.device < 0 && value != 0) return TG_EINVAL;
+		S.gpu_walk = value != 0;
 	} else {
 		g_solver_err = "unknown tuning knob " + k;
 		return TG_EINVAL;
