@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--pp-persistent", type=int, default=None, choices=(0, 1),
                     help="with a fixed --leaf: the persistent (1) or one-group-per-item (0) PP kernel")
     ap.add_argument("--node-chunk", type=int, default=None, help="with a fixed --leaf: CC/CP list entries per partial slot")
+    ap.add_argument("--host-walk", action="store_true",
+                    help="build the interaction lists with the host walk instead of the device walk")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager solves (with per-kind event timers) instead of HIP-graph replays")
     ap.add_argument("--accuracy-sample", type=int, default=10000,
@@ -239,6 +241,8 @@ def main():
         args.leaf = best["leaf"]
     free0 = device_free_bytes()
     s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
+    if not args.host_walk:
+        s.tune("gpu_walk", 1)
     if best is not None:
         from tigergrav_amd.autotune import apply
 
@@ -322,7 +326,7 @@ def main():
         "ms_per_step_eager": ms_eager,
         "s_per_force_solve_end_to_end": t_host + ms / 1e3,
         "host_build_s": {"tree": info["t_tree"], "multipoles": info["t_multipoles"], "lists": info["t_walk"],
-                         "upload": info["t_upload"]},
+                         "upload": info["t_upload"], "lists_on": "host" if args.host_walk else "device"},
         "interactions": count, "kernel_ms": kms,
         "tuning": {"best": best, "table": tuning} if tuning else None,
         "roofline": roofline,
