@@ -17,6 +17,7 @@
 #include "tg_gravity.h"
 #include "tg_local.hpp"
 #include "tg_solver.hpp"
+#include "tg_upload.hpp"
 #include "tg_walk.hpp"
 #include "tg_solver.h"
 
@@ -360,244 +361,34 @@ int Solver::upload() {
 	if (device >= 0) SCHK(hipSetDevice(device));
 	const Tree& T = tree;
 	const int64_t nn = int64_t(T.nodes.size());
-	// particles (x, y, z, mass bits), tree order
-	std::vector<int4> parts(n);
-	for (int64_t i = 0; i < n; i++) {
-		int w;
-		std::memcpy(&w, &T.mass[i], 4);
-		parts[i] = make_int4(T.pos[3 * i], T.pos[3 * i + 1], T.pos[3 * i + 2], w);
+	HostWork W;
+	{
+		const int rc = prepare_work(T, n, ewald, soft_len, node_chunk, W, err);
+		if (rc != TG_OK) return rc;
 	}
-	std::vector<MultiSrc> multis(nn);
-	std::vector<int4> npos(nn);
-	for (int64_t i = 0; i < nn; i++) {
-		const Node& nd = T.nodes[i];
-		std::memcpy(multis[i].m, nd.m, sizeof(nd.m));
-		std::memcpy(multis[i].x, nd.x, sizeof(nd.x));
-		npos[i] = make_int4(nd.x[0], nd.x[1], nd.x[2], 0);
-	}
-	// node kinds: each node's list is cut into chunks of <= node_chunk entries, one partial slot per
-	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64.
-	// Source ranges (CP, PP) are not flattened here: each becomes a run that the device expands
-	// (k_expand_runs), as do the partial-slot owners of the node kinds' entries.
-	const Lists& Ls = T.lists;
-	std::vector<int4> items[NKINDS];
-	std::vector<int> lists[NKINDS];  // host-built lists (PC kinds)
-	std::vector<ListRun> runs[NKINDS], owner_runs[NKINDS];
-	const std::vector<int32_t>* walk_list[NKINDS] = {};  // CC kinds: the walk's flat list, node order
-	walk_list[CC_DIRECT] = &Ls.cc_d;
-	walk_list[CC_EWALD] = &Ls.cc_e;
-	int64_t list_len[NKINDS] = {};
-	std::vector<int2> chunks[NKINDS];
-	std::vector<int> slot_node;
-	std::vector<int64_t> slot_ptr(nn + 1, 0);
-	int64_t slot = 0;
-	auto add_ranges = [&](int kind, const std::vector<int32_t>& ranges, int64_t b, int64_t e) {
-		for (int64_t k = b; k < e; k += 2) {
-			const int32_t len = ranges[k + 1] - ranges[k];
-			if (len <= 0) continue;
-			runs[kind].push_back(ListRun{ranges[k], len, list_len[kind]});
-			list_len[kind] += len;
-		}
-	};
-	const int chunk = std::min(64, std::max(1, node_chunk));
-	for (int64_t nd = 0; nd < nn; nd++) {
-		struct {
-			int kind;
-			const std::vector<int64_t>* ptr;
-			const std::vector<int32_t>* v;
-			bool ranges;
-		} src[4] = {{CC_DIRECT, &Ls.cc_d_ptr, &Ls.cc_d, false}, {CP_DIRECT, &Ls.cp_d_ptr, &Ls.cp_d, true},
-				{CC_EWALD, &Ls.cc_e_ptr, &Ls.cc_e, false}, {CP_EWALD, &Ls.cp_e_ptr, &Ls.cp_e, true}};
-		for (auto& q : src) {
-			const int64_t lb = list_len[q.kind];
-			if (q.ranges) {
-				add_ranges(q.kind, *q.v, (*q.ptr)[nd], (*q.ptr)[nd + 1]);
-			} else {
-				if ((*q.ptr)[nd] != lb) {
-					err = "node interaction list not in node order";
-					return TG_EINVAL;
-				}
-				list_len[q.kind] = (*q.ptr)[nd + 1];
-			}
-			const int64_t le = list_len[q.kind];
-			for (int64_t b = lb; b < le; b += chunk) {
-				const int64_t e = std::min<int64_t>(le, b + chunk);
-				chunks[q.kind].push_back(make_int2(int(b), int(e)));
-				owner_runs[q.kind].push_back(ListRun{int32_t(slot), int32_t(e - b), b});
-				slot_node.push_back(int(nd));
-				slot++;
-			}
-		}
-		slot_ptr[nd + 1] = slot;
-	}
-	std::vector<int2> waves[NKINDS];
-	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
-		int2 cur = make_int2(0, 0);
-		bool open = false;
-		for (const int2& c : chunks[k]) {
-			if (open && c.y - cur.x > 64) {
-				waves[k].push_back(cur);
-				open = false;
-			}
-			if (!open) {
-				cur = c;
-				open = true;
-			} else {
-				cur.y = c.y;
-			}
-		}
-		if (open) waves[k].push_back(cur);
-	}
-	n_slots = slot;
-	// leaf kinds: items (first sink, count <= 64, lb, le).  A PP-direct list is reordered so the
-	// source ranges whose bounding box comes within the softening length of the sink leaf's box
-	// (the leaf itself among them) come first; the item records that near prefix's length.
-	std::vector<int32_t> leaf_at(size_t(n) + 1, -1);
-	for (int32_t lf : T.leaves)
-		if (!T.nodes[lf].empty()) leaf_at[T.nodes[lf].pbegin] = lf;
-	const double hnear = soft_len * 1.001 + 1e-9;  // margin over the kernel's single-precision test
-	auto range_box = [&](int32_t b, int32_t e, double* mn, double* mx) {
-		const int32_t lf = leaf_at[b];
-		if (lf >= 0 && T.nodes[lf].pend == e) {
-			for (int d = 0; d < 3; d++) {
-				mn[d] = T.nodes[lf].rmin[d];
-				mx[d] = T.nodes[lf].rmax[d];
-			}
-			return;
-		}
-		for (int d = 0; d < 3; d++) {
-			mn[d] = 1.0;
-			mx[d] = 0.0;
-		}
-		for (int32_t i = b; i < e; i++)
-			for (int d = 0; d < 3; d++) {
-				const double x = pos_to_double(T.pos[3 * size_t(i) + d]);
-				mn[d] = std::min(mn[d], x);
-				mx[d] = std::max(mx[d], x);
-			}
-	};
-	auto may_be_near = [&](const Node& sk, const double* mn, const double* mx) {
-		double d2 = 0.0;
-		for (int d = 0; d < 3; d++) {
-			const double h1 = 0.5 * (sk.rmax[d] - sk.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
-			if (ewald && h1 + h2 > 0.25) return true;  // minimum image of the centres is not decisive
-			double dc = 0.5 * (sk.rmax[d] + sk.rmin[d]) - 0.5 * (mx[d] + mn[d]);
-			if (ewald) dc -= std::nearbyint(dc);
-			const double gap = std::fabs(dc) - h1 - h2;
-			if (gap > 0.0) d2 += gap * gap;
-		}
-		return d2 <= hnear * hnear;
-	};
-	// near classification of every PP-direct source range, in parallel over the leaves
-	std::vector<uint8_t> pp_near(Ls.pp_d.size() / 2, 0);
-#pragma omp parallel for schedule(dynamic, 64)
-	for (int64_t i = 0; i < int64_t(T.leaves.size()); i++) {
-		const int32_t lf = T.leaves[size_t(i)];
-		const Node& nd = T.nodes[lf];
-		if (nd.empty() || size_t(lf) + 1 >= Ls.pp_d_ptr.size()) continue;
-		for (int64_t k = Ls.pp_d_ptr[lf]; k < Ls.pp_d_ptr[lf + 1]; k += 2) {
-			double mn[3], mx[3];
-			range_box(Ls.pp_d[k], Ls.pp_d[k + 1], mn, mx);
-			pp_near[size_t(k / 2)] = may_be_near(nd, mn, mx) ? 1 : 0;
-		}
-	}
-	std::vector<int32_t> far_ranges;
-	std::vector<int4> leaf_items;
-	for (int32_t lf : T.leaves) {
-		const Node& nd = T.nodes[lf];
-		if (nd.empty()) continue;
-		leaf_items.push_back(make_int4(lf, nd.pbegin, nd.pend, 0));
-		struct {
-			int kind;
-			const std::vector<int64_t>* ptr;
-			const std::vector<int32_t>* v;
-			bool ranges;
-		} src[4] = {{PC_DIRECT, &Ls.pc_d_ptr, &Ls.pc_d, false}, {PP_DIRECT, &Ls.pp_d_ptr, &Ls.pp_d, true},
-				{PC_EWALD, &Ls.pc_e_ptr, &Ls.pc_e, false}, {PP_EWALD, &Ls.pp_e_ptr, &Ls.pp_e, true}};
-		for (auto& s : src) {
-			const int64_t lb = list_len[s.kind];
-			int64_t near_len = 0;
-			if (s.kind == PP_DIRECT) {
-				far_ranges.clear();
-				for (int64_t k = (*s.ptr)[lf]; k < (*s.ptr)[lf + 1]; k += 2) {
-					const int32_t b = (*s.v)[k], e = (*s.v)[k + 1];
-					if (pp_near[size_t(k / 2)]) {
-						if (e > b) {
-							runs[s.kind].push_back(ListRun{b, e - b, list_len[s.kind]});
-							list_len[s.kind] += e - b;
-						}
-					} else {
-						far_ranges.push_back(b);
-						far_ranges.push_back(e);
-					}
-				}
-				near_len = list_len[s.kind] - lb;
-				add_ranges(s.kind, far_ranges, 0, int64_t(far_ranges.size()));
-			} else if (s.kind == PP_EWALD) {
-				// the real-space images this leaf's pairs can need: |n| < 2.6 + the largest separation
-				double dmax2 = 0.0;
-				bool all = false;
-				for (int64_t k = (*s.ptr)[lf]; k < (*s.ptr)[lf + 1]; k += 2) {
-					const int32_t b = (*s.v)[k], e = (*s.v)[k + 1];
-					double mn[3], mx[3];
-					range_box(b, e, mn, mx);
-					double d2 = 0.0;
-					for (int d = 0; d < 3; d++) {
-						const double h1 = 0.5 * (nd.rmax[d] - nd.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
-						double dc = 0.5 * (nd.rmax[d] + nd.rmin[d]) - 0.5 * (mx[d] + mn[d]);
-						dc -= std::nearbyint(dc);
-						if (h1 + h2 > 0.25) all = true;
-						const double span = std::fabs(dc) + h1 + h2;
-						d2 += span * span;
-					}
-					dmax2 = std::max(dmax2, d2);
-				}
-				add_ranges(s.kind, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
-				near_len = all ? int64_t(PP_NEAR_ALL) : int64_t(ewald_images_within(2.6 + std::sqrt(dmax2) + 1e-6));
-			} else if (s.ranges) {
-				add_ranges(s.kind, *s.v, (*s.ptr)[lf], (*s.ptr)[lf + 1]);
-			} else {
-				std::vector<int>& L = lists[s.kind];
-				L.insert(L.end(), s.v->begin() + (*s.ptr)[lf], s.v->begin() + (*s.ptr)[lf + 1]);
-				list_len[s.kind] = int64_t(L.size());
-			}
-			const int64_t le = list_len[s.kind];
-			if (le == lb) continue;
-			for (int32_t b = nd.pbegin; b < nd.pend; b += 64) {
-				const int cnt = std::min<int32_t>(64, nd.pend - b);
-				const int y = (s.kind == PP_DIRECT || s.kind == PP_EWALD) ? pp_item_y(cnt, near_len) : cnt;
-				items[s.kind].push_back(make_int4(b, y, int(lb), int(le)));
-			}
-		}
-	}
-	// heaviest leaf items first: the dispatcher hands out work groups in order
-	for (int k : {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD}) {
-		std::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
-			const int na = a.y & PP_COUNT_MASK, nb = b.y & PP_COUNT_MASK;
-			const int64_t ca = int64_t(a.w - a.z) * (1 << (na <= 1 ? 0 : 32 - __builtin_clz(na - 1)));
-			const int64_t cb = int64_t(b.w - b.z) * (1 << (nb <= 1 ? 0 : 32 - __builtin_clz(nb - 1)));
-			return ca > cb;
-		});
-	}
-	n_leaf_items = int(leaf_items.size());
-	for (int k = 0; k < NKINDS; k++)
-		if (list_len[k] > int64_t(INT32_MAX)) {
-			err = "an interaction list exceeds 2^31 entries (32-bit list offsets)";
-			return TG_EINVAL;
-		}
-	// root-to-leaf paths for the local-expansion pass
-	std::vector<int> path, level(nn);
-	for (int64_t i = 0; i < nn; i++) level[i] = T.nodes[i].level;
 	if (T.max_level + 1 > TG_MAXDEPTH) {
 		err = "tree deeper than the local-expansion pass supports (64 levels)";
 		return TG_EINVAL;
 	}
-	for (auto& li : leaf_items) {
-		li.w = int(path.size());
-		std::vector<int> up;
-		for (int a = li.x; a >= 0; a = T.nodes[a].parent) up.push_back(a);
-		path.insert(path.end(), up.rbegin(), up.rend());
-	}
+	n_slots = W.n_slots;
+	n_leaf_items = int(W.leaf_items.size());
+	std::vector<int4>* items = W.items;
+	std::vector<int>* lists = W.lists;
+	std::vector<ListRun>* runs = W.runs;
+	std::vector<ListRun>* owner_runs = W.owner_runs;
+	std::vector<int2>* waves = W.waves;
+	const int64_t* list_len = W.list_len;
+	const std::vector<int32_t>* walk_list[NKINDS] = {};  // CC kinds: the walk's flat list, node order
+	walk_list[CC_DIRECT] = &T.lists.cc_d;
+	walk_list[CC_EWALD] = &T.lists.cc_e;
+	std::vector<int4>& parts = W.parts;
+	std::vector<MultiSrc>& multis = W.multis;
+	std::vector<int4>& npos = W.npos;
+	std::vector<int>& slot_node = W.slot_node;
+	std::vector<int64_t>& slot_ptr = W.slot_ptr;
+	std::vector<int4>& leaf_items = W.leaf_items;
+	std::vector<int>& path = W.path;
+	std::vector<int>& level = W.level;
 	if (device < 0) {  // host-only: keep the PP-direct work for inspection, its list expanded here
 		host_pp_items = std::move(items[PP_DIRECT]);
 		host_pp_list.assign(size_t(list_len[PP_DIRECT]), 0);

@@ -8,6 +8,7 @@
 
 #include "tg_kernels.hpp"
 #include "tg_tree.hpp"
+#include "tg_upload.hpp"
 
 namespace tg {
 
@@ -20,13 +21,6 @@ struct DeviceArray {
 	T* as() const {
 		return static_cast<T*>(p);
 	}
-};
-
-// a run of interaction-list entries expanded on the device: list[dst + l] = value + step * l, l < len
-struct ListRun {
-	int32_t value;
-	int32_t len;
-	int64_t dst;
 };
 
 // one kind's work on the device

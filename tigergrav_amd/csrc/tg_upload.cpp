@@ -1,0 +1,363 @@
+// tg_upload.cpp -- the host side of a solver upload: the particle and node records and the work
+// the batched kernels take (leaf items, node waves, list runs, partial slots, root paths), built
+// from the tree and its interaction lists.  Compiled by the host compiler with OpenMP.
+#include "tg_upload.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tg_gravity.h"
+
+namespace tg {
+
+int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err) {
+	const int64_t nn = int64_t(T.nodes.size());
+	// particles (x, y, z, mass bits), tree order
+	std::vector<int4> parts(n);
+#pragma omp parallel for schedule(static)
+	for (int64_t i = 0; i < n; i++) {
+		int w;
+		std::memcpy(&w, &T.mass[i], 4);
+		parts[i] = make_int4(T.pos[3 * i], T.pos[3 * i + 1], T.pos[3 * i + 2], w);
+	}
+	std::vector<MultiSrc> multis(nn);
+	std::vector<int4> npos(nn);
+#pragma omp parallel for schedule(static)
+	for (int64_t i = 0; i < nn; i++) {
+		const Node& nd = T.nodes[i];
+		std::memcpy(multis[i].m, nd.m, sizeof(nd.m));
+		std::memcpy(multis[i].x, nd.x, sizeof(nd.x));
+		npos[i] = make_int4(nd.x[0], nd.x[1], nd.x[2], 0);
+	}
+	// node kinds: each node's list is cut into chunks of <= node_chunk entries, one partial slot per
+	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64.
+	// Source ranges (CP, PP) are not flattened here: each becomes a run that the device expands
+	// (k_expand_runs), as do the partial-slot owners of the node kinds' entries.  Node and leaf
+	// work are built in three passes: per-node (per-leaf) counts in parallel, offsets by a serial
+	// prefix, then the runs, chunks and items in parallel at those offsets -- the arrays a single
+	// in-order pass would produce.
+	const Lists& Ls = T.lists;
+	std::vector<int4> items[NKINDS];
+	std::vector<int> lists[NKINDS];  // host-built lists (PC kinds)
+	std::vector<ListRun> runs[NKINDS], owner_runs[NKINDS];
+	int64_t list_len[NKINDS] = {};
+	std::vector<int2> chunks[NKINDS];
+	std::vector<int> slot_node;
+	std::vector<int64_t> slot_ptr(nn + 1, 0);
+	const int chunk = std::min(64, std::max(1, node_chunk));
+	struct KindSrc {
+		int kind;
+		const std::vector<int64_t>* ptr;
+		const std::vector<int32_t>* v;
+		bool ranges;
+	};
+	const KindSrc nsrc[4] = {{CC_DIRECT, &Ls.cc_d_ptr, &Ls.cc_d, false}, {CP_DIRECT, &Ls.cp_d_ptr, &Ls.cp_d, true},
+			{CC_EWALD, &Ls.cc_e_ptr, &Ls.cc_e, false}, {CP_EWALD, &Ls.cp_e_ptr, &Ls.cp_e, true}};
+	// entries and runs of a node's (a leaf's) ranges v[b..e)
+	auto range_counts = [](const std::vector<int32_t>& v, int64_t b, int64_t e, int64_t& ent, int64_t& rn) {
+		ent = rn = 0;
+		for (int64_t k = b; k < e; k += 2) {
+			const int64_t len = int64_t(v[size_t(k) + 1]) - int64_t(v[size_t(k)]);
+			if (len > 0) {
+				ent += len;
+				rn++;
+			}
+		}
+	};
+	std::vector<int64_t> nent(size_t(4 * nn)), nrun(size_t(4 * nn));
+#pragma omp parallel for schedule(static)
+	for (int64_t nd = 0; nd < nn; nd++)
+		for (int q = 0; q < 4; q++) {
+			const KindSrc& k = nsrc[q];
+			int64_t ent = 0, rn = 0;
+			if (k.ranges)
+				range_counts(*k.v, (*k.ptr)[nd], (*k.ptr)[nd + 1], ent, rn);
+			else
+				ent = (*k.ptr)[nd + 1] - (*k.ptr)[nd];
+			nent[size_t(4 * nd + q)] = ent;
+			nrun[size_t(4 * nd + q)] = rn;
+		}
+	std::vector<int64_t> oent(size_t(4 * nn)), orun(size_t(4 * nn)), ochk(size_t(4 * nn)), oslot(size_t(4 * nn));
+	int64_t trun[4] = {}, tchk[4] = {}, slot = 0;
+	for (int64_t nd = 0; nd < nn; nd++) {
+		for (int q = 0; q < 4; q++) {
+			const KindSrc& k = nsrc[q];
+			const size_t i = size_t(4 * nd + q);
+			if (!k.ranges && (*k.ptr)[nd] != list_len[k.kind]) {
+				err = "node interaction list not in node order";
+				return TG_EINVAL;
+			}
+			const int64_t nch = (nent[i] + chunk - 1) / chunk;
+			oent[i] = list_len[k.kind];
+			list_len[k.kind] += nent[i];
+			orun[i] = trun[q];
+			trun[q] += nrun[i];
+			ochk[i] = tchk[q];
+			tchk[q] += nch;
+			oslot[i] = slot;
+			slot += nch;
+		}
+		slot_ptr[nd + 1] = slot;
+	}
+	for (int q = 0; q < 4; q++) {
+		runs[nsrc[q].kind].resize(size_t(trun[q]));
+		chunks[nsrc[q].kind].resize(size_t(tchk[q]));
+		owner_runs[nsrc[q].kind].resize(size_t(tchk[q]));
+	}
+	slot_node.resize(size_t(slot));
+#pragma omp parallel for schedule(static)
+	for (int64_t nd = 0; nd < nn; nd++)
+		for (int q = 0; q < 4; q++) {
+			const KindSrc& k = nsrc[q];
+			const size_t i = size_t(4 * nd + q);
+			if (k.ranges) {
+				int64_t pos = oent[i], r = orun[i];
+				for (int64_t j = (*k.ptr)[nd]; j < (*k.ptr)[nd + 1]; j += 2) {
+					const int32_t b = (*k.v)[size_t(j)], len = (*k.v)[size_t(j) + 1] - b;
+					if (len <= 0) continue;
+					runs[k.kind][size_t(r++)] = ListRun{b, len, pos};
+					pos += len;
+				}
+			}
+			const int64_t lb = oent[i], le = lb + nent[i];
+			int64_t c = ochk[i], sl = oslot[i];
+			for (int64_t b = lb; b < le; b += chunk, c++, sl++) {
+				const int64_t e = std::min<int64_t>(le, b + chunk);
+				chunks[k.kind][size_t(c)] = make_int2(int(b), int(e));
+				owner_runs[k.kind][size_t(c)] = ListRun{int32_t(sl), int32_t(e - b), b};
+				slot_node[size_t(sl)] = int(nd);
+			}
+		}
+	std::vector<int2> waves[NKINDS];
+	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
+		int2 cur = make_int2(0, 0);
+		bool open = false;
+		for (const int2& c : chunks[k]) {
+			if (open && c.y - cur.x > 64) {
+				waves[k].push_back(cur);
+				open = false;
+			}
+			if (!open) {
+				cur = c;
+				open = true;
+			} else {
+				cur.y = c.y;
+			}
+		}
+		if (open) waves[k].push_back(cur);
+	}
+	W.n_slots = slot;
+	// leaf kinds: items (first sink, count <= 64, lb, le).  A PP-direct list is reordered so the
+	// source ranges whose bounding box comes within the softening length of the sink leaf's box
+	// (the leaf itself among them) come first; the item records that near prefix's length.
+	std::vector<int32_t> leaf_at(size_t(n) + 1, -1);
+	for (int32_t lf : T.leaves)
+		if (!T.nodes[lf].empty()) leaf_at[T.nodes[lf].pbegin] = lf;
+	const double hnear = soft_len * 1.001 + 1e-9;  // margin over the kernel's single-precision test
+	auto range_box = [&](int32_t b, int32_t e, double* mn, double* mx) {
+		const int32_t lf = leaf_at[b];
+		if (lf >= 0 && T.nodes[lf].pend == e) {
+			for (int d = 0; d < 3; d++) {
+				mn[d] = T.nodes[lf].rmin[d];
+				mx[d] = T.nodes[lf].rmax[d];
+			}
+			return;
+		}
+		for (int d = 0; d < 3; d++) {
+			mn[d] = 1.0;
+			mx[d] = 0.0;
+		}
+		for (int32_t i = b; i < e; i++)
+			for (int d = 0; d < 3; d++) {
+				const double x = pos_to_double(T.pos[3 * size_t(i) + d]);
+				mn[d] = std::min(mn[d], x);
+				mx[d] = std::max(mx[d], x);
+			}
+	};
+	auto may_be_near = [&](const Node& sk, const double* mn, const double* mx) {
+		double d2 = 0.0;
+		for (int d = 0; d < 3; d++) {
+			const double h1 = 0.5 * (sk.rmax[d] - sk.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
+			if (ewald && h1 + h2 > 0.25) return true;  // minimum image of the centres is not decisive
+			double dc = 0.5 * (sk.rmax[d] + sk.rmin[d]) - 0.5 * (mx[d] + mn[d]);
+			if (ewald) dc -= std::nearbyint(dc);
+			const double gap = std::fabs(dc) - h1 - h2;
+			if (gap > 0.0) d2 += gap * gap;
+		}
+		return d2 <= hnear * hnear;
+	};
+	// the real-space images a leaf's PP-Ewald pairs can need: |n| < 2.6 + the largest separation
+	auto pp_ewald_images = [&](const Node& nd, int64_t kb, int64_t ke) {
+		double dmax2 = 0.0;
+		bool all = false;
+		for (int64_t k = kb; k < ke; k += 2) {
+			double mn[3], mx[3];
+			range_box(Ls.pp_e[size_t(k)], Ls.pp_e[size_t(k) + 1], mn, mx);
+			double d2 = 0.0;
+			for (int d = 0; d < 3; d++) {
+				const double h1 = 0.5 * (nd.rmax[d] - nd.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
+				double dc = 0.5 * (nd.rmax[d] + nd.rmin[d]) - 0.5 * (mx[d] + mn[d]);
+				dc -= std::nearbyint(dc);
+				if (h1 + h2 > 0.25) all = true;
+				const double span = std::fabs(dc) + h1 + h2;
+				d2 += span * span;
+			}
+			dmax2 = std::max(dmax2, d2);
+		}
+		return all ? int64_t(PP_NEAR_ALL) : int64_t(ewald_images_within(2.6 + std::sqrt(dmax2) + 1e-6));
+	};
+	std::vector<int32_t> lvs;  // non-empty leaves, in the tree's leaf order
+	for (int32_t lf : T.leaves)
+		if (!T.nodes[lf].empty()) lvs.push_back(lf);
+	const int64_t nl = int64_t(lvs.size());
+	const KindSrc lsrc[4] = {{PC_DIRECT, &Ls.pc_d_ptr, &Ls.pc_d, false}, {PP_DIRECT, &Ls.pp_d_ptr, &Ls.pp_d, true},
+			{PC_EWALD, &Ls.pc_e_ptr, &Ls.pc_e, false}, {PP_EWALD, &Ls.pp_e_ptr, &Ls.pp_e, true}};
+	// per leaf and kind: entries, runs, near-prefix length (PP direct) or image count (PP Ewald), items
+	std::vector<int64_t> lent(size_t(4 * nl)), lrun(size_t(4 * nl)), lnear(size_t(4 * nl)), litem(size_t(4 * nl));
+	std::vector<uint8_t> pp_near(Ls.pp_d.size() / 2, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+	for (int64_t i = 0; i < nl; i++) {
+		const int32_t lf = lvs[size_t(i)];
+		const Node& nd = T.nodes[lf];
+		const int64_t per_leaf = (int64_t(nd.pend - nd.pbegin) + 63) / 64;
+		for (int q = 0; q < 4; q++) {
+			const KindSrc& k = lsrc[q];
+			const size_t o = size_t(4 * i + q);
+			int64_t ent = 0, rn = 0, nearl = 0;
+			if (size_t(lf) + 1 < k.ptr->size()) {
+				const int64_t kb = (*k.ptr)[lf], ke = (*k.ptr)[lf + 1];
+				if (k.kind == PP_DIRECT) {
+					for (int64_t j = kb; j < ke; j += 2) {
+						const int32_t b = (*k.v)[size_t(j)], e = (*k.v)[size_t(j) + 1];
+						double mn[3], mx[3];
+						range_box(b, e, mn, mx);
+						const bool near = may_be_near(nd, mn, mx);
+						pp_near[size_t(j / 2)] = near ? 1 : 0;
+						if (e > b) {
+							ent += e - b;
+							rn++;
+							if (near) nearl += e - b;
+						}
+					}
+				} else if (k.kind == PP_EWALD) {
+					range_counts(*k.v, kb, ke, ent, rn);
+					nearl = pp_ewald_images(nd, kb, ke);
+				} else {
+					ent = ke - kb;
+				}
+			}
+			lent[o] = ent;
+			lrun[o] = rn;
+			lnear[o] = nearl;
+			litem[o] = ent > 0 ? per_leaf : 0;
+		}
+	}
+	std::vector<int64_t> olent(size_t(4 * nl)), olrun(size_t(4 * nl)), olitem(size_t(4 * nl));
+	{
+		int64_t lrt[4] = {}, lit[4] = {};
+		for (int64_t i = 0; i < nl; i++)
+			for (int q = 0; q < 4; q++) {
+				const size_t o = size_t(4 * i + q);
+				olent[o] = list_len[lsrc[q].kind];
+				list_len[lsrc[q].kind] += lent[o];
+				olrun[o] = lrt[q];
+				lrt[q] += lrun[o];
+				olitem[o] = lit[q];
+				lit[q] += litem[o];
+			}
+		for (int q = 0; q < 4; q++) {
+			const int kind = lsrc[q].kind;
+			if (lsrc[q].ranges)
+				runs[kind].resize(size_t(lrt[q]));
+			else
+				lists[kind].resize(size_t(list_len[kind]));
+			items[kind].resize(size_t(lit[q]));
+		}
+	}
+	std::vector<int4> leaf_items(static_cast<size_t>(nl));
+#pragma omp parallel for schedule(dynamic, 64)
+	for (int64_t i = 0; i < nl; i++) {
+		const int32_t lf = lvs[size_t(i)];
+		const Node& nd = T.nodes[lf];
+		leaf_items[size_t(i)] = make_int4(lf, nd.pbegin, nd.pend, 0);
+		for (int q = 0; q < 4; q++) {
+			const KindSrc& k = lsrc[q];
+			const size_t o = size_t(4 * i + q);
+			const int64_t lb = olent[o], le = lb + lent[o];
+			if (le == lb) continue;
+			const int64_t kb = (*k.ptr)[lf], ke = (*k.ptr)[lf + 1];
+			if (k.ranges) {
+				int64_t r = olrun[o], pos = lb;
+				// PP direct: the near ranges first, then the far ones; PP Ewald: in list order
+				for (int pass = 0; pass < (k.kind == PP_DIRECT ? 2 : 1); pass++)
+					for (int64_t j = kb; j < ke; j += 2) {
+						const int32_t b = (*k.v)[size_t(j)], len = (*k.v)[size_t(j) + 1] - b;
+						if (len <= 0) continue;
+						if (k.kind == PP_DIRECT && (pp_near[size_t(j / 2)] != 0) != (pass == 0)) continue;
+						runs[k.kind][size_t(r++)] = ListRun{b, len, pos};
+						pos += len;
+					}
+			} else {
+				std::copy(k.v->begin() + kb, k.v->begin() + ke, lists[k.kind].begin() + lb);
+			}
+			int64_t it = olitem[o];
+			for (int32_t b = nd.pbegin; b < nd.pend; b += 64) {
+				const int cnt = std::min<int32_t>(64, nd.pend - b);
+				const int y = (k.kind == PP_DIRECT || k.kind == PP_EWALD) ? pp_item_y(cnt, lnear[o]) : cnt;
+				items[k.kind][size_t(it++)] = make_int4(b, y, int(lb), int(le));
+			}
+		}
+	}
+	// heaviest leaf items first: the dispatcher hands out work groups in order
+	const int leaf_kinds[4] = {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD};
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int q = 0; q < 4; q++) {
+		const int k = leaf_kinds[q];
+		std::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
+			const int na = a.y & PP_COUNT_MASK, nb = b.y & PP_COUNT_MASK;
+			const int64_t ca = int64_t(a.w - a.z) * (1 << (na <= 1 ? 0 : 32 - __builtin_clz(na - 1)));
+			const int64_t cb = int64_t(b.w - b.z) * (1 << (nb <= 1 ? 0 : 32 - __builtin_clz(nb - 1)));
+			return ca > cb;
+		});
+	}
+	
+	for (int k = 0; k < NKINDS; k++)
+		if (list_len[k] > int64_t(INT32_MAX)) {
+			err = "an interaction list exceeds 2^31 entries (32-bit list offsets)";
+			return TG_EINVAL;
+		}
+	// root-to-leaf paths for the local-expansion pass
+	std::vector<int> path, level(nn);
+	for (int64_t i = 0; i < nn; i++) level[i] = T.nodes[i].level;
+	{
+		std::vector<int64_t> poff(size_t(nl) + 1, 0);
+		for (int64_t i = 0; i < nl; i++) poff[size_t(i) + 1] = poff[size_t(i)] + T.nodes[lvs[size_t(i)]].level + 1;
+		path.resize(size_t(poff[size_t(nl)]));
+#pragma omp parallel for schedule(static)
+		for (int64_t i = 0; i < nl; i++) {
+			leaf_items[size_t(i)].w = int(poff[size_t(i)]);
+			int64_t p = poff[size_t(i) + 1];
+			for (int a = leaf_items[size_t(i)].x; a >= 0; a = T.nodes[a].parent) path[size_t(--p)] = a;
+		}
+	}
+	W.parts = std::move(parts);
+	W.multis = std::move(multis);
+	W.npos = std::move(npos);
+	for (int k = 0; k < NKINDS; k++) {
+		W.items[k] = std::move(items[k]);
+		W.lists[k] = std::move(lists[k]);
+		W.runs[k] = std::move(runs[k]);
+		W.owner_runs[k] = std::move(owner_runs[k]);
+		W.waves[k] = std::move(waves[k]);
+		W.list_len[k] = list_len[k];
+	}
+	W.slot_node = std::move(slot_node);
+	W.slot_ptr = std::move(slot_ptr);
+	W.leaf_items = std::move(leaf_items);
+	W.path = std::move(path);
+	W.level = std::move(level);
+	return TG_OK;
+}
+
+}  // namespace tg

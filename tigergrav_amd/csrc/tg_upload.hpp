@@ -1,0 +1,46 @@
+// tg_upload.hpp -- host-side preparation of a solver's device work (tg_upload.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "tg_kernels.hpp"
+#include "tg_tree.hpp"
+
+namespace tg {
+
+// a run of interaction-list entries expanded on the device: list[dst + l] = value + step * l, l < len
+struct ListRun {
+	int32_t value;
+	int32_t len;
+	int64_t dst;
+	// no zero fill when a run table is sized: every entry is written afterwards, in parallel
+	ListRun() {}
+	ListRun(int32_t v, int32_t l, int64_t d) : value(v), len(l), dst(d) {}
+};
+
+// everything a solver uploads, in host memory
+struct HostWork {
+	std::vector<int4> parts;      // (x, y, z, mass bits), tree order
+	std::vector<MultiSrc> multis;
+	std::vector<int4> npos;       // node centres
+	std::vector<int4> items[NKINDS];        // leaf kinds: (first sink, count | extra, list begin, end)
+	std::vector<int> lists[NKINDS];         // lists built on the host (PC kinds)
+	std::vector<ListRun> runs[NKINDS];      // range kinds (CP, PP): runs the device expands
+	std::vector<ListRun> owner_runs[NKINDS];  // node kinds: partial-slot owner of every entry, as runs
+	std::vector<int2> waves[NKINDS];        // node kinds: [begin, end) entry runs of <= 64
+	int64_t list_len[NKINDS] = {};
+	std::vector<int> slot_node;
+	std::vector<int64_t> slot_ptr;
+	std::vector<int4> leaf_items;  // (node, first particle, end, root-path offset)
+	std::vector<int> path, level;
+	int64_t n_slots = 0;
+};
+
+// builds W from the tree and its lists (n particles; node_chunk entries per CC/CP partial slot);
+// TG_OK or a negative status with err set
+int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err);
+
+}  // namespace tg
