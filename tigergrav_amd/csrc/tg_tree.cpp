@@ -84,8 +84,23 @@ void Tree::build(const int32_t* x, const float* masses, int64_t n, const TreePar
 	root.boxid = 1;
 	nodes.push_back(root);
 	// breadth-first: the reference refines one level per pass; any order of splitting disjoint
-	// ranges gives the same particle arrangement
-	for (size_t i = 0; i < nodes.size(); i++) refine(int(i));
+	// ranges gives the same particle arrangement, so the nodes of a level split in parallel, their
+	// children numbered in node order (as a sequential pass appends them)
+	for (size_t lb = 0; lb < nodes.size();) {
+		const size_t le = nodes.size();
+		std::vector<int32_t> first(le - lb, -1);
+		size_t nc = le;
+		for (size_t i = lb; i < le; i++)
+			if (nodes[i].pend - nodes[i].pbegin > P.parts_per_node) {
+				first[i - lb] = int32_t(nc);
+				nc += 2;
+			}
+		nodes.resize(nc);
+#pragma omp parallel for schedule(dynamic, 1)
+		for (int64_t i = int64_t(lb); i < int64_t(le); i++)
+			if (first[size_t(i) - lb] >= 0) refine(int(i), first[size_t(i) - lb]);
+		lb = le;
+	}
 	leaves.clear();
 	max_level = 0;
 	for (size_t i = 0; i < nodes.size(); i++) {
@@ -95,9 +110,8 @@ void Tree::build(const int32_t* x, const float* masses, int64_t n, const TreePar
 }
 
 // tree::refine (src/tree.cpp:136-172) with part_vect_sort (src/part_vect.cpp:895-933)
-void Tree::refine(int ni) {
+void Tree::refine(int ni, int li) {
 	Node nd = nodes[ni];
-	if (nd.pend - nd.pbegin <= P.parts_per_node) return;
 	double bmin[3], bmax[3];
 	box_id_to_range(nd.boxid, bmin, bmax);
 	double max_span = 0.0;
@@ -143,9 +157,8 @@ void Tree::refine(int ni) {
 	r.boxid = (nd.boxid << 1) + 1;
 	l.level = r.level = nd.level + 1;
 	l.parent = r.parent = ni;
-	const int li = int(nodes.size());
-	nodes.push_back(l);
-	nodes.push_back(r);
+	nodes[li] = l;
+	nodes[li + 1] = r;
 	nodes[ni].child[0] = li;
 	nodes[ni].child[1] = li + 1;
 }
@@ -230,8 +243,16 @@ void Tree::update_moments(const int32_t* pos_tree, const float* mass_tree) {
 }
 
 void Tree::compute_multipoles() {
-	// children come after their parents in `nodes`, so a reverse sweep is bottom-up
-	for (int i = int(nodes.size()) - 1; i >= 0; i--) multipoles(i);
+	// nodes are numbered level by level, children after their parents: levels from the deepest up,
+	// the nodes of a level in parallel
+	std::vector<size_t> level_begin;
+	for (size_t i = 0; i < nodes.size(); i++)
+		if (i == 0 || nodes[i].level != nodes[i - 1].level) level_begin.push_back(i);
+	level_begin.push_back(nodes.size());
+	for (size_t l = level_begin.size() - 1; l-- > 0;) {
+#pragma omp parallel for schedule(dynamic, 64)
+		for (int64_t i = int64_t(level_begin[l]); i < int64_t(level_begin[l + 1]); i++) multipoles(int(i));
+	}
 }
 
 void Tree::multipoles(int ni) {

@@ -87,7 +87,7 @@ public:
 	void count_census();
 
 private:
-	void refine(int node);
+	void refine(int node, int first_child);
 	void multipoles(int node);
 	struct Scratch;
 	void walk_node(int node, std::vector<int32_t>& dlist, std::vector<int32_t>& elist, std::vector<std::vector<int32_t>>* per_node);
