@@ -406,27 +406,36 @@ std::vector<float> Tree::sink_spans() const {
 	const size_t nn = nodes.size();
 	std::vector<double> mn(3 * nn, 1.0), mx(3 * nn, 0.0);
 	std::vector<float> span(nn, 0.0f);
-	for (size_t k = nn; k-- > 0;) {
-		const Node& nd = nodes[k];
-		if (nd.empty()) continue;
-		if (nd.leaf()) {
-			for (int64_t i = nd.pbegin; i < nd.pend; i++)
-				for (int d = 0; d < 3; d++) {
-					const double x = pos_to_double(pos[3 * i + d]);
-					mx[3 * k + d] = std::max(mx[3 * k + d], x);
-					mn[3 * k + d] = std::min(mn[3 * k + d], x);
-				}
-		} else {
-			for (int c = 0; c < 2; c++) {
-				const size_t ch = size_t(nd.child[c]);
-				if (nodes[ch].empty()) continue;
-				for (int d = 0; d < 3; d++) {
-					mx[3 * k + d] = std::max(mx[3 * k + d], mx[3 * ch + d]);
-					mn[3 * k + d] = std::min(mn[3 * k + d], mn[3 * ch + d]);
+	// nodes are numbered level by level: levels from the deepest up, each level's nodes in parallel
+	std::vector<size_t> level_begin;
+	for (size_t i = 0; i < nn; i++)
+		if (i == 0 || nodes[i].level != nodes[i - 1].level) level_begin.push_back(i);
+	level_begin.push_back(nn);
+	for (size_t l = level_begin.size() - 1; l-- > 0;) {
+#pragma omp parallel for schedule(dynamic, 64)
+		for (int64_t ki = int64_t(level_begin[l]); ki < int64_t(level_begin[l + 1]); ki++) {
+			const size_t k = size_t(ki);
+			const Node& nd = nodes[k];
+			if (nd.empty()) continue;
+			if (nd.leaf()) {
+				for (int64_t i = nd.pbegin; i < nd.pend; i++)
+					for (int d = 0; d < 3; d++) {
+						const double x = pos_to_double(pos[3 * i + d]);
+						mx[3 * k + d] = std::max(mx[3 * k + d], x);
+						mn[3 * k + d] = std::min(mn[3 * k + d], x);
+					}
+			} else {
+				for (int c = 0; c < 2; c++) {
+					const size_t ch = size_t(nd.child[c]);
+					if (nodes[ch].empty()) continue;
+					for (int d = 0; d < 3; d++) {
+						mx[3 * k + d] = std::max(mx[3 * k + d], mx[3 * ch + d]);
+						mn[3 * k + d] = std::min(mn[3 * k + d], mn[3 * ch + d]);
+					}
 				}
 			}
+			span[k] = float(range_max_span(&mn[3 * k], &mx[3 * k]));
 		}
-		span[k] = float(range_max_span(&mn[3 * k], &mx[3 * k]));
 	}
 	return span;
 }

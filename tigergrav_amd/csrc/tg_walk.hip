@@ -12,10 +12,12 @@
 //   DRAIN  (leaf n, one checklist, direct or Ewald): one round of the leaf's drain loop
 //          (src/tree.cpp:530-602): opened leaves become PP ranges, far nodes PC, the rest is
 //          opened or split for the next round.
-// Per step: k_walk_count counts every unit's outputs; the host turns the counts into output
-// offsets and the next step's units (a few integers per unit); k_walk_write writes.  Each node's
-// outputs are recorded as segments in step order, which is the host walk's order, and gathered
-// into node-order CSR at the end.  The MAC arithmetic is the host walk's, in FP64 without
+// Per step, on the device: k_walk_count counts every unit's outputs, k_walk_sizes turns the counts
+// into per-unit sizes (entries of each output, of the next checklists, next units), exclusive
+// scans give the offsets, k_walk_write writes and k_walk_units emits the next step's units; the
+// host reads back only the step's totals (to size the pools).  Each node's outputs are segments
+// in step order, which is the host walk's order; k_walk_place puts them into node-order CSR at
+// the end.  The MAC arithmetic is the host walk's, in FP64 without
 // contraction, so every decision is bit-identical.
 #include "tg_walk.hpp"
 
@@ -23,6 +25,8 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 namespace tg {
 namespace {
@@ -49,19 +53,11 @@ struct WalkParams {
 	int ewald;
 };
 
-// output pointers and per-unit offsets (six per unit: FIRST cc_d, cp_d, next_d, cc_e, cp_e,
-// next_e; DRAIN pp, pc, next)
-struct WalkOut {
-	int32_t *cc_d, *cp_d, *cc_e, *cp_e, *pp_d, *pc_d, *pp_e, *pc_e;
-	int32_t* next;
-	const int64_t* offs;
-	const int32_t* counts;
-	// capacities of the output pools (cc_d, cp_d, cc_e, cp_e, pc_d, pp_d, pc_e, pp_e) and of next: a
-	// write past one is dropped and flagged, never made (the count pass makes the same decisions,
-	// so none is expected)
-	int64_t cap[9];
-	int32_t* overflow;
-};
+// the node of every unit of a step (kept for the final placement)
+__global__ __launch_bounds__(64) void k_walk_nodes_of(const WalkUnit* __restrict__ units, int nu, int32_t* __restrict__ nodes_of) {
+	const int i = blockIdx.x * 64 + threadIdx.x;
+	if (i < nu) nodes_of[i] = units[i].node;
+}
 
 __device__ __forceinline__ double pos_d(int32_t x) {
 #pragma clang fp contract(off)
@@ -195,40 +191,144 @@ __device__ void write_pass(const WalkParams& P, const WalkNode& n, const double*
 	}
 }
 
-__global__ __launch_bounds__(64) void k_walk_write(WalkParams P, const WalkUnit* __restrict__ units, int nunits, WalkOut O) {
+enum { K_CC_D, K_CP_D, K_CC_E, K_CP_E, K_PC_D, K_PP_D, K_PC_E, K_PP_E, K_N };
+constexpr int COL_NEXT = K_N, COL_UNITS = K_N + 1, NCOL = K_N + 2;  // sizes: 8 outputs, next lists, next units
+
+// per unit, the entries of each output, of its next checklists and the number of its next units
+__global__ __launch_bounds__(64) void k_walk_sizes(const WalkNode* __restrict__ nodes, const WalkUnit* __restrict__ units, int nu,
+		const int32_t* __restrict__ counts, int ewald, int64_t* __restrict__ sizes) {
 	const int i = blockIdx.x * 64 + threadIdx.x;
-	if (i >= nunits) return;
+	if (i >= nu) return;
+	const WalkUnit u = units[i];
+	const int32_t* c = counts + 8 * int64_t(i);
+	int64_t z[NCOL] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+	if (u.mode == MODE_FIRST) {
+		const int64_t ld = int64_t(c[2]) + 2 * int64_t(c[3]);
+		const int64_t le = ewald ? int64_t(c[6]) + 2 * int64_t(c[7]) : 0;
+		z[K_CC_D] = c[0];
+		z[K_CP_D] = 2 * int64_t(c[1]);
+		if (ewald) {
+			z[K_CC_E] = c[4];
+			z[K_CP_E] = 2 * int64_t(c[5]);
+		}
+		z[COL_NEXT] = ld + le;
+		const WalkNode n = nodes[u.node];
+		if (n.c0 >= 0)
+			z[COL_UNITS] = int64_t(nodes[n.c0].pend > nodes[n.c0].pbegin) + int64_t(nodes[n.c1].pend > nodes[n.c1].pbegin);
+		else
+			z[COL_UNITS] = int64_t(ld > 0) + int64_t(le > 0);
+	} else {
+		const bool ew = u.mode == MODE_DRAIN_E;
+		const int64_t l = int64_t(c[2]) + 2 * int64_t(c[3]);
+		z[ew ? K_PP_E : K_PP_D] = 2 * int64_t(c[0]);
+		z[ew ? K_PC_E : K_PC_D] = c[1];
+		z[COL_NEXT] = l;
+		z[COL_UNITS] = l > 0;
+	}
+	for (int k = 0; k < NCOL; k++) sizes[int64_t(k) * nu + i] = z[k];
+}
+
+// every column's total: exclusive offset plus size of the last unit
+__global__ void k_walk_totals(const int64_t* __restrict__ sizes, const int64_t* __restrict__ offs, int nu, int64_t* __restrict__ tot) {
+	const int k = threadIdx.x;
+	if (k < NCOL) tot[k] = nu > 0 ? offs[int64_t(k) * nu + nu - 1] + sizes[int64_t(k) * nu + nu - 1] : 0;
+}
+
+struct WalkPools {
+	int32_t* out[K_N];   // output pools, appended step by step
+	int64_t base[K_N];   // this step's start in each output pool
+	int64_t cap[K_N + 1];
+	int32_t* next;       // the next step's checklists
+	int32_t* overflow;
+};
+
+__global__ __launch_bounds__(64) void k_walk_write(WalkParams P, const WalkUnit* __restrict__ units, int nu, const int32_t* __restrict__ counts,
+		const int64_t* __restrict__ offs, WalkPools W) {
+	const int i = blockIdx.x * 64 + threadIdx.x;
+	if (i >= nu) return;
 	const WalkUnit u = units[i];
 	const WalkNode n = P.nodes[u.node];
 	const double X[3] = {pos_d(n.x[0]), pos_d(n.x[1]), pos_d(n.x[2])};
-	const int64_t* o = O.offs + 6 * int64_t(i);
-	const int32_t* c = O.counts + 8 * int64_t(i);
+	const int32_t* c = counts + 8 * int64_t(i);
+	auto at = [&](int k) { return W.base[k] + offs[int64_t(k) * nu + i]; };
+	const int64_t nb = offs[int64_t(COL_NEXT) * nu + i];
 	if (u.mode == MODE_FIRST) {
-		write_pass(P, n, X, P.lists + u.off_d, u.len_d, false, false, O.cc_d, o[0], O.cap[0], O.cp_d, o[1], O.cap[1], O.next, o[2], O.cap[8], c[2],
-				O.overflow);
+		write_pass(P, n, X, P.lists + u.off_d, u.len_d, false, false, W.out[K_CC_D], at(K_CC_D), W.cap[K_CC_D], W.out[K_CP_D], at(K_CP_D),
+				W.cap[K_CP_D], W.next, nb, W.cap[K_N], c[2], W.overflow);
 		if (P.ewald)
-			write_pass(P, n, X, P.lists + u.off_e, u.len_e, true, false, O.cc_e, o[3], O.cap[2], O.cp_e, o[4], O.cap[3], O.next, o[5], O.cap[8],
-					c[6], O.overflow);
+			write_pass(P, n, X, P.lists + u.off_e, u.len_e, true, false, W.out[K_CC_E], at(K_CC_E), W.cap[K_CC_E], W.out[K_CP_E], at(K_CP_E),
+					W.cap[K_CP_E], W.next, nb + int64_t(c[2]) + 2 * int64_t(c[3]), W.cap[K_N], c[6], W.overflow);
 	} else if (u.mode == MODE_DRAIN_D) {
-		write_pass(P, n, X, P.lists + u.off_d, u.len_d, false, true, O.pp_d, o[0], O.cap[5], O.pc_d, o[1], O.cap[4], O.next, o[2], O.cap[8], c[2],
-				O.overflow);
+		write_pass(P, n, X, P.lists + u.off_d, u.len_d, false, true, W.out[K_PP_D], at(K_PP_D), W.cap[K_PP_D], W.out[K_PC_D], at(K_PC_D),
+				W.cap[K_PC_D], W.next, nb, W.cap[K_N], c[2], W.overflow);
 	} else {
-		write_pass(P, n, X, P.lists + u.off_d, u.len_d, true, true, O.pp_e, o[0], O.cap[7], O.pc_e, o[1], O.cap[6], O.next, o[2], O.cap[8], c[2],
-				O.overflow);
+		write_pass(P, n, X, P.lists + u.off_d, u.len_d, true, true, W.out[K_PP_E], at(K_PP_E), W.cap[K_PP_E], W.out[K_PC_E], at(K_PC_E),
+				W.cap[K_PC_E], W.next, nb, W.cap[K_N], c[2], W.overflow);
 	}
 }
 
-// dst[d + l] = src[s + l] for every run (s, d, len)
-struct CopyRun {
-	int64_t src, dst, len;
-};
-__global__ __launch_bounds__(64) void k_gather_runs(const CopyRun* __restrict__ runs, int64_t nruns, const int32_t* __restrict__ src,
-		int32_t* __restrict__ dst) {
-	const int64_t r0 = int64_t(blockIdx.x) * 64;
-	const int64_t r1 = r0 + 64 < nruns ? r0 + 64 : nruns;
-	for (int64_t r = r0; r < r1; r++) {
-		const CopyRun g = runs[r];
-		for (int64_t l = threadIdx.x; l < g.len; l += 64) dst[g.dst + l] = src[g.src + l];
+// the next step's units: an interior node's first pass hands its two next lists to its non-empty
+// children, a leaf's to its drains; a drain round with a non-empty next list continues
+__global__ __launch_bounds__(64) void k_walk_units(const WalkNode* __restrict__ nodes, const WalkUnit* __restrict__ units, int nu,
+		const int32_t* __restrict__ counts, const int64_t* __restrict__ offs, int ewald, WalkUnit* __restrict__ next_units, int cap) {
+	const int i = blockIdx.x * 64 + threadIdx.x;
+	if (i >= nu) return;
+	const WalkUnit u = units[i];
+	const int32_t* c = counts + 8 * int64_t(i);
+	int64_t q = offs[int64_t(COL_UNITS) * nu + i];
+	const int64_t nb = offs[int64_t(COL_NEXT) * nu + i];
+	auto emit = [&](WalkUnit w) {
+		if (q < cap) next_units[q] = w;
+		q++;
+	};
+	if (u.mode == MODE_FIRST) {
+		const int32_t ld = c[2] + 2 * c[3];
+		const int32_t le = ewald ? c[6] + 2 * c[7] : 0;
+		const WalkNode n = nodes[u.node];
+		if (n.c0 >= 0) {
+			const int ch[2] = {n.c0, n.c1};
+			for (int k = 0; k < 2; k++)
+				if (nodes[ch[k]].pend > nodes[ch[k]].pbegin) emit(WalkUnit{ch[k], MODE_FIRST, ld, le, nb, nb + ld});
+		} else {
+			if (ld > 0) emit(WalkUnit{u.node, MODE_DRAIN_D, ld, 0, nb, 0});
+			if (le > 0) emit(WalkUnit{u.node, MODE_DRAIN_E, le, 0, nb + ld, 0});
+		}
+	} else {
+		const int32_t l = c[2] + 2 * c[3];
+		if (l > 0) emit(WalkUnit{u.node, u.mode, l, 0, nb, 0});
+	}
+}
+
+// per node and output kind, the entries of its segment in this step (one unit per node and kind
+// per step at most: one first pass, one direct and one Ewald drain round)
+__global__ __launch_bounds__(64) void k_walk_node_len(const int32_t* __restrict__ nodes_of, int nu, const int64_t* __restrict__ sizes,
+		int64_t nn, int64_t* __restrict__ node_len) {
+	const int i = blockIdx.x * 64 + threadIdx.x;
+	if (i >= nu) return;
+	const int64_t nd = nodes_of[i];
+	for (int k = 0; k < K_N; k++) {
+		// only the one unit of (node, kind) with entries touches its counter (a direct and an Ewald
+		// drain of one leaf run in the same step and share the node)
+		const int64_t len = sizes[int64_t(k) * nu + i];
+		if (len > 0) node_len[int64_t(k) * (nn + 1) + nd] += len;
+	}
+}
+
+// a step's segments into the node-order lists: dst = ptr[node] + fill[node]
+__global__ __launch_bounds__(64) void k_walk_place(const int32_t* __restrict__ nodes_of, int nu, const int64_t* __restrict__ sizes,
+		const int64_t* __restrict__ offs, const int64_t* __restrict__ base, int64_t nn, const int64_t* __restrict__ ptr,
+		int64_t* __restrict__ fill, WalkPools src, int32_t* const* __restrict__ dst) {
+	const int i = blockIdx.x * 64 + threadIdx.x;
+	if (i >= nu) return;
+	const int64_t nd = nodes_of[i];
+	for (int k = 0; k < K_N; k++) {
+		const int64_t len = sizes[int64_t(k) * nu + i];
+		if (len == 0) continue;
+		const int64_t from = base[k] + offs[int64_t(k) * nu + i];
+		int64_t& f = fill[int64_t(k) * (nn + 1) + nd];
+		const int64_t to = ptr[int64_t(k) * (nn + 1) + nd] + f;
+		for (int64_t l = 0; l < len; l++) dst[k][to + l] = src.out[k][from + l];
+		f += len;
 	}
 }
 
@@ -236,18 +336,18 @@ __global__ __launch_bounds__(64) void k_gather_runs(const CopyRun* __restrict__ 
 struct Pool {
 	int32_t* p = nullptr;
 	int64_t cap = 0;
-	hipError_t reserve(int64_t n, hipStream_t s) {
+	hipError_t reserve(int64_t n, hipStream_t s, bool keep = true) {
 		if (n <= cap) return hipSuccess;
 		int64_t nc = std::max<int64_t>(n, cap * 2);
 		nc = std::max<int64_t>(nc, 1024);
 		int32_t* q = nullptr;
 		hipError_t e = hipMalloc(&q, size_t(nc) * sizeof(int32_t));
 		if (e != hipSuccess) return e;
-		if (p && cap > 0) {
+		if (p && cap > 0 && keep) {
 			if ((e = hipMemcpyAsync(q, p, size_t(cap) * sizeof(int32_t), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
 			if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-			(void) hipFree(p);
 		}
+		if (p) (void) hipFree(p);
 		p = q;
 		cap = nc;
 		return hipSuccess;
@@ -259,12 +359,15 @@ struct Pool {
 	}
 };
 
-struct Seg {
-	int32_t node;
-	int64_t off, len;
+// one step's record for the final placement: its units' nodes, sizes and offsets (device) and
+// the output bases (host)
+struct StepLog {
+	int32_t* nodes_of = nullptr;
+	int64_t* sizes = nullptr;
+	int64_t* offs = nullptr;
+	int nu = 0;
+	int64_t base[K_N];
 };
-
-enum { K_CC_D, K_CP_D, K_CC_E, K_CP_E, K_PC_D, K_PP_D, K_PC_E, K_PP_E, K_N };
 
 }  // namespace
 
@@ -272,18 +375,23 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 	const int64_t nn = int64_t(T.nodes.size());
 	hipError_t e = hipSuccess;
 	void* dev_nodes = nullptr;
-	void* dev_units = nullptr;
-	void* dev_counts = nullptr;
-	void* dev_offs = nullptr;
+	WalkUnit* dev_units = nullptr;
+	WalkUnit* dev_next_units = nullptr;
+	int32_t* dev_counts = nullptr;
+	int64_t* dev_tot = nullptr;
+	int64_t* dev_node_len = nullptr;
+	int64_t* dev_fill = nullptr;
+	int64_t* dev_base = nullptr;
+	int32_t** dev_dst = nullptr;
 	int32_t* dev_overflow = nullptr;
+	void* scan_tmp = nullptr;
+	size_t scan_tmp_bytes = 0;
 	int32_t overflow = 0;
-	int64_t units_cap = 0;
-	Pool lists_in, lists_out, out[K_N];
-	std::vector<Seg> segs[K_N];
+	int64_t units_cap = 0, next_units_cap = 0;
+	Pool lists_in, lists_out, out[K_N], fin[K_N];
+	std::vector<StepLog> log;
 	int64_t tot[K_N] = {};
-	std::vector<WalkUnit> units, next_units;
-	std::vector<int32_t> counts;
-	std::vector<int64_t> offs;
+	int nu = 0;
 	WalkParams P{};
 #define WCHK(call)                                     \
 	do {                                               \
@@ -313,6 +421,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 	}
 	WCHK(hipMalloc(&dev_overflow, sizeof(int32_t)));
 	WCHK(hipMemsetAsync(dev_overflow, 0, sizeof(int32_t), s));
+	WCHK(hipMalloc(&dev_tot, NCOL * sizeof(int64_t)));
 	P.nodes = static_cast<const WalkNode*>(dev_nodes);
 	P.theta_inv = 1.0 / T.P.theta;
 	P.h = T.P.soft_len;
@@ -321,146 +430,161 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 	if (nn > 0 && T.nodes[0].pend > T.nodes[0].pbegin) {
 		// the root's checklists: {root} for both kinds
 		const int32_t root[2] = {0, 0};
+		const WalkUnit r0{0, MODE_FIRST, 1, P.ewald ? 1 : 0, 0, 1};
 		WCHK(lists_in.reserve(2, s));
 		WCHK(hipMemcpyAsync(lists_in.p, root, sizeof(root), hipMemcpyHostToDevice, s));
-		units.push_back(WalkUnit{0, MODE_FIRST, 1, P.ewald ? 1 : 0, 0, 1});
+		units_cap = 1024;
+		WCHK(hipMalloc(&dev_units, size_t(units_cap) * sizeof(WalkUnit)));
+		WCHK(hipMemcpyAsync(dev_units, &r0, sizeof(r0), hipMemcpyHostToDevice, s));
+		nu = 1;
 	}
-	while (!units.empty()) {
-		const int nu = int(units.size());
-		if (nu > units_cap) {
-			if (dev_units) (void) hipFree(dev_units);
-			if (dev_counts) (void) hipFree(dev_counts);
-			if (dev_offs) (void) hipFree(dev_offs);
-			dev_units = dev_counts = dev_offs = nullptr;
-			units_cap = std::max<int64_t>(nu, 2 * units_cap);
-			WCHK(hipMalloc(&dev_units, size_t(units_cap) * sizeof(WalkUnit)));
-			WCHK(hipMalloc(&dev_counts, size_t(units_cap) * 8 * sizeof(int32_t)));
-			WCHK(hipMalloc(&dev_offs, size_t(units_cap) * 6 * sizeof(int64_t)));
-		}
-		WCHK(hipMemcpyAsync(dev_units, units.data(), size_t(nu) * sizeof(WalkUnit), hipMemcpyHostToDevice, s));
+	while (nu > 0) {
+		const unsigned g = unsigned((nu + 63) / 64);
+		StepLog L;
+		L.nu = nu;
+		for (int k = 0; k < K_N; k++) L.base[k] = tot[k];
+		WCHK(hipMalloc(&L.nodes_of, size_t(nu) * sizeof(int32_t)));
+		WCHK(hipMalloc(&L.sizes, size_t(nu) * NCOL * sizeof(int64_t)));
+		WCHK(hipMalloc(&L.offs, size_t(nu) * NCOL * sizeof(int64_t)));
+		log.push_back(L);
+		WCHK(hipMalloc(&dev_counts, size_t(nu) * 8 * sizeof(int32_t)));
 		P.lists = lists_in.p;
-		hipLaunchKernelGGL(k_walk_count, dim3(unsigned((nu + 63) / 64)), dim3(64), 0, s, P, static_cast<const WalkUnit*>(dev_units), nu,
-				static_cast<int32_t*>(dev_counts));
+		hipLaunchKernelGGL(k_walk_count, dim3(g), dim3(64), 0, s, P, dev_units, nu, dev_counts);
 		WCHK(hipGetLastError());
-		counts.resize(size_t(nu) * 8);
-		WCHK(hipMemcpyAsync(counts.data(), dev_counts, counts.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-		WCHK(hipStreamSynchronize(s));
-		// offsets, segments and the next step's units
-		offs.assign(size_t(nu) * 6, 0);
-		next_units.clear();
-		int64_t next_len = 0;
-		for (int i = 0; i < nu; i++) {
-			const WalkUnit& u = units[size_t(i)];
-			const int32_t* c = &counts[size_t(i) * 8];
-			int64_t* o = &offs[size_t(i) * 6];
-			auto put = [&](int kind, int64_t len) {
-				const int64_t off = tot[kind];
-				if (len > 0) segs[kind].push_back(Seg{u.node, off, len});
-				tot[kind] += len;
-				return off;
-			};
-			const Node& nd = T.nodes[size_t(u.node)];
-			if (u.mode == MODE_FIRST) {
-				o[0] = put(K_CC_D, c[0]);
-				o[1] = put(K_CP_D, 2 * int64_t(c[1]));
-				const int64_t ld = int64_t(c[2]) + 2 * int64_t(c[3]);
-				o[2] = next_len;
-				next_len += ld;
-				int64_t le = 0;
-				if (P.ewald) {
-					o[3] = put(K_CC_E, c[4]);
-					o[4] = put(K_CP_E, 2 * int64_t(c[5]));
-					le = int64_t(c[6]) + 2 * int64_t(c[7]);
-					o[5] = next_len;
-					next_len += le;
-				}
-				if (!nd.leaf()) {
-					for (int k = 0; k < 2; k++) {
-						const int ch = nd.child[k];
-						if (T.nodes[size_t(ch)].empty()) continue;
-						next_units.push_back(WalkUnit{ch, MODE_FIRST, int32_t(ld), int32_t(le), o[2], o[5]});
-					}
-				} else {
-					if (ld > 0) next_units.push_back(WalkUnit{u.node, MODE_DRAIN_D, int32_t(ld), 0, o[2], 0});
-					if (le > 0) next_units.push_back(WalkUnit{u.node, MODE_DRAIN_E, int32_t(le), 0, o[5], 0});
-				}
-			} else {
-				const bool ew = u.mode == MODE_DRAIN_E;
-				o[0] = put(ew ? K_PP_E : K_PP_D, 2 * int64_t(c[0]));
-				o[1] = put(ew ? K_PC_E : K_PC_D, c[1]);
-				const int64_t l = int64_t(c[2]) + 2 * int64_t(c[3]);
-				o[2] = next_len;
-				next_len += l;
-				if (l > 0) next_units.push_back(WalkUnit{u.node, u.mode, int32_t(l), 0, o[2], 0});
+		hipLaunchKernelGGL(k_walk_sizes, dim3(g), dim3(64), 0, s, P.nodes, dev_units, nu, dev_counts, P.ewald, L.sizes);
+		WCHK(hipGetLastError());
+		hipLaunchKernelGGL(k_walk_nodes_of, dim3(g), dim3(64), 0, s, dev_units, nu, L.nodes_of);
+		WCHK(hipGetLastError());
+		{
+			size_t need = 0;
+			WCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, L.sizes, L.offs, nu, s));
+			if (need > scan_tmp_bytes) {
+				if (scan_tmp) (void) hipFree(scan_tmp);
+				scan_tmp = nullptr;
+				scan_tmp_bytes = std::max(need, 2 * scan_tmp_bytes);
+				WCHK(hipMalloc(&scan_tmp, scan_tmp_bytes));
+			}
+			for (int k = 0; k < NCOL; k++) {
+				size_t b = scan_tmp_bytes;
+				WCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, L.sizes + int64_t(k) * nu, L.offs + int64_t(k) * nu, nu, s));
 			}
 		}
-		if (next_len > INT32_MAX) {
+		hipLaunchKernelGGL(k_walk_totals, dim3(1), dim3(64), 0, s, L.sizes, L.offs, nu, dev_tot);
+		WCHK(hipGetLastError());
+		int64_t t[NCOL];
+		WCHK(hipMemcpyAsync(t, dev_tot, sizeof(t), hipMemcpyDeviceToHost, s));
+		WCHK(hipStreamSynchronize(s));
+		if (t[COL_NEXT] > INT32_MAX || t[COL_UNITS] > INT32_MAX) {
 			err = "walk checklists exceed 2^31 entries";
 			e = hipErrorInvalidValue;
 			goto done;
 		}
-		for (int k = 0; k < K_N; k++) WCHK(out[k].reserve(std::max<int64_t>(tot[k], 1), s));
-		WCHK(lists_out.reserve(std::max<int64_t>(next_len, 1), s));
-		WCHK(hipMemcpyAsync(dev_offs, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+		for (int k = 0; k < K_N; k++) {
+			WCHK(out[k].reserve(std::max<int64_t>(tot[k] + t[k], 1), s));
+			tot[k] += t[k];
+		}
+		WCHK(lists_out.reserve(std::max<int64_t>(t[COL_NEXT], 1), s, false));
+		if (t[COL_UNITS] > next_units_cap) {
+			if (dev_next_units) (void) hipFree(dev_next_units);
+			dev_next_units = nullptr;
+			next_units_cap = std::max<int64_t>(t[COL_UNITS], 2 * next_units_cap);
+			WCHK(hipMalloc(&dev_next_units, size_t(next_units_cap) * sizeof(WalkUnit)));
+		}
 		{
-			WalkOut O{out[K_CC_D].p, out[K_CP_D].p, out[K_CC_E].p, out[K_CP_E].p, out[K_PP_D].p, out[K_PC_D].p, out[K_PP_E].p, out[K_PC_E].p,
-					lists_out.p, static_cast<const int64_t*>(dev_offs), static_cast<const int32_t*>(dev_counts),
-					{out[K_CC_D].cap, out[K_CP_D].cap, out[K_CC_E].cap, out[K_CP_E].cap, out[K_PC_D].cap, out[K_PP_D].cap, out[K_PC_E].cap,
-							out[K_PP_E].cap, lists_out.cap},
-					dev_overflow};
-			hipLaunchKernelGGL(k_walk_write, dim3(unsigned((nu + 63) / 64)), dim3(64), 0, s, P, static_cast<const WalkUnit*>(dev_units), nu, O);
+			WalkPools W{};
+			for (int k = 0; k < K_N; k++) {
+				W.out[k] = out[k].p;
+				W.base[k] = L.base[k];
+				W.cap[k] = out[k].cap;
+			}
+			W.cap[K_N] = lists_out.cap;
+			W.next = lists_out.p;
+			W.overflow = dev_overflow;
+			hipLaunchKernelGGL(k_walk_write, dim3(g), dim3(64), 0, s, P, dev_units, nu, dev_counts, L.offs, W);
+			WCHK(hipGetLastError());
+			hipLaunchKernelGGL(k_walk_units, dim3(g), dim3(64), 0, s, P.nodes, dev_units, nu, dev_counts, L.offs, P.ewald, dev_next_units,
+					int(next_units_cap));
 			WCHK(hipGetLastError());
 		}
-		std::swap(lists_in, lists_out);
-		units.swap(next_units);
-		// the units and offsets arrays are rewritten next step: wait for this step's kernels first
-		WCHK(hipMemcpyAsync(&overflow, dev_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
 		WCHK(hipStreamSynchronize(s));
-		if (overflow) {
-			err = "device walk: a write pass disagreed with its count pass";
-			e = hipErrorUnknown;
-			goto done;
-		}
+		(void) hipFree(dev_counts);
+		dev_counts = nullptr;
+		std::swap(lists_in, lists_out);
+		std::swap(dev_units, dev_next_units);
+		std::swap(units_cap, next_units_cap);
+		nu = int(t[COL_UNITS]);
+	}
+	WCHK(hipMemcpyAsync(&overflow, dev_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+	WCHK(hipStreamSynchronize(s));
+	if (overflow) {
+		err = "device walk: a write pass disagreed with its count pass";
+		e = hipErrorUnknown;
+		goto done;
 	}
 	{
-		// node-order CSR per kind: each node's segments in step order
+		// node-order CSR per kind: per-node lengths over all steps, their scan, then every step's
+		// segments placed in step order
+		const int64_t np = nn + 1;
+		WCHK(hipMalloc(&dev_node_len, size_t(K_N * np) * sizeof(int64_t)));
+		WCHK(hipMalloc(&dev_fill, size_t(K_N * np) * sizeof(int64_t)));
+		WCHK(hipMemsetAsync(dev_node_len, 0, size_t(K_N * np) * sizeof(int64_t), s));
+		WCHK(hipMemsetAsync(dev_fill, 0, size_t(K_N * np) * sizeof(int64_t), s));
+		for (const StepLog& L : log) {
+			hipLaunchKernelGGL(k_walk_node_len, dim3(unsigned((L.nu + 63) / 64)), dim3(64), 0, s, L.nodes_of, L.nu, L.sizes, nn, dev_node_len);
+			WCHK(hipGetLastError());
+		}
+		int64_t* dev_ptr = nullptr;
+		WCHK(hipMalloc(&dev_ptr, size_t(K_N * np) * sizeof(int64_t)));
+		for (int k = 0; k < K_N; k++) {
+			size_t need = 0;
+			e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, dev_node_len + k * np, dev_ptr + k * np, np, s);
+			if (e == hipSuccess && need > scan_tmp_bytes) {
+				if (scan_tmp) (void) hipFree(scan_tmp);
+				scan_tmp = nullptr;
+				scan_tmp_bytes = need;
+				e = hipMalloc(&scan_tmp, scan_tmp_bytes);
+			}
+			size_t b = scan_tmp_bytes;
+			if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, dev_node_len + k * np, dev_ptr + k * np, np, s);
+			if (e == hipSuccess) e = fin[k].reserve(std::max<int64_t>(tot[k], 1), s, false);
+			if (e != hipSuccess) {
+				(void) hipFree(dev_ptr);
+				err = std::string("list assembly: ") + hipGetErrorString(e);
+				goto done;
+			}
+		}
+		int32_t* dst_h[K_N];
+		for (int k = 0; k < K_N; k++) dst_h[k] = fin[k].p;
+		e = hipMalloc(&dev_dst, sizeof(dst_h));
+		if (e == hipSuccess) e = hipMemcpyAsync(dev_dst, dst_h, sizeof(dst_h), hipMemcpyHostToDevice, s);
+		if (e == hipSuccess) e = hipMalloc(&dev_base, K_N * sizeof(int64_t) * std::max<size_t>(log.size(), 1));
+		WalkPools src{};
+		for (int k = 0; k < K_N; k++) src.out[k] = out[k].p;
+		for (size_t st = 0; st < log.size() && e == hipSuccess; st++) {
+			const StepLog& L = log[st];
+			e = hipMemcpyAsync(dev_base + K_N * st, L.base, K_N * sizeof(int64_t), hipMemcpyHostToDevice, s);
+			if (e != hipSuccess) break;
+			hipLaunchKernelGGL(k_walk_place, dim3(unsigned((L.nu + 63) / 64)), dim3(64), 0, s, L.nodes_of, L.nu, L.sizes, L.offs,
+					dev_base + K_N * st, nn, dev_ptr, dev_fill, src, dev_dst);
+			e = hipGetLastError();
+		}
 		Lists& Ls = T.lists;
 		Ls = Lists();
 		std::vector<int64_t>* ptrs[K_N] = {&Ls.cc_d_ptr, &Ls.cp_d_ptr, &Ls.cc_e_ptr, &Ls.cp_e_ptr,
 				&Ls.pc_d_ptr, &Ls.pp_d_ptr, &Ls.pc_e_ptr, &Ls.pp_e_ptr};
 		std::vector<int32_t>* vals[K_N] = {&Ls.cc_d, &Ls.cp_d, &Ls.cc_e, &Ls.cp_e, &Ls.pc_d, &Ls.pp_d, &Ls.pc_e, &Ls.pp_e};
-		std::vector<int64_t> fill(size_t(nn), 0);
-		for (int k = 0; k < K_N; k++) {
-			std::vector<int64_t>& ptr = *ptrs[k];
-			ptr.assign(size_t(nn) + 1, 0);
-			for (const Seg& g : segs[k]) ptr[size_t(g.node) + 1] += g.len;
-			for (int64_t i = 0; i < nn; i++) ptr[size_t(i) + 1] += ptr[size_t(i)];
-			vals[k]->assign(size_t(tot[k]), 0);
-			if (tot[k] == 0) continue;
-			std::fill(fill.begin(), fill.end(), 0);
-			std::vector<CopyRun> runs;
-			runs.reserve(segs[k].size());
-			for (const Seg& g : segs[k]) {
-				runs.push_back(CopyRun{g.off, ptr[size_t(g.node)] + fill[size_t(g.node)], g.len});
-				fill[size_t(g.node)] += g.len;
-			}
-			Pool dst, rt;
-			e = dst.reserve(tot[k], s);
-			if (e == hipSuccess) e = rt.reserve(int64_t(runs.size() * sizeof(CopyRun) / sizeof(int32_t)) + 1, s);
-			if (e == hipSuccess) e = hipMemcpyAsync(rt.p, runs.data(), runs.size() * sizeof(CopyRun), hipMemcpyHostToDevice, s);
-			if (e == hipSuccess) {
-				hipLaunchKernelGGL(k_gather_runs, dim3(unsigned((runs.size() + 63) / 64)), dim3(64), 0, s,
-						reinterpret_cast<const CopyRun*>(rt.p), int64_t(runs.size()), out[k].p, dst.p);
-				e = hipGetLastError();
-			}
-			if (e == hipSuccess) e = hipMemcpyAsync(vals[k]->data(), dst.p, size_t(tot[k]) * sizeof(int32_t), hipMemcpyDeviceToHost, s);
-			if (e == hipSuccess) e = hipStreamSynchronize(s);
-			dst.release();
-			rt.release();
-			if (e != hipSuccess) {
-				err = std::string("list gather: ") + hipGetErrorString(e);
-				goto done;
-			}
+		for (int k = 0; k < K_N && e == hipSuccess; k++) {
+			ptrs[k]->resize(size_t(np));
+			vals[k]->resize(size_t(tot[k]));
+			e = hipMemcpyAsync(ptrs[k]->data(), dev_ptr + k * np, size_t(np) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
+			if (e == hipSuccess && tot[k] > 0)
+				e = hipMemcpyAsync(vals[k]->data(), fin[k].p, size_t(tot[k]) * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+		}
+		if (e == hipSuccess) e = hipStreamSynchronize(s);
+		(void) hipFree(dev_ptr);
+		if (e != hipSuccess) {
+			err = std::string("list assembly: ") + hipGetErrorString(e);
+			goto done;
 		}
 		T.count_census();
 	}
@@ -469,12 +593,24 @@ done:
 	(void) hipStreamSynchronize(s);
 	if (dev_nodes) (void) hipFree(dev_nodes);
 	if (dev_units) (void) hipFree(dev_units);
+	if (dev_next_units) (void) hipFree(dev_next_units);
 	if (dev_counts) (void) hipFree(dev_counts);
-	if (dev_offs) (void) hipFree(dev_offs);
+	if (dev_tot) (void) hipFree(dev_tot);
+	if (dev_node_len) (void) hipFree(dev_node_len);
+	if (dev_fill) (void) hipFree(dev_fill);
+	if (dev_base) (void) hipFree(dev_base);
+	if (dev_dst) (void) hipFree(dev_dst);
 	if (dev_overflow) (void) hipFree(dev_overflow);
+	if (scan_tmp) (void) hipFree(scan_tmp);
+	for (auto& L : log) {
+		if (L.nodes_of) (void) hipFree(L.nodes_of);
+		if (L.sizes) (void) hipFree(L.sizes);
+		if (L.offs) (void) hipFree(L.offs);
+	}
 	lists_in.release();
 	lists_out.release();
 	for (auto& p : out) p.release();
+	for (auto& p : fin) p.release();
 	return e;
 }
 
