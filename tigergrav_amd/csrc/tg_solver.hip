@@ -53,44 +53,50 @@ __global__ __launch_bounds__(256) void k_node_sum(const int64_t* __restrict__ sl
 	dL[i] = acc;
 }
 
-// The local-expansion pass of one leaf (block of 64).  The shift of a degree-4 Taylor polynomial
-// is exact, so shifts compose and the leaf's expansion is the sum over its root-to-leaf path of
-// every node's own contribution dL_a re-expanded about the leaf centre:
+// The local-expansion pass, LPG leaves per block of 64 (64 / LPG lanes each).  The shift of a
+// degree-4 Taylor polynomial is exact, so shifts compose and a leaf's expansion is the sum over its
+// root-to-leaf path of every node's own contribution dL_a re-expanded about the leaf centre:
 //   L_leaf = sum_a shift(dL_a, X_leaf - X_a)
 // (the reference shifts the running sum node by node, src/tree.cpp:400-401; the two agree up to
-// rounding).  Lane k shifts the k-th node of the path with the unrolled 210-FMA re-expansion, so no
-// lane waits on a table; lanes 0..34 then add the path's terms coefficient by coefficient, root
-// first.  leaf item: (node, pbegin, pend, path begin); the path has level(node) + 1 <= TG_MAXDEPTH
-// entries.
+// rounding).  Lane k of a leaf shifts the k-th node of its path (and k + 64 / LPG, ...) with the
+// unrolled 210-FMA re-expansion, so no lane waits on a table; the leaf's lanes then add the path's
+// terms coefficient by coefficient, root first, and evaluate at its particles.  Two leaves per
+// block put the lanes a single path leaves idle to work.  leaf item: (node, pbegin, pend, path
+// begin); the path has level(node) + 1 <= TG_MAXDEPTH entries.
 #define TG_MAXDEPTH 64
-static_assert(TG_MAXDEPTH <= 64, "one lane per path entry");
-__global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, const int* __restrict__ path,
+template <int LPG>
+__global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, int nleaves, const int* __restrict__ path,
 		const int* __restrict__ level, const int4* __restrict__ pos, const int4* __restrict__ parts,
 		const double* __restrict__ dL, double* __restrict__ out, double phi_self, int maxdepth, int wrap) {
-	const int4 lf = leaves[blockIdx.x];
+	constexpr int W = 64 / LPG;  // lanes per leaf
 	const int t = threadIdx.x;
-	const int depth = level[lf.x] + 1;
-	// dynamic LDS: [35][maxdepth] shifted contributions, then [35] the leaf's expansion
+	const int q = t / W, tl = t % W;
+	const int li = int(blockIdx.x) * LPG + q;
+	const bool active = li < nleaves;
+	const int4 lf = active ? leaves[li] : make_int4(0, 0, 0, 0);
+	const int depth = active ? level[lf.x] + 1 : 0;
+	// dynamic LDS per leaf: [35][maxdepth] shifted contributions, then [35] the leaf's expansion
 	extern __shared__ double lds[];
-	double* Ls = lds + 35 * maxdepth;
+	double* mine = lds + q * (35 * maxdepth + 35);
+	double* Ls = mine + 35 * maxdepth;
 	const int4 c = pos[lf.x];
-	if (t < depth) {
-		const int a = path[lf.w + t];
+	for (int k = tl; k < depth; k += W) {
+		const int a = path[lf.w + k];
 		const int4 A = pos[a];
 		const double d[3] = {double(int64_t(c.x) - int64_t(A.x)) * POS_INV, double(int64_t(c.y) - int64_t(A.y)) * POS_INV,
 				double(int64_t(c.z) - int64_t(A.z)) * POS_INV};
 		const double* src = dL + 35 * int64_t(a);
 		double La[35], Lt[35];
 #pragma unroll
-		for (int q = 0; q < 35; q++) La[q] = src[q];
+		for (int r = 0; r < 35; r++) La[r] = src[r];
 		l2l(Lt, La, d);
 #pragma unroll
-		for (int q = 0; q < 35; q++) lds[q * maxdepth + t] = Lt[q];
+		for (int r = 0; r < 35; r++) mine[r * maxdepth + k] = Lt[r];
 	}
 	__syncthreads();
-	if (t < 35) {
+	for (int cc = tl; cc < 35 && active; cc += W) {
 		// four partial sums over the path (root first within each), short dependency chains
-		const double* row = lds + t * maxdepth;
+		const double* row = mine + cc * maxdepth;
 		double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 		int k = 0;
 		for (; k + 4 <= depth; k += 4) {
@@ -100,16 +106,17 @@ __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, c
 			a3 += row[k + 3];
 		}
 		for (; k < depth; k++) a0 += row[k];
-		Ls[t] = (a0 + a1) + (a2 + a3);
+		Ls[cc] = (a0 + a1) + (a2 + a3);
 	}
 	__syncthreads();
+	if (!active) return;
 	// evaluate at the leaf's particles
-	for (int i = lf.y + t; i < lf.z; i += 64) {
-		const int4 q = parts[i];
+	for (int i = lf.y + tl; i < lf.z; i += W) {
+		const int4 p = parts[i];
 		// the reference rounds the offset from the leaf centre to float (src/tree.cpp:607)
-		const double d[3] = {double(float(double(centre_offset(q.x, c.x, wrap)) * POS_INV)),
-				double(float(double(centre_offset(q.y, c.y, wrap)) * POS_INV)),
-				double(float(double(centre_offset(q.z, c.z, wrap)) * POS_INV))};
+		const double d[3] = {double(float(double(centre_offset(p.x, c.x, wrap)) * POS_INV)),
+				double(float(double(centre_offset(p.y, c.y, wrap)) * POS_INV)),
+				double(float(double(centre_offset(p.z, c.z, wrap)) * POS_INV))};
 		double phi, g[3];
 		l2p(phi, g, Ls, d);
 		double* o = out + 4 * size_t(i);
@@ -119,8 +126,9 @@ __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, c
 		o[3] = g[2];
 	}
 }
+constexpr int LOCAL_LPG = 2;
 
-static size_t local_lds_bytes(int maxdepth) { return size_t(35 * maxdepth + 35) * sizeof(double); }
+static size_t local_lds_bytes(int maxdepth) { return size_t(LOCAL_LPG) * size_t(35 * maxdepth + 35) * sizeof(double); }
 
 // ---- host side ---------------------------------------------------------------------------------
 #define SCHK(call)                                           \
@@ -505,8 +513,8 @@ int Solver::enqueue_solve(bool timed) {
 		hipLaunchKernelGGL(k_node_sum, dim3(unsigned((nn * 35 + 255) / 256)), dim3(256), 0, s,
 				d_node_slot_ptr.as<int64_t>(), d_partial.as<double>(), d_dL.as<double>(), nn);
 		SCHK(hipGetLastError());
-		hipLaunchKernelGGL(k_local, dim3(n_leaf_items), dim3(64), local_lds_bytes(maxdepth), s, d_leaf_items.as<int4>(),
-				d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(), d_dL.as<double>(),
+		hipLaunchKernelGGL(k_local<LOCAL_LPG>, dim3(unsigned((n_leaf_items + LOCAL_LPG - 1) / LOCAL_LPG)), dim3(64), local_lds_bytes(maxdepth), s,
+				d_leaf_items.as<int4>(), n_leaf_items, d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(), d_dL.as<double>(),
 				d_out.as<double>(), phi_self, maxdepth, ewald);
 		SCHK(hipGetLastError());
 	}
