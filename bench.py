@@ -241,8 +241,8 @@ def main():
         args.leaf = best["leaf"]
     free0 = device_free_bytes()
     s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
-    if not args.host_walk:
-        s.tune("gpu_walk", 1)
+    if args.host_walk:
+        s.tune("gpu_walk", 0)
     if best is not None:
         from tigergrav_amd.autotune import apply
 

@@ -36,8 +36,9 @@ typedef struct tg_solve_info {
  * p->device = -1 gives a host-only solver: tree, multipoles, lists and export, no solve. */
 int tg_solver_create(const tg_params* p, double theta, int parts_per_node, tg_solver** out);
 int tg_solver_destroy(tg_solver* s);
-/* particles (int32 x3 fixed point, n of them; mass NULL = all p->m): builds tree, multipoles and
- * lists on the host (tree::refine, compute_multipoles, kick_fmm's list construction) and makes
+/* particles (int32 x3 fixed point, n of them; mass NULL = all p->m): builds the tree and
+ * multipoles on the host (tree::refine, compute_multipoles) and the lists (kick_fmm's list
+ * construction) on the device or, with "gpu_walk" 0 or a host-only solver, on the host; makes
  * them resident on the device */
 int tg_solver_set_particles(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
 /* new positions (and masses, or NULL to keep them) for the same particles, in input order, with
@@ -57,9 +58,9 @@ int tg_solver_adjoint(tg_solver* s, const float* w, double* grad, double* ms);
 int tg_solver_solve(tg_solver* s);
 /* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent
  * queue-driven kernel; "node_chunk" 1..64 sources per CC/CP partial slot (applies from the next
- * tg_solver_set_particles); "resident" work groups of the persistent form; "gpu_walk" 0/1 builds
- * the interaction lists on the device (the same lists as the host walk of kick_fmm,
- * src/tree.cpp:378-635; from the next tg_solver_set_particles) */
+ * tg_solver_set_particles); "resident" work groups of the persistent form; "gpu_walk" 1 (the default
+ * for device solvers) builds the interaction lists on the device, 0 on the host (the same lists
+ * either way: the walk of kick_fmm, src/tree.cpp:378-635; from the next tg_solver_set_particles) */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);
 /* the same solve replayed from a HIP graph captured on its first call after set_particles (one
  * submission instead of nine launches; records no per-kind timings) */

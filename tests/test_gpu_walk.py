@@ -24,8 +24,8 @@ CASES = {
 def _build(x, ewald, theta, leaf, device, gpu_walk):
     n = x.shape[0]
     s = Solver(ewald=ewald, soft_len=ics.soft_len(n), m=1.0 / n, theta=theta, parts_per_node=leaf, device=device)
-    if gpu_walk:
-        s.tune("gpu_walk", 1)
+    if device >= 0:
+        s.tune("gpu_walk", 1 if gpu_walk else 0)
     s.set_particles(x)
     return s
 

@@ -157,6 +157,7 @@ int Solver::init(int dev) {
 		return TG_EINVAL;
 	}
 	SCHK(hipSetDevice(dev));
+	gpu_walk = true;  // the lists on the device by default (bit-identical to the host walk)
 	SCHK(init_device_tables(dev));
 	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 	SCHK(d_queue.alloc(256));

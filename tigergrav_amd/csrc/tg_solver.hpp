@@ -41,7 +41,7 @@ struct Solver {
 	bool uniform_mass = true;
 	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64); applies from the next set_particles
 	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
-	bool gpu_walk = false;       // build the interaction lists on the device (device_walk)
+	bool gpu_walk = false;       // build the interaction lists on the device (device_walk); on for device solvers
 	int resident_groups = 1024;  // work groups of the persistent form (set from the CU count)
 	Tree tree;
 	int64_t n = 0;
