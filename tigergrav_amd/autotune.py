@@ -6,7 +6,7 @@ import itertools
 from .solver import Solver
 
 
-def autotune(x, ewald, soft_len, m, theta, leaves=(32, 64, 128), chunks=(32, 64), persistent=(0, 1), solves=3,
+def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(32, 64), persistent=(0, 1), solves=3,
              device=0):
     """Returns (best, table): best = {"leaf", "node_chunk", "pp_persistent", "ms"}; table lists
     every configuration tried with its mean device ms per solve (sum over the solve's kernels)."""
