@@ -4,7 +4,7 @@
 
 Workload (default): 128^3 particles displaced by a Zel'dovich field of a seeded Gaussian random
 field with P(k) ~ k^-2 (cut at the grid Nyquist), rms displacement 0.3 cell, periodic (Ewald),
-theta = 0.7, softening 0.02 N^(-1/3), equal masses 1/N.  By default the leaf size (8 to 128), the
+theta = 0.7, softening 0.02 N^(-1/3), equal masses 1/N.  By default the leaf size (32/64/128), the
 CC/CP chunk and the PP kernel form are autotuned on this workload's own device time (--leaf N
 fixes the leaf size instead), and the timed steps replay the solve from a captured HIP graph.
 
@@ -191,7 +191,8 @@ def main():
     ap.add_argument("--workload", default="zeldovich128")
     ap.add_argument("--theta", type=float, default=0.7)
     ap.add_argument("--leaf", type=int, default=0,
-                    help="particles per leaf; 0 autotunes leaf (8 to 128), CC/CP chunk and the PP kernel form")
+                    help="particles per leaf; 0 autotunes leaf (32, 64, 128), CC/CP chunk and the PP kernel form")
+    ap.add_argument("--wide-autotune", action="store_true", help="autotune over leaves of 8 and 16 as well")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pp-persistent", type=int, default=None, choices=(0, 1),
                     help="with a fixed --leaf: the persistent (1) or one-group-per-item (0) PP kernel")
@@ -237,7 +238,9 @@ def main():
         from tigergrav_amd.autotune import autotune
 
         # the adjoint handles leaves of at most 64 particles
-        best, tuning = autotune(x, ewald, h, m, args.theta, leaves=(8, 16, 32, 64) if args.adjoint else (8, 16, 32, 64, 128))
+        # the workload's own candidate set is leaf 32/64/128; --wide-autotune adds 8 and 16
+        leaves = (8, 16, 32, 64, 128) if args.wide_autotune else (32, 64, 128)
+        best, tuning = autotune(x, ewald, h, m, args.theta, leaves=tuple(l for l in leaves if l <= 64 or not args.adjoint))
         args.leaf = best["leaf"]
     free0 = device_free_bytes()
     s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
