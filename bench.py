@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--pp-persistent", type=int, default=None, choices=(0, 1),
                     help="with a fixed --leaf: the persistent (1) or one-group-per-item (0) PP kernel")
     ap.add_argument("--node-chunk", type=int, default=None, help="with a fixed --leaf: CC/CP list entries per partial slot")
+    ap.add_argument("--local-sweep", type=int, default=None, choices=(0, 1),
+                    help="with a fixed --leaf: the local-expansion pass level by level (1) or by per-leaf path sums (0)")
     ap.add_argument("--host-walk", action="store_true",
                     help="build the interaction lists with the host walk instead of the device walk")
     ap.add_argument("--no-graph", action="store_true",
@@ -255,6 +257,8 @@ def main():
             s.tune("pp_persistent", args.pp_persistent)
         if args.node_chunk is not None:
             s.tune("node_chunk", args.node_chunk)
+        if args.local_sweep is not None:
+            s.tune("local_sweep", args.local_sweep)
     t0 = time.time()
     s.set_particles(x)
     t_host = time.time() - t0
@@ -323,6 +327,7 @@ def main():
         "config": {"workload": desc, "N": N, "theta": args.theta, "leaf": args.leaf, "ewald": ewald, "softening": h,
                    "pp_persistent": best["pp_persistent"] if best else args.pp_persistent,
                    "node_chunk": best["node_chunk"] if best else args.node_chunk,
+                   "local_sweep": best.get("local_sweep", 0) if best else args.local_sweep,
                    "accumulation": "f64", "parallelism": "single GPU"},
         "s_per_force_solve": ms / 1e3,
         "step": "eager solves" if args.no_graph else "HIP-graph replays",

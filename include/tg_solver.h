@@ -58,7 +58,9 @@ int tg_solver_adjoint(tg_solver* s, const float* w, double* grad, double* ms);
 int tg_solver_solve(tg_solver* s);
 /* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent
  * queue-driven kernel; "node_chunk" 1..64 sources per CC/CP partial slot (applies from the next
- * tg_solver_set_particles); "resident" work groups of the persistent form; "gpu_walk" 1 (the default
+ * tg_solver_set_particles); "resident" work groups of the persistent form; "local_sweep" 1 runs the
+ * local-expansion pass level by level (the reference's order, src/tree.cpp:400-401) instead of
+ * per-leaf sums over the root path (0, the default); "gpu_walk" 1 (the default
  * for device solvers) builds the interaction lists on the device, 0 on the host (the same lists
  * either way: the walk of kick_fmm, src/tree.cpp:378-635; from the next tg_solver_set_particles) */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);

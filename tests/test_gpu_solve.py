@@ -151,7 +151,8 @@ def test_graph_replay_matches_eager():
 def test_tuning_knobs():
     """The persistent queue-driven PP kernel evaluates each item exactly as the one-group-per-item
     kernel does (bit-identical forces, any residency); a different CC/CP chunk only regroups FP64
-    partial sums (within the parity tolerance of the default)."""
+    partial sums (within the parity tolerance of the default); the level-by-level local pass
+    differs from the path sums by FP64 rounding only."""
     from tigergrav_amd import ics
     from tigergrav_amd.solver import Solver
 
@@ -180,6 +181,18 @@ def test_tuning_knobs():
     s2.synchronize()
     f2 = s2.forces()
     assert rms_rel(f2[:, 1:], base[:, 1:]) < 1e-6
+    # the level-by-level local pass shifts the running sum node by node (the reference's order):
+    # the same expansion up to FP64 rounding, and graph replays of it bit-identical to eager solves
+    s.tune("pp_persistent", 0)
+    s.tune("local_sweep", 1)
+    s.solve()
+    s.synchronize()
+    f3 = s.forces()
+    assert rms_rel(f3[:, 1:], base[:, 1:]) < 1e-12 and np.abs(f3[:, 0] - base[:, 0]).max() < 1e-12 * np.abs(base[:, 0]).max()
+    for _ in range(3):
+        s.solve_graph()
+        s.synchronize()
+        np.testing.assert_array_equal(s.forces(), f3)
     with pytest.raises(Exception):
         s2.tune("no_such_knob", 1)
 

@@ -128,6 +128,59 @@ __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, i
 }
 constexpr int LOCAL_LPG = 2;
 
+// The local-expansion pass as the reference runs it (src/tree.cpp:400-401): a node's expansion is
+// its own contribution plus its parent's shifted to it, L_n = dL_n + shift(L_parent, X_n - X_parent),
+// level by level from the root, one thread per node of the level (nodes are numbered level by
+// level).  Every node is shifted once, where summing over each leaf's path shifts every ancestor
+// once per leaf below it.
+__global__ __launch_bounds__(64) void k_l2l_level(int begin, int end, const int* __restrict__ parent, const int4* __restrict__ pos,
+		const double* __restrict__ dL, double* __restrict__ L) {
+	const int n = begin + int(blockIdx.x) * 64 + int(threadIdx.x);
+	if (n >= end) return;
+	const int p = parent[n];
+	const double* dn = dL + 35 * int64_t(n);
+	double* Ln = L + 35 * int64_t(n);
+	if (p < 0) {
+#pragma unroll
+		for (int r = 0; r < 35; r++) Ln[r] = dn[r];
+		return;
+	}
+	const int4 C = pos[n], A = pos[p];
+	const double d[3] = {double(int64_t(C.x) - int64_t(A.x)) * POS_INV, double(int64_t(C.y) - int64_t(A.y)) * POS_INV,
+			double(int64_t(C.z) - int64_t(A.z)) * POS_INV};
+	double Lp[35], Lt[35];
+	const double* lp = L + 35 * int64_t(p);
+#pragma unroll
+	for (int r = 0; r < 35; r++) Lp[r] = lp[r];
+	l2l(Lt, Lp, d);
+#pragma unroll
+	for (int r = 0; r < 35; r++) Ln[r] = Lt[r] + dn[r];
+}
+
+// L2P at every particle from its leaf's expansion (one thread per particle), with the reference's
+// float rounding of the offset from the leaf centre (src/tree.cpp:607) and phi_self (quirk Q2)
+__global__ __launch_bounds__(64) void k_l2p_parts(int64_t n, const int* __restrict__ pleaf, const int4* __restrict__ pos,
+		const int4* __restrict__ parts, const double* __restrict__ L, double* __restrict__ out, double phi_self, int wrap) {
+	const int64_t i = int64_t(blockIdx.x) * 64 + threadIdx.x;
+	if (i >= n) return;
+	const int lf = pleaf[i];
+	const int4 c = pos[lf], p = parts[i];
+	const double d[3] = {double(float(double(centre_offset(p.x, c.x, wrap)) * POS_INV)),
+			double(float(double(centre_offset(p.y, c.y, wrap)) * POS_INV)),
+			double(float(double(centre_offset(p.z, c.z, wrap)) * POS_INV))};
+	double Ls[35];
+	const double* src = L + 35 * int64_t(lf);
+#pragma unroll
+	for (int r = 0; r < 35; r++) Ls[r] = src[r];
+	double phi, g[3];
+	l2p(phi, g, Ls, d);
+	double* o = out + 4 * i;
+	o[0] = phi - phi_self;
+	o[1] = g[0];
+	o[2] = g[1];
+	o[3] = g[2];
+}
+
 static size_t local_lds_bytes(int maxdepth) { return size_t(LOCAL_LPG) * size_t(35 * maxdepth + 35) * sizeof(double); }
 
 // ---- host side ---------------------------------------------------------------------------------
@@ -236,6 +289,9 @@ Solver::~Solver() {
 		d_slot_node.release();
 		d_path.release();
 		d_level.release();
+		d_parent.release();
+		d_pleaf.release();
+		d_Lnode.release();
 		for (auto& w : work) {
 			w.items.release();
 			w.list.release();
@@ -416,6 +472,25 @@ int Solver::upload() {
 	SCHK(put(d_slot_node, slot_node, s));
 	SCHK(put(d_path, path, s));
 	SCHK(put(d_level, level, s));
+	{
+		// the level-by-level local pass: node parents, level ranges, each particle's leaf
+		std::vector<int> parent(static_cast<size_t>(nn)), pleaf(static_cast<size_t>(n));
+		level_begin.clear();
+		for (int64_t i = 0; i < nn; i++) {
+			parent[size_t(i)] = T.nodes[size_t(i)].parent;
+			if (i == 0 || T.nodes[size_t(i)].level != T.nodes[size_t(i) - 1].level) level_begin.push_back(int(i));
+			if (i > 0 && T.nodes[size_t(i)].level < T.nodes[size_t(i) - 1].level) {
+				err = "nodes not numbered level by level";
+				return TG_EINVAL;
+			}
+		}
+		level_begin.push_back(int(nn));
+		for (const int4& li : leaf_items)
+			for (int j = li.y; j < li.z; j++) pleaf[size_t(j)] = li.x;
+		SCHK(put(d_parent, parent, s));
+		SCHK(put(d_pleaf, pleaf, s));
+		SCHK(d_Lnode.alloc(std::max<size_t>(size_t(nn) * 35 * sizeof(double), 16)));
+	}
 	list_bytes = 0;
 	// run tables stay allocated until the stream has drained below
 	std::vector<DeviceArray> run_tables;
@@ -514,10 +589,22 @@ int Solver::enqueue_solve(bool timed) {
 		hipLaunchKernelGGL(k_node_sum, dim3(unsigned((nn * 35 + 255) / 256)), dim3(256), 0, s,
 				d_node_slot_ptr.as<int64_t>(), d_partial.as<double>(), d_dL.as<double>(), nn);
 		SCHK(hipGetLastError());
-		hipLaunchKernelGGL(k_local<LOCAL_LPG>, dim3(unsigned((n_leaf_items + LOCAL_LPG - 1) / LOCAL_LPG)), dim3(64), local_lds_bytes(maxdepth), s,
-				d_leaf_items.as<int4>(), n_leaf_items, d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(), d_dL.as<double>(),
-				d_out.as<double>(), phi_self, maxdepth, ewald);
-		SCHK(hipGetLastError());
+		if (local_sweep) {
+			for (size_t l = 0; l + 1 < level_begin.size(); l++) {
+				const int b = level_begin[l], e2 = level_begin[l + 1];
+				hipLaunchKernelGGL(k_l2l_level, dim3(unsigned((e2 - b + 63) / 64)), dim3(64), 0, s, b, e2, d_parent.as<int>(), d_nodepos.as<int4>(),
+						d_dL.as<double>(), d_Lnode.as<double>());
+				SCHK(hipGetLastError());
+			}
+			hipLaunchKernelGGL(k_l2p_parts, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, n, d_pleaf.as<int>(), d_nodepos.as<int4>(),
+					d_parts.as<int4>(), d_Lnode.as<double>(), d_out.as<double>(), phi_self, ewald);
+			SCHK(hipGetLastError());
+		} else {
+			hipLaunchKernelGGL(k_local<LOCAL_LPG>, dim3(unsigned((n_leaf_items + LOCAL_LPG - 1) / LOCAL_LPG)), dim3(64), local_lds_bytes(maxdepth),
+					s, d_leaf_items.as<int4>(), n_leaf_items, d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(),
+					d_dL.as<double>(), d_out.as<double>(), phi_self, maxdepth, ewald);
+			SCHK(hipGetLastError());
+		}
 	}
 	SCHK(mark(e + 1));
 	LeafArgs la{};
@@ -660,6 +747,8 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 	} else if (k == "resident") {
 		if (value < 1) return TG_EINVAL;
 		S.resident_groups = value;
+	} else if (k == "local_sweep") {
+		S.local_sweep = value != 0;
 	} else if (k == "gpu_walk") {
 		if (S.device < 0 && value != 0) return TG_EINVAL;
 		S.gpu_walk = value != 0;

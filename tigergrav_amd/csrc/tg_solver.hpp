@@ -41,6 +41,7 @@ struct Solver {
 	bool uniform_mass = true;
 	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64); applies from the next set_particles
 	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
+	bool local_sweep = false;    // local pass level by level (k_l2l_level) instead of per-leaf path sums (k_local)
 	bool gpu_walk = false;       // build the interaction lists on the device (device_walk); on for device solvers
 	int resident_groups = 1024;  // work groups of the persistent form (set from the CU count)
 	Tree tree;
@@ -48,6 +49,8 @@ struct Solver {
 	// device state
 	DeviceArray d_parts, d_multis, d_nodepos, d_out, d_partial, d_node_slot_ptr;
 	DeviceArray d_leaf_items, d_slot_node, d_path, d_level, d_dL, d_queue;
+	DeviceArray d_parent, d_pleaf, d_Lnode;  // level-by-level local pass: parents, particle -> leaf, node expansions
+	std::vector<int> level_begin;          // first node of each level, then the node count
 	// adjoint buffers: loss weights, gradients, expansion and multipole adjoints
 	DeviceArray d_adj_w, d_adj_grad, d_adj_lam, d_adj_mu;
 	bool adj_pp_symmetric = false;  // checked once per tree at the first adjoint call
