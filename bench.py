@@ -334,7 +334,8 @@ def main():
         "ms_per_step_eager": ms_eager,
         "s_per_force_solve_end_to_end": t_host + ms / 1e3,
         "host_build_s": {"tree": info["t_tree"], "multipoles": info["t_multipoles"], "lists": info["t_walk"],
-                         "upload": info["t_upload"], "lists_on": "host" if args.host_walk else "device"},
+                         "upload": info["t_upload"], "lists_on": "host" if args.host_walk else "device",
+                         "detail": info["t_detail"]},
         "interactions": count, "kernel_ms": kms,
         "tuning": {"best": best, "table": tuning} if tuning else None,
         "roofline": roofline,

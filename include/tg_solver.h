@@ -29,6 +29,9 @@ typedef struct tg_solve_info {
 	int64_t list_len[8];   /* source indices per kind on the device */
 	int64_t list_bytes;    /* device bytes of lists and items */
 	double t_tree, t_multipoles, t_walk, t_upload;  /* seconds, last tg_solver_set_particles */
+	/* finer split of t_walk and t_upload (seconds): device walk set-up, its steps, list assembly
+	   and copy to the host, census; work-array preparation on the host, transfers */
+	double t_detail[6];
 } tg_solve_info;
 
 /* replaces the tree root construction + options set-up (src/main.cpp:133-174): kernel

@@ -892,6 +892,10 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 	}
 	ACHK(hipSetDevice(device));
 	if (!adj_lists_ready) {
+		{
+			const int rc = host_lists();  // the by-source transposes read the CC list values
+			if (rc != TG_OK) return rc;
+		}
 		// CC entries by source: (source, sink node) pairs of the direct and Ewald lists, sorted by source
 		auto put_v = [&](DeviceArray& d, const void* h, size_t bytes) -> hipError_t {
 			hipError_t r = d.alloc(std::max<size_t>(bytes, 16));
@@ -899,8 +903,8 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 			return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
 		};
 		for (int e = 0; e < 2; e++) {
-			const std::vector<int64_t>& ptr = e ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
-			const std::vector<int32_t>& v = e ? tree.lists.cc_e : tree.lists.cc_d;
+			const hvec<int64_t>& ptr = e ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
+			const hvec<int32_t>& v = e ? tree.lists.cc_e : tree.lists.cc_d;
 			std::vector<std::pair<int32_t, int32_t>> pr;
 			for (size_t a = 0; a + 1 < ptr.size(); a++)
 				for (int64_t q = ptr[a]; q < ptr[a + 1]; q++) pr.emplace_back(v[q], int32_t(a));
@@ -922,8 +926,8 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 		for (int32_t lf : tree.leaves)
 			if (!tree.nodes[lf].empty()) leaf_at[size_t(tree.nodes[lf].pbegin)] = lf;
 		for (int e = 0; e < 2; e++) {
-			const std::vector<int64_t>& ptr = e ? tree.lists.cp_e_ptr : tree.lists.cp_d_ptr;
-			const std::vector<int32_t>& v = e ? tree.lists.cp_e : tree.lists.cp_d;
+			const hvec<int64_t>& ptr = e ? tree.lists.cp_e_ptr : tree.lists.cp_d_ptr;
+			const hvec<int32_t>& v = e ? tree.lists.cp_e : tree.lists.cp_d;
 			std::vector<std::pair<int32_t, int32_t>> pr;  // (source leaf first particle, sink node)
 			for (size_t a = 0; a + 1 < ptr.size(); a++)
 				for (int64_t q = ptr[a]; q < ptr[a + 1]; q += 2) {
@@ -961,8 +965,8 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 			return TG_EINVAL;
 		}
 		for (int e = 0; e < 2; e++) {
-			const std::vector<int64_t>& ptr = e ? tree.lists.pc_e_ptr : tree.lists.pc_d_ptr;
-			const std::vector<int32_t>& v = e ? tree.lists.pc_e : tree.lists.pc_d;
+			const hvec<int64_t>& ptr = e ? tree.lists.pc_e_ptr : tree.lists.pc_d_ptr;
+			const hvec<int32_t>& v = e ? tree.lists.pc_e : tree.lists.pc_d;
 			std::vector<std::pair<int32_t, int32_t>> pr;  // (source node, sink leaf)
 			for (int32_t lf : tree.leaves) {
 				if (tree.nodes[lf].empty() || size_t(lf) + 1 >= ptr.size()) continue;

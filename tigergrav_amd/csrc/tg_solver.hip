@@ -316,10 +316,33 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	tree.compute_multipoles();
 	double t2 = now();
 	if (gpu_walk) {
-		const hipError_t e = device_walk(tree, device, stream, err);
+		// the CC lists stay on the device as the work lists (the host reads only their pointers);
+		// host_lists() copies them back when something asks for the values
+		int32_t* kept[8] = {};
+		const hipError_t e = device_walk(tree, device, stream, err, t_detail, (1u << 0) | (1u << 2), kept);
+		const int kinds[2] = {CC_DIRECT, CC_EWALD};
+		cc_on_device = false;
+		for (int q = 0; q < 2; q++) {
+			DeviceArray& d = work[kinds[q]].list;
+			d.release();
+			const hvec<int64_t>& ptr = q ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
+			if (e == hipSuccess && kept[2 * q] && !ptr.empty()) {
+				d.p = kept[2 * q];
+				// at least 1024 entries are allocated, so claiming 16 bytes for a shorter list is safe
+				d.bytes = std::max<size_t>(size_t(ptr.back()) * 4, 16);
+			} else if (kept[2 * q]) {
+				(void) hipFree(kept[2 * q]);
+			}
+		}
 		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
+		if (!work[CC_DIRECT].list.p || !work[CC_EWALD].list.p) {
+			err = "device walk returned no CC lists";
+			return TG_EHIP;
+		}
+		cc_on_device = true;
 	} else {
 		tree.walk();
+		cc_on_device = false;
 	}
 	double t3 = now();
 	t_tree = t1 - t0;
@@ -330,8 +353,8 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	return rc;
 }
 
-template <class T>
-static hipError_t put(DeviceArray& d, const std::vector<T>& h, hipStream_t s);
+template <class T, class A>
+static hipError_t put(DeviceArray& d, const std::vector<T, A>& h, hipStream_t s);
 
 int Solver::update_particles(const int32_t* x, const float* mass) {
 	if (tree.nodes.empty() || n == 0) {
@@ -363,8 +386,8 @@ bool Solver::pp_lists_symmetric() const {
 	for (int32_t lf : T.leaves)
 		if (!T.nodes[lf].empty()) leaf_at[T.nodes[lf].pbegin] = lf;
 	for (int kind = 0; kind < 2; kind++) {
-		const std::vector<int64_t>& ptr = kind ? T.lists.pp_e_ptr : T.lists.pp_d_ptr;
-		const std::vector<int32_t>& v = kind ? T.lists.pp_e : T.lists.pp_d;
+		const hvec<int64_t>& ptr = kind ? T.lists.pp_e_ptr : T.lists.pp_d_ptr;
+		const hvec<int32_t>& v = kind ? T.lists.pp_e : T.lists.pp_d;
 		std::vector<std::pair<int32_t, int32_t>> pairs;
 		for (int32_t lf : T.leaves) {
 			if (size_t(lf) + 1 >= ptr.size()) continue;
@@ -414,12 +437,27 @@ __global__ __launch_bounds__(64) void k_expand_runs(const ListRun* __restrict__ 
 	}
 }
 
-template <class T>
-static hipError_t put(DeviceArray& d, const std::vector<T>& h, hipStream_t s) {
+template <class T, class A>
+static hipError_t put(DeviceArray& d, const std::vector<T, A>& h, hipStream_t s) {
 	hipError_t e = d.alloc(std::max<size_t>(h.size() * sizeof(T), 16));
 	if (e != hipSuccess) return e;
 	if (h.empty()) return hipSuccess;
 	return hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+int Solver::host_lists() {
+	if (!cc_on_device) return TG_OK;
+	SCHK(hipSetDevice(device));
+	const int kinds[2] = {CC_DIRECT, CC_EWALD};
+	for (int q = 0; q < 2; q++) {
+		hvec<int32_t>& v = q ? tree.lists.cc_e : tree.lists.cc_d;
+		const hvec<int64_t>& ptr = q ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
+		v.resize(ptr.empty() ? 0 : size_t(ptr.back()));
+		if (!v.empty()) SCHK(hipMemcpyAsync(v.data(), work[kinds[q]].list.p, v.size() * 4, hipMemcpyDeviceToHost, stream));
+	}
+	SCHK(hipStreamSynchronize(stream));
+	cc_on_device = false;
+	return TG_OK;
 }
 
 int Solver::upload() {
@@ -427,33 +465,37 @@ int Solver::upload() {
 	const Tree& T = tree;
 	const int64_t nn = int64_t(T.nodes.size());
 	HostWork W;
+	const double tp0 = now();
 	{
 		const int rc = prepare_work(T, n, ewald, soft_len, node_chunk, W, err);
 		if (rc != TG_OK) return rc;
 	}
+	t_detail[4] = now() - tp0;
 	if (T.max_level + 1 > TG_MAXDEPTH) {
 		err = "tree deeper than the local-expansion pass supports (64 levels)";
 		return TG_EINVAL;
 	}
 	n_slots = W.n_slots;
 	n_leaf_items = int(W.leaf_items.size());
-	std::vector<int4>* items = W.items;
-	std::vector<int>* lists = W.lists;
-	std::vector<ListRun>* runs = W.runs;
-	std::vector<ListRun>* owner_runs = W.owner_runs;
-	std::vector<int2>* waves = W.waves;
+	auto* items = W.items;
+	auto* lists = W.lists;
+	auto* runs = W.runs;
+	auto* owner_runs = W.owner_runs;
+	auto* waves = W.waves;
 	const int64_t* list_len = W.list_len;
-	const std::vector<int32_t>* walk_list[NKINDS] = {};  // CC kinds: the walk's flat list, node order
+	const hvec<int32_t>* walk_list[NKINDS] = {};  // CC kinds: the walk's flat list, node order
+	bool on_device[NKINDS] = {};                   // the device walk left it in work[k].list
 	walk_list[CC_DIRECT] = &T.lists.cc_d;
 	walk_list[CC_EWALD] = &T.lists.cc_e;
-	std::vector<int4>& parts = W.parts;
-	std::vector<MultiSrc>& multis = W.multis;
-	std::vector<int4>& npos = W.npos;
-	std::vector<int>& slot_node = W.slot_node;
-	std::vector<int64_t>& slot_ptr = W.slot_ptr;
-	std::vector<int4>& leaf_items = W.leaf_items;
-	std::vector<int>& path = W.path;
-	std::vector<int>& level = W.level;
+	on_device[CC_DIRECT] = on_device[CC_EWALD] = cc_on_device;
+	auto& parts = W.parts;
+	auto& multis = W.multis;
+	auto& npos = W.npos;
+	auto& slot_node = W.slot_node;
+	auto& slot_ptr = W.slot_ptr;
+	auto& leaf_items = W.leaf_items;
+	auto& path = W.path;
+	auto& level = W.level;
 	if (device < 0) {  // host-only: keep the PP-direct work for inspection, its list expanded here
 		host_pp_items = std::move(items[PP_DIRECT]);
 		host_pp_list.assign(size_t(list_len[PP_DIRECT]), 0);
@@ -474,7 +516,7 @@ int Solver::upload() {
 	SCHK(put(d_level, level, s));
 	{
 		// the level-by-level local pass: node parents, level ranges, each particle's leaf
-		std::vector<int> parent(static_cast<size_t>(nn)), pleaf(static_cast<size_t>(n));
+		hvec<int> parent(static_cast<size_t>(nn)), pleaf(static_cast<size_t>(n));
 		level_begin.clear();
 		for (int64_t i = 0; i < nn; i++) {
 			parent[size_t(i)] = T.nodes[size_t(i)].parent;
@@ -486,7 +528,7 @@ int Solver::upload() {
 		}
 		level_begin.push_back(int(nn));
 		for (const int4& li : leaf_items)
-			for (int j = li.y; j < li.z; j++) pleaf[size_t(j)] = li.x;
+			std::fill(pleaf.begin() + li.y, pleaf.begin() + li.z, li.x);
 		SCHK(put(d_parent, parent, s));
 		SCHK(put(d_pleaf, pleaf, s));
 		SCHK(d_Lnode.alloc(std::max<size_t>(size_t(nn) * 35 * sizeof(double), 16)));
@@ -494,7 +536,7 @@ int Solver::upload() {
 	list_bytes = 0;
 	// run tables stay allocated until the stream has drained below
 	std::vector<DeviceArray> run_tables;
-	auto expand_runs = [&](const std::vector<ListRun>& r, int step, DeviceArray& out) -> hipError_t {
+	auto expand_runs = [&](const hvec<ListRun>& r, int step, DeviceArray& out) -> hipError_t {
 		if (r.empty()) return hipSuccess;
 		run_tables.emplace_back();
 		hipError_t e = put(run_tables.back(), r, s);
@@ -520,7 +562,7 @@ int Solver::upload() {
 			list_bytes += int64_t(items[k].size() * sizeof(int4));
 		}
 		if (e == hipSuccess) e = work[k].list.alloc(std::max<size_t>(lbytes, 16));
-		if (e == hipSuccess && lbytes > 0) {
+		if (e == hipSuccess && lbytes > 0 && !on_device[k]) {
 			if (walk_list[k])
 				e = hipMemcpyAsync(work[k].list.p, walk_list[k]->data(), lbytes, hipMemcpyHostToDevice, s);
 			else if (!lists[k].empty())
@@ -537,6 +579,7 @@ int Solver::upload() {
 	}
 	const hipError_t es = hipStreamSynchronize(s);
 	for (auto& t : run_tables) t.release();
+	t_detail[5] = now() - tp0 - t_detail[4];
 	if (rc != TG_OK) return rc;
 	SCHK(es);
 	SCHK(d_partial.alloc(std::max<size_t>(n_slots * 35 * sizeof(double), 16)));
@@ -832,6 +875,7 @@ int tg_solver_info(tg_solver* h, tg_solve_info* info) {
 	info->t_multipoles = h->s.t_multipoles;
 	info->t_walk = h->s.t_walk;
 	info->t_upload = h->s.t_upload;
+	for (int k = 0; k < 6; k++) info->t_detail[k] = h->s.t_detail[k];
 	return TG_OK;
 }
 
@@ -839,6 +883,10 @@ int tg_solver_info(tg_solver* h, tg_solve_info* info) {
 // array's size in bytes, or a negative status
 int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capacity) {
 	if (!h || !name) return TG_EINVAL;
+	if (h->s.device >= 0) {
+		const int rc = h->s.host_lists();
+		if (rc != TG_OK) return fail(h, rc);
+	}
 	const tg::Tree& T = h->s.tree;
 	std::vector<char> buf;
 	auto put = [&](const void* p, size_t bytes) {

@@ -73,13 +73,15 @@ struct Solver {
 	bool adj_lists_ready = false;
 	KindWork work[NKINDS];
 	// host-only solvers (device < 0) keep the PP-direct items and flattened source list
-	std::vector<int4> host_pp_items;
+	hvec<int4> host_pp_items;
 	std::vector<int> host_pp_list;
 	int n_leaf_items = 0;
 	int64_t n_slots = 0;
 	int64_t list_bytes = 0;
 	// timings (seconds) of the last build and solve
+	bool cc_on_device = false;  // the CC list values are only in work[CC_*].list (device walk)
 	double t_tree = 0, t_multipoles = 0, t_walk = 0, t_upload = 0;
+	double t_detail[6] = {};  // walk: set-up, steps, assembly, census; upload: preparation, transfers
 	// per-solve event sets: a ring of TIMER_RING solves, each 2 events per timed span (8 kinds,
 	// then the local-expansion pass); sets are folded into acc_ms before they are reused
 	static constexpr int TIMER_RING = 32, TIMER_SPANS = NKINDS + 1;
@@ -91,6 +93,7 @@ struct Solver {
 	int init(int dev);
 	int set_particles(const int32_t* x, const float* mass, int64_t count);
 	int upload();
+	int host_lists();  // copies list values the device walk kept on the device into tree.lists
 	int upload_sources();
 	int update_particles(const int32_t* x, const float* mass);
 	// gradients of sum_i w_i . g_i (w: 3 per particle, input order) with respect to positions and

@@ -455,7 +455,7 @@ void Tree::walk() {
 	// flatten, in node order
 	Lists& Ls = lists;
 	Ls = Lists();
-	auto flat = [&](std::vector<int64_t>& ptr, std::vector<int32_t>& dst, std::vector<int32_t> NodeLists::*mem) {
+	auto flat = [&](hvec<int64_t>& ptr, hvec<int32_t>& dst, std::vector<int32_t> NodeLists::*mem) {
 		ptr.assign(nodes.size() + 1, 0);
 		size_t tot = 0;
 		for (size_t i = 0; i < nodes.size(); i++) tot += (per[i].*mem).size();
@@ -479,7 +479,7 @@ void Tree::walk() {
 }
 
 // total length of the [begin, end) ranges v[b..e) (2 ints each)
-static uint64_t range_total(const std::vector<int32_t>& v, int64_t b, int64_t e) {
+static uint64_t range_total(const hvec<int32_t>& v, int64_t b, int64_t e) {
 	uint64_t t = 0;
 	for (int64_t k = b; k < e; k += 2) t += uint64_t(int64_t(v[size_t(k) + 1]) - int64_t(v[size_t(k)]));
 	return t;

@@ -8,9 +8,36 @@
 #pragma once
 #include <stdint.h>
 
+#include <memory>
+#include <new>
+#include <utility>
 #include <vector>
 
 namespace tg {
+
+// An allocator whose value-less construct default-initialises: resizing a vector of a trivial type
+// leaves the new elements unwritten, so a large host array is first touched (page-faulted) by the
+// parallel loop that fills it rather than zeroed on one thread.
+template <class T>
+struct NoInit : std::allocator<T> {
+	template <class U>
+	struct rebind {
+		using other = NoInit<U>;
+	};
+	NoInit() = default;
+	template <class U>
+	NoInit(const NoInit<U>&) {}
+	template <class U>
+	void construct(U* p) {
+		::new (static_cast<void*>(p)) U;
+	}
+	template <class U, class... A>
+	void construct(U* p, A&&... a) {
+		::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+	}
+};
+template <class T>
+using hvec = std::vector<T, NoInit<T>>;
 
 using boxid_t = unsigned __int128;
 
@@ -42,13 +69,13 @@ struct Node {
 // leaves have entries).  Particle sources are [begin, end) ranges into the tree-ordered particles.
 struct Lists {
 	// per node (first MAC pass, src/tree.cpp:424-494)
-	std::vector<int64_t> cc_d_ptr, cp_d_ptr, cc_e_ptr, cp_e_ptr;
-	std::vector<int32_t> cc_d, cc_e;      // source node ids
-	std::vector<int32_t> cp_d, cp_e;      // ranges, 2 ints each
+	hvec<int64_t> cc_d_ptr, cp_d_ptr, cc_e_ptr, cp_e_ptr;
+	hvec<int32_t> cc_d, cc_e;      // source node ids
+	hvec<int32_t> cp_d, cp_e;      // ranges, 2 ints each
 	// per leaf (drain loops, src/tree.cpp:530-602)
-	std::vector<int64_t> pc_d_ptr, pp_d_ptr, pc_e_ptr, pp_e_ptr;
-	std::vector<int32_t> pc_d, pc_e;      // source node ids
-	std::vector<int32_t> pp_d, pp_e;      // ranges, 2 ints each
+	hvec<int64_t> pc_d_ptr, pp_d_ptr, pc_e_ptr, pp_e_ptr;
+	hvec<int32_t> pc_d, pc_e;      // source node ids
+	hvec<int32_t> pp_d, pp_e;      // ranges, 2 ints each
 };
 
 // interaction census as the reference counts it (src/tree.cpp:483-494,618-628)

@@ -14,15 +14,15 @@ namespace tg {
 int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err) {
 	const int64_t nn = int64_t(T.nodes.size());
 	// particles (x, y, z, mass bits), tree order
-	std::vector<int4> parts(n);
+	hvec<int4> parts(static_cast<size_t>(n));
 #pragma omp parallel for schedule(static)
 	for (int64_t i = 0; i < n; i++) {
 		int w;
 		std::memcpy(&w, &T.mass[i], 4);
 		parts[i] = make_int4(T.pos[3 * i], T.pos[3 * i + 1], T.pos[3 * i + 2], w);
 	}
-	std::vector<MultiSrc> multis(nn);
-	std::vector<int4> npos(nn);
+	hvec<MultiSrc> multis(static_cast<size_t>(nn));
+	hvec<int4> npos(static_cast<size_t>(nn));
 #pragma omp parallel for schedule(static)
 	for (int64_t i = 0; i < nn; i++) {
 		const Node& nd = T.nodes[i];
@@ -38,24 +38,25 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 	// prefix, then the runs, chunks and items in parallel at those offsets -- the arrays a single
 	// in-order pass would produce.
 	const Lists& Ls = T.lists;
-	std::vector<int4> items[NKINDS];
-	std::vector<int> lists[NKINDS];  // host-built lists (PC kinds)
-	std::vector<ListRun> runs[NKINDS], owner_runs[NKINDS];
+	hvec<int4> items[NKINDS];
+	hvec<int> lists[NKINDS];  // host-built lists (PC kinds)
+	hvec<ListRun> runs[NKINDS], owner_runs[NKINDS];
 	int64_t list_len[NKINDS] = {};
-	std::vector<int2> chunks[NKINDS];
-	std::vector<int> slot_node;
-	std::vector<int64_t> slot_ptr(nn + 1, 0);
+	hvec<int2> chunks[NKINDS];
+	hvec<int> slot_node;
+	hvec<int64_t> slot_ptr(static_cast<size_t>(nn + 1));
+	slot_ptr[0] = 0;
 	const int chunk = std::min(64, std::max(1, node_chunk));
 	struct KindSrc {
 		int kind;
-		const std::vector<int64_t>* ptr;
-		const std::vector<int32_t>* v;
+		const hvec<int64_t>* ptr;
+		const hvec<int32_t>* v;
 		bool ranges;
 	};
 	const KindSrc nsrc[4] = {{CC_DIRECT, &Ls.cc_d_ptr, &Ls.cc_d, false}, {CP_DIRECT, &Ls.cp_d_ptr, &Ls.cp_d, true},
 			{CC_EWALD, &Ls.cc_e_ptr, &Ls.cc_e, false}, {CP_EWALD, &Ls.cp_e_ptr, &Ls.cp_e, true}};
 	// entries and runs of a node's (a leaf's) ranges v[b..e)
-	auto range_counts = [](const std::vector<int32_t>& v, int64_t b, int64_t e, int64_t& ent, int64_t& rn) {
+	auto range_counts = [](const hvec<int32_t>& v, int64_t b, int64_t e, int64_t& ent, int64_t& rn) {
 		ent = rn = 0;
 		for (int64_t k = b; k < e; k += 2) {
 			const int64_t len = int64_t(v[size_t(k) + 1]) - int64_t(v[size_t(k)]);
@@ -65,7 +66,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			}
 		}
 	};
-	std::vector<int64_t> nent(size_t(4 * nn)), nrun(size_t(4 * nn));
+	hvec<int64_t> nent(size_t(4 * nn)), nrun(size_t(4 * nn));
 #pragma omp parallel for schedule(static)
 	for (int64_t nd = 0; nd < nn; nd++)
 		for (int q = 0; q < 4; q++) {
@@ -78,7 +79,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			nent[size_t(4 * nd + q)] = ent;
 			nrun[size_t(4 * nd + q)] = rn;
 		}
-	std::vector<int64_t> oent(size_t(4 * nn)), orun(size_t(4 * nn)), ochk(size_t(4 * nn)), oslot(size_t(4 * nn));
+	hvec<int64_t> oent(size_t(4 * nn)), orun(size_t(4 * nn)), ochk(size_t(4 * nn)), oslot(size_t(4 * nn));
 	int64_t trun[4] = {}, tchk[4] = {}, slot = 0;
 	for (int64_t nd = 0; nd < nn; nd++) {
 		for (int q = 0; q < 4; q++) {
@@ -129,7 +130,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 				slot_node[size_t(sl)] = int(nd);
 			}
 		}
-	std::vector<int2> waves[NKINDS];
+	hvec<int2> waves[NKINDS];
 	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
 		int2 cur = make_int2(0, 0);
 		bool open = false;
@@ -151,7 +152,9 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 	// leaf kinds: items (first sink, count <= 64, lb, le).  A PP-direct list is reordered so the
 	// source ranges whose bounding box comes within the softening length of the sink leaf's box
 	// (the leaf itself among them) come first; the item records that near prefix's length.
-	std::vector<int32_t> leaf_at(size_t(n) + 1, -1);
+	hvec<int32_t> leaf_at(size_t(n) + 1);
+#pragma omp parallel for schedule(static)
+	for (int64_t i = 0; i <= n; i++) leaf_at[size_t(i)] = -1;
 	for (int32_t lf : T.leaves)
 		if (!T.nodes[lf].empty()) leaf_at[T.nodes[lf].pbegin] = lf;
 	const double hnear = soft_len * 1.001 + 1e-9;  // margin over the kernel's single-precision test
@@ -214,7 +217,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 	const KindSrc lsrc[4] = {{PC_DIRECT, &Ls.pc_d_ptr, &Ls.pc_d, false}, {PP_DIRECT, &Ls.pp_d_ptr, &Ls.pp_d, true},
 			{PC_EWALD, &Ls.pc_e_ptr, &Ls.pc_e, false}, {PP_EWALD, &Ls.pp_e_ptr, &Ls.pp_e, true}};
 	// per leaf and kind: entries, runs, near-prefix length (PP direct) or image count (PP Ewald), items
-	std::vector<int64_t> lent(size_t(4 * nl)), lrun(size_t(4 * nl)), lnear(size_t(4 * nl)), litem(size_t(4 * nl));
+	hvec<int64_t> lent(size_t(4 * nl)), lrun(size_t(4 * nl)), lnear(size_t(4 * nl)), litem(size_t(4 * nl));
 	std::vector<uint8_t> pp_near(Ls.pp_d.size() / 2, 0);
 #pragma omp parallel for schedule(dynamic, 64)
 	for (int64_t i = 0; i < nl; i++) {
@@ -253,7 +256,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			litem[o] = ent > 0 ? per_leaf : 0;
 		}
 	}
-	std::vector<int64_t> olent(size_t(4 * nl)), olrun(size_t(4 * nl)), olitem(size_t(4 * nl));
+	hvec<int64_t> olent(size_t(4 * nl)), olrun(size_t(4 * nl)), olitem(size_t(4 * nl));
 	{
 		int64_t lrt[4] = {}, lit[4] = {};
 		for (int64_t i = 0; i < nl; i++)
@@ -275,7 +278,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			items[kind].resize(size_t(lit[q]));
 		}
 	}
-	std::vector<int4> leaf_items(static_cast<size_t>(nl));
+	hvec<int4> leaf_items(static_cast<size_t>(nl));
 #pragma omp parallel for schedule(dynamic, 64)
 	for (int64_t i = 0; i < nl; i++) {
 		const int32_t lf = lvs[size_t(i)];
@@ -328,8 +331,8 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			return TG_EINVAL;
 		}
 	// root-to-leaf paths for the local-expansion pass
-	std::vector<int> path, level(nn);
-	for (int64_t i = 0; i < nn; i++) level[i] = T.nodes[i].level;
+	hvec<int> path, level(static_cast<size_t>(nn));
+	for (int64_t i = 0; i < nn; i++) level[size_t(i)] = T.nodes[size_t(i)].level;
 	{
 		std::vector<int64_t> poff(size_t(nl) + 1, 0);
 		for (int64_t i = 0; i < nl; i++) poff[size_t(i) + 1] = poff[size_t(i)] + T.nodes[lvs[size_t(i)]].level + 1;

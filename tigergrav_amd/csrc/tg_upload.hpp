@@ -16,26 +16,25 @@ struct ListRun {
 	int32_t value;
 	int32_t len;
 	int64_t dst;
-	// no zero fill when a run table is sized: every entry is written afterwards, in parallel
-	ListRun() {}
+	ListRun() = default;
 	ListRun(int32_t v, int32_t l, int64_t d) : value(v), len(l), dst(d) {}
 };
 
 // everything a solver uploads, in host memory
 struct HostWork {
-	std::vector<int4> parts;      // (x, y, z, mass bits), tree order
-	std::vector<MultiSrc> multis;
-	std::vector<int4> npos;       // node centres
-	std::vector<int4> items[NKINDS];        // leaf kinds: (first sink, count | extra, list begin, end)
-	std::vector<int> lists[NKINDS];         // lists built on the host (PC kinds)
-	std::vector<ListRun> runs[NKINDS];      // range kinds (CP, PP): runs the device expands
-	std::vector<ListRun> owner_runs[NKINDS];  // node kinds: partial-slot owner of every entry, as runs
-	std::vector<int2> waves[NKINDS];        // node kinds: [begin, end) entry runs of <= 64
+	hvec<int4> parts;      // (x, y, z, mass bits), tree order
+	hvec<MultiSrc> multis;
+	hvec<int4> npos;       // node centres
+	hvec<int4> items[NKINDS];        // leaf kinds: (first sink, count | extra, list begin, end)
+	hvec<int> lists[NKINDS];         // lists built on the host (PC kinds)
+	hvec<ListRun> runs[NKINDS];      // range kinds (CP, PP): runs the device expands
+	hvec<ListRun> owner_runs[NKINDS];  // node kinds: partial-slot owner of every entry, as runs
+	hvec<int2> waves[NKINDS];        // node kinds: [begin, end) entry runs of <= 64
 	int64_t list_len[NKINDS] = {};
-	std::vector<int> slot_node;
-	std::vector<int64_t> slot_ptr;
-	std::vector<int4> leaf_items;  // (node, first particle, end, root-path offset)
-	std::vector<int> path, level;
+	hvec<int> slot_node;
+	hvec<int64_t> slot_ptr;
+	hvec<int4> leaf_items;  // (node, first particle, end, root-path offset)
+	hvec<int> path, level;
 	int64_t n_slots = 0;
 };
 

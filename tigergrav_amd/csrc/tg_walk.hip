@@ -22,6 +22,7 @@
 #include "tg_walk.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -371,7 +372,10 @@ struct StepLog {
 
 }  // namespace
 
-hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
+static double wall() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t_out, unsigned keep_mask, int32_t** kept) {
+	double tw[5] = {wall(), 0, 0, 0, 0};
 	const int64_t nn = int64_t(T.nodes.size());
 	hipError_t e = hipSuccess;
 	void* dev_nodes = nullptr;
@@ -438,6 +442,8 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 		WCHK(hipMemcpyAsync(dev_units, &r0, sizeof(r0), hipMemcpyHostToDevice, s));
 		nu = 1;
 	}
+	WCHK(hipStreamSynchronize(s));
+	tw[1] = wall();
 	while (nu > 0) {
 		const unsigned g = unsigned((nu + 63) / 64);
 		StepLog L;
@@ -521,6 +527,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 		e = hipErrorUnknown;
 		goto done;
 	}
+	tw[2] = wall();
 	{
 		// node-order CSR per kind: per-node lengths over all steps, their scan, then every step's
 		// segments placed in step order
@@ -570,13 +577,20 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 		}
 		Lists& Ls = T.lists;
 		Ls = Lists();
-		std::vector<int64_t>* ptrs[K_N] = {&Ls.cc_d_ptr, &Ls.cp_d_ptr, &Ls.cc_e_ptr, &Ls.cp_e_ptr,
+		hvec<int64_t>* ptrs[K_N] = {&Ls.cc_d_ptr, &Ls.cp_d_ptr, &Ls.cc_e_ptr, &Ls.cp_e_ptr,
 				&Ls.pc_d_ptr, &Ls.pp_d_ptr, &Ls.pc_e_ptr, &Ls.pp_e_ptr};
-		std::vector<int32_t>* vals[K_N] = {&Ls.cc_d, &Ls.cp_d, &Ls.cc_e, &Ls.cp_e, &Ls.pc_d, &Ls.pp_d, &Ls.pc_e, &Ls.pp_e};
+		hvec<int32_t>* vals[K_N] = {&Ls.cc_d, &Ls.cp_d, &Ls.cc_e, &Ls.cp_e, &Ls.pc_d, &Ls.pp_d, &Ls.pc_e, &Ls.pp_e};
 		for (int k = 0; k < K_N && e == hipSuccess; k++) {
 			ptrs[k]->resize(size_t(np));
-			vals[k]->resize(size_t(tot[k]));
+			if (!(kept && (keep_mask >> k) & 1u)) vals[k]->resize(size_t(tot[k]));
 			e = hipMemcpyAsync(ptrs[k]->data(), dev_ptr + k * np, size_t(np) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
+			if (kept && (keep_mask >> k) & 1u) {
+				vals[k]->clear();
+				kept[k] = fin[k].p;  // the placement kernels above write it; the caller owns it now
+				fin[k].p = nullptr;
+				fin[k].cap = 0;
+				continue;
+			}
 			if (e == hipSuccess && tot[k] > 0)
 				e = hipMemcpyAsync(vals[k]->data(), fin[k].p, size_t(tot[k]) * sizeof(int32_t), hipMemcpyDeviceToHost, s);
 		}
@@ -586,7 +600,11 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err) {
 			err = std::string("list assembly: ") + hipGetErrorString(e);
 			goto done;
 		}
+		tw[3] = wall();
 		T.count_census();
+		tw[4] = wall();
+		if (t_out)
+			for (int k = 0; k < 4; k++) t_out[k] = tw[k + 1] - tw[k];
 	}
 done:
 #undef WCHK

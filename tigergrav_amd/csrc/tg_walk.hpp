@@ -10,6 +10,12 @@ namespace tg {
 
 // Tree::walk on the device: fills T.lists and T.census with exactly the lists the host walk
 // builds (same entries, same order).  Returns hipSuccess or the first HIP error; err names it.
-hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err);
+// t, if given, receives the seconds spent in set-up, the walk steps, list assembly with the copy
+// to the host, and the census.  Lists k (in the order cc_d, cp_d, cc_e, cp_e, pc_d, pp_d, pc_e,
+// pp_e) with bit k of keep_mask set stay on the device: kept[k] receives the device array of
+// their values (the caller frees it with hipFree, whatever the return status) and the host
+// vector of values is left empty; their pointer arrays are copied to the host as for the others.
+hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t = nullptr, unsigned keep_mask = 0,
+		int32_t** kept = nullptr);
 
 }  // namespace tg

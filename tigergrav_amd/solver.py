@@ -21,7 +21,7 @@ class SolveInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("n_nodes", C.c_int64), ("n_leaves", C.c_int64), ("max_level", C.c_int64),
                 ("count", C.c_uint64 * 8), ("items", C.c_int64 * 8), ("list_len", C.c_int64 * 8),
                 ("list_bytes", C.c_int64), ("t_tree", C.c_double), ("t_multipoles", C.c_double),
-                ("t_walk", C.c_double), ("t_upload", C.c_double)]
+                ("t_walk", C.c_double), ("t_upload", C.c_double), ("t_detail", C.c_double * 6)]
 
 
 _bound = False
@@ -164,6 +164,8 @@ class Solver:
             "list_len": {k: int(s.list_len[i]) for i, k in enumerate(KINDS)},
             "list_bytes": s.list_bytes, "t_tree": s.t_tree, "t_multipoles": s.t_multipoles, "t_walk": s.t_walk,
             "t_upload": s.t_upload,
+            "t_detail": dict(zip(("walk_setup", "walk_steps", "walk_assembly", "census", "prepare", "transfer"),
+                                 (float(v) for v in s.t_detail))),
         }
 
     _DT = {"pos": (np.int32, 3), "order": (np.int64, 1), "mass": (np.float32, 1), "leaves": (np.int32, 1),
