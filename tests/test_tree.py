@@ -79,11 +79,14 @@ def py_build(pos, ppn):
     return np.array(pos, np.int32), np.array(order), sorted(ranges)
 
 
-@pytest.mark.parametrize("n,ppn", [(1, 1), (7, 1), (100, 4), (2000, 16), (3000, 64)])
-def test_partition_matches_restatement(n, ppn):
+@pytest.mark.parametrize("n,ppn,gen", [(1, 1, "uniform"), (7, 1, "uniform"), (100, 4, "uniform"), (2000, 16, "uniform"),
+                                       (3000, 64, "uniform"),
+                                       # nodes above 2^16 particles are split by all threads at once
+                                       (140000, 64, "uniform"), (140000, 128, "plummer")])
+def test_partition_matches_restatement(n, ppn, gen):
     from tigergrav_amd import ics
 
-    x = ics.uniform(n, seed=n)
+    x = ics.uniform(n, seed=n) if gen == "uniform" else ics.plummer_spheres(n, seed=n)
     s = make_solver(x, ewald=True, ppn=ppn)
     pos, order, ranges = py_build(x, ppn)
     np.testing.assert_array_equal(s.export("pos"), pos)
