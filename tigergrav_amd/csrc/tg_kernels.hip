@@ -21,13 +21,10 @@
 
 namespace tg {
 
-__constant__ float c_real[TG_NREAL * 3];
-__constant__ float c_four[TG_NFOUR * 38];
-__constant__ double c_ser[5 * TG_NSER];
-__constant__ float c_four_f0;  // the Fourier part of the correction at x = 0: sum of the D0 coefficients
+__constant__ const float* c_ewt;
 
-// ---- per-device lattice tables -------------------------------------------------------------
-static void build_tables(std::vector<float>& real, std::vector<float>& four, std::vector<double>& ser) {
+// ---- lattice tables and the tabulated correction -------------------------------------------
+static void build_tables(std::vector<float>& real, std::vector<float>& four) {
 	real.clear();
 	// ordered by |n|^2, so a prefix of the table holds every image within a given distance
 	for (int n2want = 1; n2want <= 11; n2want++)
@@ -64,22 +61,10 @@ static void build_tables(std::vector<float>& real, std::vector<float>& four, std
 							h[T4[t].i[3]] * c0);
 				four.insert(four.end(), row, row + 38);
 			}
-	// erf(2r)/r = sum_q c_q u^q, u = r^2, c_q = (2/sqrt(pi)) (-1)^q 2^(2q+1) / (q! (2q+1));
-	// d_k = (r^-1 d/dr)^k of it = sum_q c_q (2q)(2q-2)...(2q-2k+2) u^(q-k)
-	ser.assign(5 * TG_NSER, 0.0);
-	for (int k = 0; k < 5; k++)
-		for (int j = 0; j < TG_NSER; j++) {
-			const int q = j + k;
-			double c = 2.0 / std::sqrt(M_PI) * ((q & 1) ? -1.0 : 1.0) * std::ldexp(1.0, 2 * q + 1) / std::tgamma(q + 1.0) /
-			           (2 * q + 1);
-			for (int t = 0; t < k; t++) c *= double(2 * q - 2 * t);
-			ser[k * TG_NSER + j] = c;
-		}
 }
 
 void ewald_tables(std::vector<float>& real, std::vector<float>& four) {
-	std::vector<double> ser;
-	build_tables(real, four, ser);
+	build_tables(real, four);
 }
 
 int ewald_images_within(double radius) {
@@ -100,8 +85,7 @@ hipError_t init_device_tables(int device) {
 	if ((int) done.size() <= device) done.resize(device + 1, 0);
 	if (done[device]) return hipSuccess;
 	std::vector<float> real, four;
-	std::vector<double> ser;
-	build_tables(real, four, ser);
+	build_tables(real, four);
 	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
 	// the image prefixes (ewald_images_upto) assume the table sorted by |n|^2
 	for (int i = 0; i < TG_NREAL; i++) {
@@ -110,12 +94,17 @@ hipError_t init_device_tables(int device) {
 	}
 	hipError_t e;
 	if ((e = hipSetDevice(device)) != hipSuccess) return e;
-	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;
-	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_four), four.data(), four.size() * sizeof(float))) != hipSuccess) return e;
-	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ser), ser.data(), ser.size() * sizeof(double))) != hipSuccess) return e;
-	float f0 = 0.0f;
-	for (int i = 0; i < TG_NFOUR; i++) f0 += four[38 * i + 3];
-	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_four_f0), &f0, sizeof(f0))) != hipSuccess) return e;
+	{
+		// the tabulated correction: built once per process on the host, copied to every device, and
+		// kept until the process ends
+		static std::vector<float> table;
+		if (table.empty()) ewald_table_build(table);
+		float* dt = nullptr;
+		if ((e = hipMalloc(&dt, table.size() * sizeof(float))) != hipSuccess) return e;
+		if ((e = hipMemcpy(dt, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) return e;
+		const float* cdt = dt;
+		if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ewt), &cdt, sizeof(cdt))) != hipSuccess) return e;
+	}
 	done[device] = 1;
 	return hipSuccess;
 }
@@ -411,45 +400,15 @@ hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s) {
 }
 
 // ---- PP Ewald correction -------------------------------------------------------------------------
-// The periodic correction between particle pairs (src/gravity.cpp:491-614), per item in three parts:
-//  1. structure factors of the item's sources for the 61 half-space Fourier vectors (lane = h):
-//     C_h = sum_j w_j cos(2 pi h.x_j), S_h = sum_j w_j sin(2 pi h.x_j), with the phase h.X taken
-//     exactly in 32-bit integer arithmetic (positions are 2^-32 fixed point) and sin/cos in
-//     revolutions;
-//  2. per pair the real-space part: the n = 0 term analytically and the images of the |n|-sorted
-//     table that can lie within 2.6 of any pair of the item (the item's image count), + pi/4;
-//  3. per sink the Fourier part sum_h c_h (cos_i C_h + sin_i S_h), which equals the per-pair sums
-//     sum_j w_j c_h cos(2 pi h.(x_i - x_j)) of the reference.
-// Pairs closer than 1e-6 take the reference's constant 2.8372975 and zero force (its zmask); their
-// share of the structure-factor sum (c_h per unit weight at coincidence) is taken back out.
-__device__ __forceinline__ void green_ewald1_real(float& D0, float* D1, const float* x, int nimg) {
-	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
-	float d[2];
-	ewald_self_factors<1>(double(r2), d);
-	float a0 = d[0];
-	float a1[3] = {x[0] * d[1], x[1] * d[1], x[2] * d[1]};
-#pragma unroll 4
-	for (int i = 0; i < nimg; i++) {
-		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
-		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
-		float e[2];
-		ewald_real_factors<1>(q2, e);
-		a0 += e[0];
-#pragma unroll
-		for (int a = 0; a < 3; a++) a1[a] = fmaf(dx[a], e[1], a1[a]);
-	}
-	a0 += 0.78539816339744831f;
-	const bool z = !(r2 > 1e-12f);
-	D0 = z ? 2.8372975f - c_four_f0 : a0;
-#pragma unroll
-	for (int a = 0; a < 3; a++) D1[a] = z ? 0.0f : a1[a];
-}
-
+// The periodic correction between particle pairs (src/gravity.cpp:491-614): per pair the potential
+// and gradient of the tabulated correction (green_ewald_tab<4>: the reference's real-space images,
+// Fourier terms, n = 0 term and pi/4 in one table), weighted by the source mass; pairs closer than
+// 1e-6 take the reference's constant 2.8372975 and zero force (its zmask).  Sinks and sources are
+// laid out as in k_pc: 2^lg lanes per sink, the other lane bits split the tile's sources.
 __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 	const int4 it = A.items[blockIdx.x];
 	const int lane = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
-	const int nimg = min(TG_NREAL, int(unsigned(it.y) >> PP_NEAR_SHIFT));
 	const int lg = ceil_log2_64(ns);
 	const int nsp = 1 << lg;
 	const int P = 64 >> lg;
@@ -457,49 +416,30 @@ __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 	const int ph = lane >> lg;
 	const int4 me = A.sinks[it.x + min(si, ns - 1)];
 	__shared__ int4 tile[64];
-	__shared__ float sfc[64], sfs[64];
-	constexpr float REV = 2.3283064365386963e-10f;  // 2^-32
-	// 1. structure factors, lane = Fourier vector
-	const int hx = lane < TG_NFOUR ? int(c_four[38 * lane]) : 0;
-	const int hy = lane < TG_NFOUR ? int(c_four[38 * lane + 1]) : 0;
-	const int hz = lane < TG_NFOUR ? int(c_four[38 * lane + 2]) : 0;
-	float C = 0.f, S = 0.f;
 	double dphi = 0.0, dg[3] = {0.0, 0.0, 0.0};
 	for (int base = it.z; base < it.w; base += 64) {
 		const int n = min(64, it.w - base);
 		__syncthreads();
 		if (lane < n) tile[lane] = A.srcs[A.list ? A.list[base + lane] : base + lane];
 		__syncthreads();
-		for (int k = 0; k < n; k++) {
-			const int4 s = tile[k];
-			const unsigned phs = unsigned(hx) * unsigned(s.x) + unsigned(hy) * unsigned(s.y) + unsigned(hz) * unsigned(s.z);
-			const float t = float(int(phs)) * REV;
-			const float w = __int_as_float(s.w);
-			C = fmaf(w, __builtin_amdgcn_cosf(t), C);
-			S = fmaf(w, __builtin_amdgcn_sinf(t), S);
-		}
-		// 2. real-space part of every pair of this tile
 		float phi = 0.f, g[3] = {0.f, 0.f, 0.f};
 		for (int k = ph; k < n; k += P) {
 			const int4 s = tile[k];
 			const float x[3] = {sepu<true>(me.x, s.x) * POS_INVF, sepu<true>(me.y, s.y) * POS_INVF,
 					sepu<true>(me.z, s.z) * POS_INVF};
-			float D0, D1[3];
-			green_ewald1_real(D0, D1, x, nimg);
+			float D[4];
+			green_ewald_tab<4>(D, x);
 			const float w = __int_as_float(s.w);
-			phi = fmaf(w, D0, phi);
-			g[0] = fmaf(-w, D1[0], g[0]);
-			g[1] = fmaf(-w, D1[1], g[1]);
-			g[2] = fmaf(-w, D1[2], g[2]);
+			phi = fmaf(w, D[0], phi);
+			g[0] = fmaf(-w, D[1], g[0]);
+			g[1] = fmaf(-w, D[2], g[1]);
+			g[2] = fmaf(-w, D[3], g[2]);
 		}
 		dphi += phi;
 		dg[0] += g[0];
 		dg[1] += g[1];
 		dg[2] += g[2];
 	}
-	sfc[lane] = C;
-	sfs[lane] = S;
-	__syncthreads();
 	for (int off = nsp; off < 64; off <<= 1) {
 		dphi += wave_xor(dphi, off);
 		dg[0] += wave_xor(dg[0], off);
@@ -507,26 +447,11 @@ __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 		dg[2] += wave_xor(dg[2], off);
 	}
 	if (ph == 0 && si < ns) {
-		// 3. Fourier part at this sink
-		float fphi = 0.f, fg[3] = {0.f, 0.f, 0.f};
-		for (int h = 0; h < TG_NFOUR; h++) {
-			const float* H = c_four + 38 * h;
-			const unsigned phs = unsigned(int(H[0])) * unsigned(me.x) + unsigned(int(H[1])) * unsigned(me.y) +
-			                     unsigned(int(H[2])) * unsigned(me.z);
-			const float t = float(int(phs)) * REV;
-			const float cs = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
-			const float Ch = sfc[h], Sh = sfs[h];
-			fphi = fmaf(H[3], fmaf(cs, Ch, sn * Sh), fphi);
-			const float q = fmaf(sn, Ch, -cs * Sh);
-			fg[0] = fmaf(-H[4], q, fg[0]);
-			fg[1] = fmaf(-H[5], q, fg[1]);
-			fg[2] = fmaf(-H[6], q, fg[2]);
-		}
 		double* o = A.out + 4 * (size_t) (it.x + si);
-		o[0] += dphi + double(fphi);
-		o[1] += dg[0] + double(fg[0]);
-		o[2] += dg[1] + double(fg[1]);
-		o[3] += dg[2] + double(fg[2]);
+		o[0] += dphi;
+		o[1] += dg[0];
+		o[2] += dg[1];
+		o[3] += dg[2];
 	}
 }
 

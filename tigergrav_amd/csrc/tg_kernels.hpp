@@ -2,11 +2,12 @@
 //
 // Work is described by int4 items.  Leaf kinds (PP/PC: sinks are particles):
 //   item = (first sink, sink count <= 64, list begin, list end)
-// The count field of every leaf item is read through PP_COUNT_MASK; for PP Ewald the upper bits
-// carry the number of real-space images the item needs (PP_NEAR_ALL: all); for PP direct they carry the length of the list's near prefix (sources that
-// may lie within the softening length): y = count | near_len << PP_NEAR_SHIFT, with
-// near_len == PP_NEAR_ALL meaning the whole list.
-// and results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
+// The count field of every leaf item is read through PP_COUNT_MASK.  For PP direct the upper bits
+// carry the length of the list's near prefix (sources that may lie within the softening length):
+// y = count | near_len << PP_NEAR_SHIFT, with near_len == PP_NEAR_ALL meaning the whole list.  For
+// PP Ewald they carry the number of real-space images the adjoint's direct sums need (PP_NEAR_ALL:
+// all); the forward reads the tabulated correction instead.
+// Results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
 // atomics are needed while kernels of one stream run in order).
 // Node kinds (CC/CP: the sink is an expansion centre): a wave takes a run of <= 64 consecutive list
 // entries, waves[w] = [begin, end); entry j belongs to partial slot owner[j], whose expansion centre

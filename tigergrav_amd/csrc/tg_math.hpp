@@ -9,6 +9,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
+#include "tg_ewald_table.hpp"
+
 namespace tg {
 
 // ---- compile-time slot tables ----------------------------------------------------------------
@@ -204,17 +208,13 @@ __device__ __forceinline__ void m2p(float& phi, float* g, const float* M, const 
 }
 
 // ---- Ewald periodic correction ----------------------------------------------------------------
-// Lattice tables (filled once per device by the host, see tg_kernels.hip):
+// Lattice tables of the adjoint's direct sums (built by ewald_tables, tg_kernels.hip):
 //   real-space vectors n != 0 with |n|^2 <= 11 (170 of them); the reference sums |n|^2 <= 17 but the
 //   terms with |n|^2 >= 12 have |x - n| >= 2.6 for every wrapped x and contribute below 1e-10;
 //   Fourier vectors h != 0, |h|^2 <= 9, one of each +-h pair (61), with the 35 coefficient-tensor
 //   entries doubled to account for the partner (tigergrav/green.hpp:30-59).
 #define TG_NREAL 170
 #define TG_NFOUR 61
-#define TG_NSER 32
-extern __constant__ float c_real[TG_NREAL * 3];
-extern __constant__ float c_four[TG_NFOUR * 38];  // h_x, h_y, h_z, then 35 coefficients
-extern __constant__ double c_ser[5 * TG_NSER];    // series of erf(2r)/r radial factors in r^2
 
 // erfc for x >= 0 with relative error < 1.2e-7 everywhere (Numerical Recipes erfcc, Chebyshev
 // fit); e = exp(-x^2) is returned as a by-product.
@@ -235,122 +235,6 @@ __device__ __forceinline__ float erfc_nr(float x, float& e) {
 	return t * e * __builtin_amdgcn_exp2f(p * L2E);
 }
 
-// radial factors d0..d4 of -erfc(2r)/r for a lattice image at distance r (r >= ~0.5)
-template <int ORDER>
-__device__ __forceinline__ void ewald_real_factors(float r2, float* d) {
-	const float ri = __builtin_amdgcn_rsqf(r2);
-	const float r = r2 * ri;
-	float e;
-	const float erfc = erfc_nr(2.0f * r, e);
-	const float expfactor = 2.2567583341910251f * r * e;  // 4/sqrt(pi) r exp(-4 r^2)
-	const float ri2 = ri * ri;
-	d[0] = -erfc * ri;
-	const float ri3 = ri2 * ri;
-	d[1] = (expfactor + erfc) * ri3;
-	if (ORDER >= 2) {
-		const float ri5 = ri3 * ri2, ri7 = ri5 * ri2, ri9 = ri7 * ri2;
-		const float r4 = r2 * r2;
-		d[2] = -fmaf(expfactor, fmaf(8.0f, r2, 3.0f), 3.0f * erfc) * ri5;
-		d[3] = fmaf(expfactor, 15.0f + fmaf(40.0f, r2, 64.0f * r4), 15.0f * erfc) * ri7;
-		d[4] = -fmaf(expfactor, fmaf(8.0f * r2, 35.0f + fmaf(56.0f, r2, 64.0f * r4), 105.0f), 105.0f * erfc) * ri9;
-	}
-}
-
-// radial factors of erf(2r)/r (= the n = 0 image minus the Newtonian -1/r it corrects), FP64
-// series in u = r^2, valid for r <= sqrt(3)/2 (every minimum-image separation)
-template <int ORDER>
-__device__ __forceinline__ void ewald_self_factors(double u, float* d) {
-#pragma unroll
-	for (int k = 0; k <= ORDER; k++) {
-		double acc = c_ser[k * TG_NSER + TG_NSER - 1];
-#pragma unroll
-		for (int q = TG_NSER - 2; q >= 0; q--) acc = fma(acc, u, c_ser[k * TG_NSER + q]);
-		d[k] = float(acc);
-	}
-}
-
-// accumulators of the real-space sum: A-terms are the x-monomial parts, B/C the delta parts
-struct EwaldAcc {
-	float A0, A1[3], B1, A2[6], B2[3], C2, A3[10], B3[6], A4[15];
-};
-
-__device__ __forceinline__ void acc_zero(EwaldAcc& s) {
-	s.A0 = s.B1 = s.C2 = 0.0f;
-#pragma unroll
-	for (int i = 0; i < 3; i++) s.A1[i] = s.B2[i] = 0.0f;
-#pragma unroll
-	for (int i = 0; i < 6; i++) s.A2[i] = s.B3[i] = 0.0f;
-#pragma unroll
-	for (int i = 0; i < 10; i++) s.A3[i] = 0.0f;
-#pragma unroll
-	for (int i = 0; i < 15; i++) s.A4[i] = 0.0f;
-}
-
-__device__ __forceinline__ void acc_add(EwaldAcc& s, const float* x, const float* d) {
-	s.A0 += d[0];
-	s.B1 += d[1];
-	s.C2 += d[2];
-#pragma unroll
-	for (int a = 0; a < 3; a++) {
-		s.A1[a] = fmaf(x[a], d[1], s.A1[a]);
-		s.B2[a] = fmaf(x[a], d[2], s.B2[a]);
-	}
-	float x2[6];
-#pragma unroll
-	for (int t = 0; t < 6; t++) {
-		x2[t] = x[T2[t].i[0]] * x[T2[t].i[1]];
-		s.A2[t] = fmaf(x2[t], d[2], s.A2[t]);
-		s.B3[t] = fmaf(x2[t], d[3], s.B3[t]);
-	}
-#pragma unroll
-	for (int t = 0; t < 10; t++) {
-		const float x3 = x2[s2(T3[t].i[0], T3[t].i[1])] * x[T3[t].i[2]];
-		s.A3[t] = fmaf(x3, d[3], s.A3[t]);
-		const float x3d4 = x3 * d[4];
-		// the order-4 slots (abc,d) with d >= c
-#pragma unroll
-		for (int dd = 0; dd < 3; dd++)
-			if (dd >= T3[t].i[2]) {
-				const int q = s4sorted(T3[t].i[0], T3[t].i[1], T3[t].i[2], dd);
-				s.A4[q] = fmaf(x3d4, x[dd], s.A4[q]);
-			}
-	}
-}
-
-__device__ __forceinline__ void acc_assemble(const EwaldAcc& s, float* D) {
-	D[0] = s.A0;
-#pragma unroll
-	for (int a = 0; a < 3; a++) D[1 + a] = s.A1[a];
-#pragma unroll
-	for (int t = 0; t < 6; t++) D[4 + t] = s.A2[t] + (T2[t].i[0] == T2[t].i[1] ? s.B1 : 0.0f);
-#pragma unroll
-	for (int t = 0; t < 10; t++) {
-		const int a = T3[t].i[0], b = T3[t].i[1], c = T3[t].i[2];
-		float v = s.A3[t];
-		if (a == b) v += s.B2[c];
-		if (a == c) v += s.B2[b];
-		if (b == c) v += s.B2[a];
-		D[10 + t] = v;
-	}
-#pragma unroll
-	for (int t = 0; t < 15; t++) {
-		const int a = T4[t].i[0], b = T4[t].i[1], c = T4[t].i[2], d = T4[t].i[3];
-		float v = s.A4[t];
-		if (a == b) v += s.B3[s2(c, d)];
-		if (a == c) v += s.B3[s2(b, d)];
-		if (a == d) v += s.B3[s2(b, c)];
-		if (b == c) v += s.B3[s2(a, d)];
-		if (b == d) v += s.B3[s2(a, c)];
-		if (c == d) v += s.B3[s2(a, b)];
-		const int dd2 = (a == b && c == d) + (a == c && b == d) + (a == d && b == c);
-		if (dd2) v = fmaf(float(dd2), s.C2, v);
-		D[20 + t] = v;
-	}
-}
-
-// Full periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x.
-// The n = 0 image is combined with the subtracted Newtonian term analytically (erf(2r)/r), so
-// nothing cancels; at |x| <= 1e-6 the reference's convention D = (2.8372975, 0, ..., 0) applies.
 // real-space images in the table (sorted by |n|^2: 6, 12, 8, 6, 24, 24, 0, 12, 30, 24, 24 with
 // |n|^2 = 1..11) with 0 < |n|^2 <= k
 __host__ __device__ constexpr int ewald_images_upto(int k) {
@@ -365,46 +249,80 @@ __device__ __forceinline__ int ewald_images_for(float r2) {
 	return R2 > 11.0f ? TG_NREAL : ewald_images_upto(int(ceilf(R2)) - 1);
 }
 
-__device__ __forceinline__ void green_ewald(float* D, const float* x) {
-	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
-	EwaldAcc s;
-	acc_zero(s);
-	{
-		float d[5];
-		ewald_self_factors<4>(double(r2), d);
-		acc_add(s, x, d);
-	}
-	const int nimg = ewald_images_for(r2);
-#pragma unroll 2
-	for (int i = 0; i < nimg; i++) {
-		const float dx[3] = {x[0] - c_real[3 * i], x[1] - c_real[3 * i + 1], x[2] - c_real[3 * i + 2]};
-		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
-		float d[5];
-		ewald_real_factors<4>(q2, d);
-		acc_add(s, dx, d);
-	}
-	acc_assemble(s, D);
-	for (int i = 0; i < TG_NFOUR; i++) {
-		const float* H = c_four + 38 * i;
-		const float w = fmaf(H[2], x[2], fmaf(H[1], x[1], H[0] * x[0]));
-		const float sn = __builtin_amdgcn_sinf(w);  // sin(2 pi w): v_sin_f32 takes revolutions
-		const float cs = __builtin_amdgcn_cosf(w);
-		D[0] = fmaf(H[3], cs, D[0]);
-#pragma unroll
-		for (int a = 0; a < 3; a++) D[1 + a] = fmaf(H[4 + a], sn, D[1 + a]);
-#pragma unroll
-		for (int t = 0; t < 6; t++) D[4 + t] = fmaf(H[7 + t], cs, D[4 + t]);
-#pragma unroll
-		for (int t = 0; t < 10; t++) D[10 + t] = fmaf(H[13 + t], sn, D[10 + t]);
-#pragma unroll
-		for (int t = 0; t < 15; t++) D[20 + t] = fmaf(H[23 + t], cs, D[20 + t]);
-	}
-	D[0] += 0.78539816339744831f;  // pi/4
-	if (!(r2 > 1e-12f)) {
-		D[0] = 2.8372975f;
-#pragma unroll
-		for (int t = 1; t < 35; t++) D[t] = 0.0f;
-	}
+// The tabulated correction (tg_ewald_table.hpp): c_ewt points at the per-device table.
+extern __constant__ const float* c_ewt;
+
+template <int V>
+constexpr int ewt_cx = V;  // forces a constant slot index to be folded at compile time
+
+template <int... S, class F>
+__device__ __forceinline__ void ewt_for_seq(std::integer_sequence<int, S...>, F&& f) {
+	(f(std::integral_constant<int, S>{}), ...);
 }
+
+// The first NOUT components of the periodic-correction tensor (tigergrav/green.hpp:189-286) at a
+// wrapped separation x (|x_i| <= 1/2): a third-order Taylor expansion about the table point nearest
+// |x| (FP64 values of the correction, rounded to float, on a 1/64 grid), the sign of each component
+// flipped by the coordinates that are negative and occur an odd number of times.  NOUT = 35 gives
+// orders 0..4 (the table's slots up to order 7 are read), NOUT = 4 the potential and gradient
+// (slots up to order 4).  Interpolation error below 4e-6 of each order's largest value (2e-6 RMS
+// at order 4, 1.2e-7 at order 1).  At |x| <= 1e-6 the reference's convention
+// D = (2.8372975, 0, ..., 0) applies.
+template <int NOUT>
+__device__ __forceinline__ void green_ewald_tab(float* D, const float* x) {
+	constexpr int NLOAD = NOUT == 35 ? TG_EWT_S : 36;  // slots read: up to order 7, or up to order 4
+	static_assert(NOUT == 35 || NOUT == 4, "orders 0..4 or 0..1");
+	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
+	float d[3], sg[3];
+	int gi[3];
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		const float u = fabsf(x[a]);
+		sg[a] = x[a] < 0.0f ? -1.0f : 1.0f;
+		gi[a] = min(int(fmaf(u, float(TG_EWT_INV), 0.5f)), TG_EWT_N - 1);
+		d[a] = u - float(gi[a]) * (1.0f / TG_EWT_INV);
+	}
+	const float4* tp = reinterpret_cast<const float4*>(c_ewt + size_t((gi[0] * TG_EWT_N + gi[1]) * TG_EWT_N + gi[2]) * TG_EWT_S);
+	float t[NLOAD];
+#pragma unroll
+	for (int q = 0; q < NLOAD / 4; q++) {
+		const float4 v = tp[q];
+		t[4 * q] = v.x;
+		t[4 * q + 1] = v.y;
+		t[4 * q + 2] = v.z;
+		t[4 * q + 3] = v.w;
+	}
+	// the Taylor monomials d^j / j! of orders 1..3 (slots 1..19)
+	float w[20];
+	w[0] = 1.0f;
+	ewt_for_seq(std::make_integer_sequence<int, 19>{}, [&](auto j_) {
+		constexpr int j = decltype(j_)::value + 1;
+		constexpr int n0 = ewt::ST.c[j].n0, n1 = ewt::ST.c[j].n1, n2 = ewt::ST.c[j].n2;
+		float m = ewt::taylor_coef(j);
+#pragma unroll
+		for (int e = 0; e < n0; e++) m *= d[0];
+#pragma unroll
+		for (int e = 0; e < n1; e++) m *= d[1];
+#pragma unroll
+		for (int e = 0; e < n2; e++) m *= d[2];
+		w[j] = m;
+	});
+	const bool z = !(r2 > 1e-12f);
+	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
+		constexpr int k = decltype(k_)::value;
+		float v = t[k];
+		ewt_for_seq(std::make_integer_sequence<int, 19>{}, [&](auto j_) {
+			constexpr int j = decltype(j_)::value + 1;
+			static_assert(ewt::merge(k, j) < NLOAD, "slot read");
+			v = fmaf(t[ewt_cx<ewt::merge(k, j)>], w[j], v);
+		});
+		if constexpr (ewt::odd_in(k, 0)) v *= sg[0];
+		if constexpr (ewt::odd_in(k, 1)) v *= sg[1];
+		if constexpr (ewt::odd_in(k, 2)) v *= sg[2];
+		D[k] = z ? (k == 0 ? 2.8372975f : 0.0f) : v;
+	});
+}
+
+__device__ __forceinline__ void green_ewald(float* D, const float* x) { green_ewald_tab<35>(D, x); }
 
 }  // namespace tg
