@@ -41,13 +41,14 @@ namespace tg {
 __constant__ double adj::c_adj_ser[6 * TG_ADJ_NSER];
 __constant__ float c_adj_real[TG_NREAL * 3];
 __constant__ float c_adj_four[TG_NFOUR * 38];
+__constant__ const float* c_adj_ewt;  // the device's tabulated correction (tg_kernels.hip)
 
 namespace {
 
 using adj::add;
 using adj::merge;
 
-hipError_t init_adjoint_tables() {
+hipError_t init_adjoint_tables(int device) {
 	// erf(2r)/r = sum_q c_q u^q, u = r^2; d_k = (r^-1 d/dr)^k = sum_q c_q (2q)(2q-2)...(2q-2k+2) u^(q-k)
 	std::vector<double> ser(6 * TG_ADJ_NSER, 0.0);
 	for (int k = 0; k <= 5; k++)
@@ -63,7 +64,10 @@ hipError_t init_adjoint_tables() {
 	ewald_tables(real, four);
 	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
 	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_adj_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;
-	return hipMemcpyToSymbol(HIP_SYMBOL(c_adj_four), four.data(), four.size() * sizeof(float));
+	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_adj_four), four.data(), four.size() * sizeof(float))) != hipSuccess) return e;
+	const float* table = ewald_table_on(device);
+	if (!table) return hipErrorNotInitialized;
+	return hipMemcpyToSymbol(HIP_SYMBOL(c_adj_ewt), &table, sizeof(table));
 }
 
 // ---- compile-time contraction helpers ---------------------------------------------------------
@@ -374,16 +378,13 @@ __global__ __launch_bounds__(64) void k_adj_pp(AdjArgs A, AdjItems I) {
 	}
 }
 
-// PP Ewald: the correction's F = -D1, J = -D2.  Real space per pair over the item's image prefix
-// (as the forward); the Fourier part through structure factors of the item's sources, mass- and
-// weight-weighted: C, S = sum_j m_j (cos, sin) 2 pi h.x_j and Cw, Sw = sum_j w_j (cos, sin), so
-//   sum_j D2F(d_kj) (m_j w_k - m_k w_j) = sum_h H2_h [(cos_k C + sin_k S) w_k - m_k (cos_k Cw + sin_k Sw)]
-//   sum_j w_j . D1F(d_kj)             = sum_h H1_h . (sin_k Cw - cos_k Sw)
+// PP Ewald: the correction's F = -D1, J = -D2, per pair from the tabulated correction (its
+// real-space images, Fourier terms and n = 0 term together, as the forward's k_pp_ewald reads them):
+//   dL/dx_k += sum_j J(d_kj) (m_j w_k - m_k w_j),  dL/dm_k += sum_j w_j . D1(d_kj)
 __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 	const int4 it = I.items[blockIdx.x];
 	const int t = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
-	const int nimg = min(TG_NREAL, int(unsigned(it.y) >> PP_NEAR_SHIFT));
 	const bool sink = t < ns;
 	const int pi = it.x + min(t, ns - 1);
 	const int4 me = A.parts[pi];
@@ -392,12 +393,6 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 	const float mi = __int_as_float(me.w);
 	__shared__ int4 tp[64];
 	__shared__ float4 tw[64];
-	__shared__ float sf[8][64];  // C, S, Cw[3], Sw[3] per Fourier vector
-	constexpr float REV = 2.3283064365386963e-10f;
-	const int hx = t < TG_NFOUR ? int(c_adj_four[38 * t]) : 0;
-	const int hy = t < TG_NFOUR ? int(c_adj_four[38 * t + 1]) : 0;
-	const int hz = t < TG_NFOUR ? int(c_adj_four[38 * t + 2]) : 0;
-	float C = 0.f, S = 0.f, Cw[3] = {0.f, 0.f, 0.f}, Sw[3] = {0.f, 0.f, 0.f};
 	double gx[3] = {0.0, 0.0, 0.0}, gm = 0.0;
 	for (int base = it.z; base < it.w; base += 64) {
 		const int n = min(64, it.w - base);
@@ -408,23 +403,6 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 			tw[t] = A.w[j];
 		}
 		__syncthreads();
-		for (int k = 0; k < n; k++) {
-			const int4 s = tp[k];
-			const float4 ws = tw[k];
-			const unsigned ph = unsigned(hx) * unsigned(s.x) + unsigned(hy) * unsigned(s.y) + unsigned(hz) * unsigned(s.z);
-			const float tt = float(int(ph)) * REV;
-			const float cs = __builtin_amdgcn_cosf(tt), sn = __builtin_amdgcn_sinf(tt);
-			const float mj = __int_as_float(s.w);
-			C = fmaf(mj, cs, C);
-			S = fmaf(mj, sn, S);
-			Cw[0] = fmaf(ws.x, cs, Cw[0]);
-			Cw[1] = fmaf(ws.y, cs, Cw[1]);
-			Cw[2] = fmaf(ws.z, cs, Cw[2]);
-			Sw[0] = fmaf(ws.x, sn, Sw[0]);
-			Sw[1] = fmaf(ws.y, sn, Sw[1]);
-			Sw[2] = fmaf(ws.z, sn, Sw[2]);
-		}
-		// real-space part of every pair of this tile
 		float fx[3] = {0.f, 0.f, 0.f}, fm = 0.f;
 		for (int k = 0; k < n; k++) {
 			const int4 s = tp[k];
@@ -434,17 +412,7 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 			const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
 			if (!(r2 > 1e-12f)) continue;  // the reference's constant below r = 1e-6: no derivatives
 			float D[10];
-#pragma unroll
-			for (int q = 0; q < 10; q++) D[q] = 0.0f;
-			float dd[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-			adj::ewald_self_factors5<2>(double(r2), dd);
-			adj::radial_add<10>(D, dd, x);
-			for (int i = 0; i < nimg; i++) {
-				const float y[3] = {x[0] - c_adj_real[3 * i], x[1] - c_adj_real[3 * i + 1], x[2] - c_adj_real[3 * i + 2]};
-				const float q2 = fmaf(y[2], y[2], fmaf(y[1], y[1], y[0] * y[0]));
-				adj::ewald_real_factors5<2>(q2, dd);
-				adj::radial_add<10>(D, dd, y);
-			}
+			green_ewald_tab<10>(D, x, c_adj_ewt);
 			const float v[3] = {mj * wi[0] - mi * ws.x, mj * wi[1] - mi * ws.y, mj * wi[2] - mi * ws.z};
 			for_n<3>([&](auto b_) {
 				constexpr int b = decltype(b_)::value;
@@ -461,43 +429,12 @@ __global__ __launch_bounds__(64) void k_adj_pp_ewald(AdjArgs A, AdjItems I) {
 		for (int b = 0; b < 3; b++) gx[b] += fx[b];
 		gm += fm;
 	}
-	sf[0][t] = C;
-	sf[1][t] = S;
-	sf[2][t] = Cw[0];
-	sf[3][t] = Cw[1];
-	sf[4][t] = Cw[2];
-	sf[5][t] = Sw[0];
-	sf[6][t] = Sw[1];
-	sf[7][t] = Sw[2];
-	__syncthreads();
 	if (!sink) return;
-	// Fourier part at this sink (rows: h, then D0, D1 (3), D2 (6) coefficients of the doubled pair)
-	float fg[3] = {0.f, 0.f, 0.f}, fgm = 0.f;
-	for (int h = 0; h < TG_NFOUR; h++) {
-		const float* H = c_adj_four + 38 * h;
-		const unsigned ph = unsigned(int(H[0])) * unsigned(me.x) + unsigned(int(H[1])) * unsigned(me.y) + unsigned(int(H[2])) * unsigned(me.z);
-		const float tt = float(int(ph)) * REV;
-		const float cs = __builtin_amdgcn_cosf(tt), sn = __builtin_amdgcn_sinf(tt);
-		const float a0 = fmaf(cs, sf[0][h], sn * sf[1][h]);             // sum_j m_j cos(theta_k - theta_j)
-		const float aw[3] = {fmaf(cs, sf[2][h], sn * sf[5][h]), fmaf(cs, sf[3][h], sn * sf[6][h]), fmaf(cs, sf[4][h], sn * sf[7][h])};
-		const float bw[3] = {fmaf(sn, sf[2][h], -cs * sf[5][h]), fmaf(sn, sf[3][h], -cs * sf[6][h]), fmaf(sn, sf[4][h], -cs * sf[7][h])};
-		// D2F_ab(d) = H[7 + s2(a,b)] cos(2 pi h.d), D1F_a(d) = H[4 + a] sin(2 pi h.d)
-		for_n<3>([&](auto b_) {
-			constexpr int b = decltype(b_)::value;
-			float jv = 0.0f;
-			for_n<3>([&](auto a_) {
-				constexpr int a = decltype(a_)::value;
-				jv = fmaf(-H[7 + s2(a, b)], fmaf(a0, wi[a], -mi * aw[a]), jv);
-			});
-			fg[b] += jv;
-			fgm = fmaf(H[4 + b], bw[b], fgm);
-		});
-	}
 	double* o = A.grad + 4 * int64_t(pi);
-	o[0] += gx[0] + double(fg[0]);
-	o[1] += gx[1] + double(fg[1]);
-	o[2] += gx[2] + double(fg[2]);
-	o[3] += gm + double(fgm);
+	o[0] += gx[0];
+	o[1] += gx[1];
+	o[2] += gx[2];
+	o[3] += gm;
 }
 
 // PC, sink side: lane = sink, over the item's multipoles, dL/dx_i += w . dg/dx_i (fifth derivatives)
@@ -1008,7 +945,7 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 	}
 	static thread_local int tables_ready = -1;
 	if (tables_ready != device) {
-		ACHK(init_adjoint_tables());
+		ACHK(init_adjoint_tables(device));
 		tables_ready = device;
 	}
 	const int64_t nn = int64_t(tree.nodes.size());

@@ -67,15 +67,11 @@ void ewald_tables(std::vector<float>& real, std::vector<float>& four) {
 	build_tables(real, four);
 }
 
-int ewald_images_within(double radius) {
-	int n = 0;
-	for (int i = -4; i <= 4; i++)
-		for (int j = -4; j <= 4; j++)
-			for (int k = -4; k <= 4; k++) {
-				const int n2 = i * i + j * j + k * k;
-				if (n2 > 0 && n2 <= 11 && double(n2) < radius * radius) n++;
-			}
-	return n;
+
+static std::vector<const float*> device_table;  // the tabulated correction on each device
+
+const float* ewald_table_on(int device) {
+	return device >= 0 && device < (int) device_table.size() ? device_table[size_t(device)] : nullptr;
 }
 
 hipError_t init_device_tables(int device) {
@@ -104,6 +100,8 @@ hipError_t init_device_tables(int device) {
 		if ((e = hipMemcpy(dt, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) return e;
 		const float* cdt = dt;
 		if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ewt), &cdt, sizeof(cdt))) != hipSuccess) return e;
+		if ((int) device_table.size() <= device) device_table.resize(size_t(device) + 1, nullptr);
+		device_table[size_t(device)] = cdt;
 	}
 	done[device] = 1;
 	return hipSuccess;
@@ -428,7 +426,7 @@ __global__ __launch_bounds__(64) void k_pp_ewald(LeafArgs A) {
 			const float x[3] = {sepu<true>(me.x, s.x) * POS_INVF, sepu<true>(me.y, s.y) * POS_INVF,
 					sepu<true>(me.z, s.z) * POS_INVF};
 			float D[4];
-			green_ewald_tab<4>(D, x);
+			green_ewald_tab<4>(D, x, c_ewt);
 			const float w = __int_as_float(s.w);
 			phi = fmaf(w, D[0], phi);
 			g[0] = fmaf(-w, D[1], g[0]);

@@ -4,9 +4,7 @@
 //   item = (first sink, sink count <= 64, list begin, list end)
 // The count field of every leaf item is read through PP_COUNT_MASK.  For PP direct the upper bits
 // carry the length of the list's near prefix (sources that may lie within the softening length):
-// y = count | near_len << PP_NEAR_SHIFT, with near_len == PP_NEAR_ALL meaning the whole list.  For
-// PP Ewald they carry the number of real-space images the adjoint's direct sums need (PP_NEAR_ALL:
-// all); the forward reads the tabulated correction instead.
+// y = count | near_len << PP_NEAR_SHIFT, with near_len == PP_NEAR_ALL meaning the whole list.
 // Results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
 // atomics are needed while kernels of one stream run in order).
 // Node kinds (CC/CP: the sink is an expansion centre): a wave takes a run of <= 64 consecutive list
@@ -69,10 +67,9 @@ enum Kind { PP_DIRECT = 0, PC_DIRECT, CC_DIRECT, CP_DIRECT, PP_EWALD, PC_EWALD, 
 
 // one-time per-device setup of the Ewald lattice tables; returns hipSuccess or the error
 hipError_t init_device_tables(int device);
+// the tabulated correction on a device (after init_device_tables), for other translation units
+const float* ewald_table_on(int device);
 
-// number of real-space Ewald images (n != 0, |n|^2 <= 11) with |n| < radius: the prefix of the
-// |n|-sorted table a PP Ewald item needs when no pair of it is farther apart than radius - 2.6
-int ewald_images_within(double radius);
 // the Ewald lattice tables the kernels use (real-space vectors by |n|, 3 floats each; half-space
 // Fourier rows of 38 floats: h, then the 35 doubled coefficients), for other translation units
 void ewald_tables(std::vector<float>& real, std::vector<float>& four);

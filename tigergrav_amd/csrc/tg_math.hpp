@@ -249,8 +249,17 @@ __device__ __forceinline__ int ewald_images_for(float r2) {
 	return R2 > 11.0f ? TG_NREAL : ewald_images_upto(int(ceilf(R2)) - 1);
 }
 
-// The tabulated correction (tg_ewald_table.hpp): c_ewt points at the per-device table.
+// The tabulated correction (tg_ewald_table.hpp): c_ewt points at the per-device table (defined
+// in tg_kernels.hip; another translation unit keeps its own copy of the pointer).
 extern __constant__ const float* c_ewt;
+
+// slots read for the first nout outputs: every slot up to three orders above the highest output
+// order, rounded up to whole float4 loads
+constexpr int ewt_nload(int nout) {
+	const int o = nout <= 1 ? 0 : nout <= 4 ? 1 : nout <= 10 ? 2 : nout <= 20 ? 3 : 4;
+	const int s = (o + 4) * (o + 5) * (o + 6) / 6;
+	return (s + 3) / 4 * 4;
+}
 
 template <int V>
 constexpr int ewt_cx = V;  // forces a constant slot index to be folded at compile time
@@ -260,18 +269,18 @@ __device__ __forceinline__ void ewt_for_seq(std::integer_sequence<int, S...>, F&
 	(f(std::integral_constant<int, S>{}), ...);
 }
 
-// The first NOUT components of the periodic-correction tensor (tigergrav/green.hpp:189-286) at a
-// wrapped separation x (|x_i| <= 1/2): a third-order Taylor expansion about the table point nearest
-// |x| (FP64 values of the correction, rounded to float, on a 1/64 grid), the sign of each component
-// flipped by the coordinates that are negative and occur an odd number of times.  NOUT = 35 gives
-// orders 0..4 (the table's slots up to order 7 are read), NOUT = 4 the potential and gradient
-// (slots up to order 4).  Interpolation error below 4e-6 of each order's largest value (2e-6 RMS
-// at order 4, 1.2e-7 at order 1).  At |x| <= 1e-6 the reference's convention
-// D = (2.8372975, 0, ..., 0) applies.
+// The first NOUT components (all of orders 0..1, 0..2 or 0..4: NOUT = 4, 10, 35) of the
+// periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x (|x_i| <= 1/2):
+// a third-order Taylor expansion about the point of `table` nearest |x| (FP64 values of the
+// correction, rounded to float, on a 1/64 grid), the sign of each component flipped by the
+// coordinates that are negative and occur an odd number of times.  Interpolation error below 4e-6
+// of each order's largest value (2e-6 RMS at order 4, 1.2e-7 at order 1).  At |x| <= 1e-6 the
+// reference's convention D = (2.8372975, 0, ..., 0) applies.
 template <int NOUT>
-__device__ __forceinline__ void green_ewald_tab(float* D, const float* x) {
-	constexpr int NLOAD = NOUT == 35 ? TG_EWT_S : 36;  // slots read: up to order 7, or up to order 4
-	static_assert(NOUT == 35 || NOUT == 4, "orders 0..4 or 0..1");
+__device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const float* table) {
+	constexpr int NLOAD = ewt_nload(NOUT);
+	static_assert(NOUT == 35 || NOUT == 10 || NOUT == 4, "orders 0..4, 0..2 or 0..1");
+	static_assert(NLOAD <= TG_EWT_S, "slots read");
 	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
 	float d[3], sg[3];
 	int gi[3];
@@ -282,7 +291,7 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x) {
 		gi[a] = min(int(fmaf(u, float(TG_EWT_INV), 0.5f)), TG_EWT_N - 1);
 		d[a] = u - float(gi[a]) * (1.0f / TG_EWT_INV);
 	}
-	const float4* tp = reinterpret_cast<const float4*>(c_ewt + size_t((gi[0] * TG_EWT_N + gi[1]) * TG_EWT_N + gi[2]) * TG_EWT_S);
+	const float4* tp = reinterpret_cast<const float4*>(table + size_t((gi[0] * TG_EWT_N + gi[1]) * TG_EWT_N + gi[2]) * TG_EWT_S);
 	float t[NLOAD];
 #pragma unroll
 	for (int q = 0; q < NLOAD / 4; q++) {
@@ -323,6 +332,6 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x) {
 	});
 }
 
-__device__ __forceinline__ void green_ewald(float* D, const float* x) { green_ewald_tab<35>(D, x); }
+__device__ __forceinline__ void green_ewald(float* D, const float* x) { green_ewald_tab<35>(D, x, c_ewt); }
 
 }  // namespace tg

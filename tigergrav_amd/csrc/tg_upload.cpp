@@ -190,26 +190,6 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 		}
 		return d2 <= hnear * hnear;
 	};
-	// the real-space images a leaf's PP-Ewald pairs can need: |n| < 2.6 + the largest separation
-	auto pp_ewald_images = [&](const Node& nd, int64_t kb, int64_t ke) {
-		double dmax2 = 0.0;
-		bool all = false;
-		for (int64_t k = kb; k < ke; k += 2) {
-			double mn[3], mx[3];
-			range_box(Ls.pp_e[size_t(k)], Ls.pp_e[size_t(k) + 1], mn, mx);
-			double d2 = 0.0;
-			for (int d = 0; d < 3; d++) {
-				const double h1 = 0.5 * (nd.rmax[d] - nd.rmin[d]), h2 = 0.5 * (mx[d] - mn[d]);
-				double dc = 0.5 * (nd.rmax[d] + nd.rmin[d]) - 0.5 * (mx[d] + mn[d]);
-				dc -= std::nearbyint(dc);
-				if (h1 + h2 > 0.25) all = true;
-				const double span = std::fabs(dc) + h1 + h2;
-				d2 += span * span;
-			}
-			dmax2 = std::max(dmax2, d2);
-		}
-		return all ? int64_t(PP_NEAR_ALL) : int64_t(ewald_images_within(2.6 + std::sqrt(dmax2) + 1e-6));
-	};
 	std::vector<int32_t> lvs;  // non-empty leaves, in the tree's leaf order
 	for (int32_t lf : T.leaves)
 		if (!T.nodes[lf].empty()) lvs.push_back(lf);
@@ -245,7 +225,6 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 					}
 				} else if (k.kind == PP_EWALD) {
 					range_counts(*k.v, kb, ke, ent, rn);
-					nearl = pp_ewald_images(nd, kb, ke);
 				} else {
 					ent = ke - kb;
 				}
