@@ -38,9 +38,6 @@
 
 namespace tg {
 
-__constant__ double adj::c_adj_ser[6 * TG_ADJ_NSER];
-__constant__ float c_adj_real[TG_NREAL * 3];
-__constant__ float c_adj_four[TG_NFOUR * 38];
 __constant__ const float* c_adj_ewt;  // the device's tabulated correction (tg_kernels.hip)
 
 namespace {
@@ -49,22 +46,6 @@ using adj::add;
 using adj::merge;
 
 hipError_t init_adjoint_tables(int device) {
-	// erf(2r)/r = sum_q c_q u^q, u = r^2; d_k = (r^-1 d/dr)^k = sum_q c_q (2q)(2q-2)...(2q-2k+2) u^(q-k)
-	std::vector<double> ser(6 * TG_ADJ_NSER, 0.0);
-	for (int k = 0; k <= 5; k++)
-		for (int j = 0; j < TG_ADJ_NSER; j++) {
-			const int q = j + k;
-			double c = 2.0 / std::sqrt(M_PI) * ((q & 1) ? -1.0 : 1.0) * std::ldexp(1.0, 2 * q + 1) / std::tgamma(q + 1.0) / (2 * q + 1);
-			for (int t = 0; t < k; t++) c *= double(2 * q - 2 * t);
-			ser[k * TG_ADJ_NSER + j] = c;
-		}
-	hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(adj::c_adj_ser), ser.data(), ser.size() * sizeof(double));
-	if (e != hipSuccess) return e;
-	std::vector<float> real, four;
-	ewald_tables(real, four);
-	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
-	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_adj_real), real.data(), real.size() * sizeof(float))) != hipSuccess) return e;
-	if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_adj_four), four.data(), four.size() * sizeof(float))) != hipSuccess) return e;
 	const float* table = ewald_table_on(device);
 	if (!table) return hipErrorNotInitialized;
 	return hipMemcpyToSymbol(HIP_SYMBOL(c_adj_ewt), &table, sizeof(table));
@@ -458,7 +439,7 @@ __global__ __launch_bounds__(64) void k_adj_pc_sink(AdjArgs A, AdjItems I) {
 		const float d[3] = {sepf(me.x, ms.x[0], wrapx), sepf(me.y, ms.x[1], wrapx), sepf(me.z, ms.x[2], wrapx)};
 		float D[adj::NS5];
 		if (EWALD)
-			adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
+			green_ewald_tab<adj::NS5>(D, d, c_adj_ewt);
 		else
 			adj::green_direct_n<adj::NS5>(D, d);
 		m2p_sink_adjoint(gx, M, D, wi);
@@ -510,7 +491,7 @@ __global__ __launch_bounds__(64) void k_adj_pc_t(AdjPCArgs A) {
 		const float d[3] = {sepf(me.x, cx0, wrapx), sepf(me.y, cx1, wrapx), sepf(me.z, cx2, wrapx)};
 		float D[35];
 		if (EWALD)
-			adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
+			green_ewald_tab<35>(D, d, c_adj_ewt);
 		else
 			adj::green_direct_n<35>(D, d);
 		m2p_source_adjoint(mu, D, wi);
@@ -575,7 +556,7 @@ __global__ __launch_bounds__(64) void k_adj_cp_t(AdjCPArgs A) {
 			const float d[3] = {sepf(X.x, me.x, wrapx), sepf(X.y, me.y, wrapx), sepf(X.z, me.z, wrapx)};
 			float D[adj::NS5];
 			if (EWALD)
-				adj::green_ewald_n<adj::NS5>(D, d, c_adj_real, c_adj_four);
+				green_ewald_tab<adj::NS5>(D, d, c_adj_ewt);
 			else
 				adj::green_direct_n<adj::NS5>(D, d);
 			for_n<35>([&](auto t_) {
@@ -645,7 +626,7 @@ __global__ __launch_bounds__(64) void k_adj_cc_t(AdjCCArgs A) {
 		const float d[3] = {sepf(X.x, ms.x[0], wrapx), sepf(X.y, ms.x[1], wrapx), sepf(X.z, ms.x[2], wrapx)};
 		float D[35];
 		if (EWALD)
-			adj::green_ewald_n<35>(D, d, c_adj_real, c_adj_four);
+			green_ewald_tab<35>(D, d, c_adj_ewt);
 		else
 			adj::green_direct_n<35>(D, d);
 		m2l_adjoint(mu, Lam, D);

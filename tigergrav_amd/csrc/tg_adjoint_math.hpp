@@ -9,6 +9,7 @@
 // ([0], [1+a], [4+s2], [10+s3], [20+s4]; tigergrav/expansion.hpp:77-137) followed by the 21
 // sorted index quintuples in lexicographic order.  Slot k holds the index multiset with counts
 // (n0, n1, n2) of the values 0, 1, 2; every loop over slots is unrolled with compile-time indices.
+// The periodic correction's derivatives come from the forward's table (green_ewald_tab, tg_math.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -123,103 +124,6 @@ __device__ __forceinline__ void green_direct_n(float* D, const float* x) {
 #pragma unroll
 	for (int k = 0; k < NS; k++) D[k] = 0.0f;
 	radial_add<NS>(D, d, x);
-}
-
-// radial factors d0..d5 of -erfc(2r)/r:
-//   d_k = -(-1)^k (2k-1)!! r^-(2k+1) [erfc(2r) + (4/sqrt(pi)) e^(-4r^2) sum_{j=1..k} 8^(j-1) r^(2j-1) / (2j-1)!!]
-template <int K = 5>
-__device__ __forceinline__ void ewald_real_factors5(float r2, float* d) {
-	const float ri = __builtin_amdgcn_rsqf(r2);
-	const float r = r2 * ri;
-	float e;
-	const float ec = erfc_nr(2.0f * r, e);
-	const float g = 2.2567583341910251f * e;  // (2 alpha / sqrt(pi)) e^(-alpha^2 r^2), alpha = 2
-	float sum = 0.0f, term = r, dfact = 1.0f, rpow = ri;  // term = 8^(j-1) r^(2j-1) / (2j-1)!!
-	float sign = -1.0f;
-	d[0] = -ec * ri;
-#pragma unroll
-	for (int k = 1; k <= K; k++) {
-		sum += term;
-		term *= 8.0f * r2 / float(2 * k + 1);
-		dfact *= float(2 * k - 1);
-		rpow *= ri * ri;
-		sign = -sign;
-		d[k] = sign * dfact * rpow * (ec + g * sum);
-	}
-}
-
-// FP64 series of erf(2r)/r in u = r^2 and its radial factors to order 5 (host fills the table)
-#define TG_ADJ_NSER 32
-extern __constant__ double c_adj_ser[6 * TG_ADJ_NSER];
-template <int K = 5>
-__device__ __forceinline__ void ewald_self_factors5(double u, float* d) {
-#pragma unroll
-	for (int k = 0; k <= K; k++) {
-		double acc = c_adj_ser[k * TG_ADJ_NSER + TG_ADJ_NSER - 1];
-#pragma unroll
-		for (int q = TG_ADJ_NSER - 2; q >= 0; q--) acc = fma(acc, u, c_adj_ser[k * TG_ADJ_NSER + q]);
-		d[k] = float(acc);
-	}
-}
-
-// the periodic correction (tigergrav/green.hpp:189-286) and its derivatives to order 5: the n = 0
-// image as erf(2r)/r, the real-space images of the forward's table, the Fourier terms
-// d^k[-(c0/pi) cos(2 pi h.x)] = -(c0/pi) (2 pi)^k h^k cos^(k)(2 pi h.x), and +pi/4; every
-// derivative vanishes below r = 1e-6 where the reference takes the constant 2.8372975
-constexpr float twopi_pow(int n) {
-	float r = 1.0f;
-	for (int i = 0; i < n; i++) r *= 6.283185307179586f;
-	return r;
-}
-template <int K>
-__device__ __forceinline__ float fourier_comp(const float* ph, const float* qh, const float* rh, float c0pi, float cs, float sn) {
-	constexpr int n0 = ST.c[K].n0, n1 = ST.c[K].n1, n2 = ST.c[K].n2, n = n0 + n1 + n2;
-	// d^n/dt^n cos t: cos, -sin, -cos, sin, cos, -sin
-	const float trig = (n % 4 == 0) ? cs : ((n % 4 == 1) ? -sn : ((n % 4 == 2) ? -cs : sn));
-	return -c0pi * twopi_pow(n) * ph[n0] * qh[n1] * rh[n2] * trig;
-}
-template <int... K>
-__device__ __forceinline__ void fourier_add_seq(float* D, const float* ph, const float* qh, const float* rh, float c0pi, float cs,
-		float sn, std::integer_sequence<int, K...>) {
-	((D[K] += fourier_comp<K>(ph, qh, rh, c0pi, cs, sn)), ...);
-}
-
-// the periodic correction (tigergrav/green.hpp:189-286) and its derivatives (first NS slots), in the
-// forward's decomposition: the n = 0 image as erf(2r)/r (FP64 series), the real-space images of the
-// forward's table (|n|^2 <= 11) within the forward's prefix |n| < 2.6 + r, the Fourier vectors of one half-space with doubled coefficients,
-// and +pi/4 on the potential.  Below r = 1e-6 the reference takes 2.8372975 and zero derivatives.
-template <int NS>
-__device__ __forceinline__ void green_ewald_n(float* D, const float* x, const float* real, const float* four) {
-#pragma unroll
-	for (int k = 0; k < NS; k++) D[k] = 0.0f;
-	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
-	if (!(r2 > 1e-12f)) {
-		D[0] = 2.8372975f;
-		return;
-	}
-	constexpr int K = NS <= 1 ? 0 : (NS <= 4 ? 1 : (NS <= 10 ? 2 : (NS <= 20 ? 3 : (NS <= 35 ? 4 : 5))));
-	float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-	ewald_self_factors5<K>(double(r2), d);
-	radial_add<NS>(D, d, x);
-	const int nimg = ewald_images_for(r2);  // the forward's image prefix
-	for (int i = 0; i < nimg; i++) {
-		const float dx[3] = {x[0] - real[3 * i], x[1] - real[3 * i + 1], x[2] - real[3 * i + 2]};
-		const float q2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
-		ewald_real_factors5<K>(q2, d);
-		radial_add<NS>(D, d, dx);
-	}
-	for (int i = 0; i < TG_NFOUR; i++) {
-		const float* H = four + 38 * i;
-		const float w = fmaf(H[2], x[2], fmaf(H[1], x[1], H[0] * x[0]));
-		const float sn = __builtin_amdgcn_sinf(w), cs = __builtin_amdgcn_cosf(w);
-		float ph[6], qh[6], rh[6];
-		powers5(ph, H[0]);
-		powers5(qh, H[1]);
-		powers5(rh, H[2]);
-		// H[3] = -c0/pi (doubled): the D0 coefficient of this pair of vectors
-		fourier_add_seq(D, ph, qh, rh, -H[3], cs, sn, std::make_integer_sequence<int, NS>{});
-	}
-	D[0] += 0.78539816339744831f;
 }
 
 }  // namespace adj

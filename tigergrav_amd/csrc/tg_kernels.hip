@@ -23,51 +23,7 @@ namespace tg {
 
 __constant__ const float* c_ewt;
 
-// ---- lattice tables and the tabulated correction -------------------------------------------
-static void build_tables(std::vector<float>& real, std::vector<float>& four) {
-	real.clear();
-	// ordered by |n|^2, so a prefix of the table holds every image within a given distance
-	for (int n2want = 1; n2want <= 11; n2want++)
-		for (int i = -4; i <= 4; i++)
-			for (int j = -4; j <= 4; j++)
-				for (int k = -4; k <= 4; k++) {
-					if (i * i + j * j + k * k != n2want) continue;
-					real.push_back(float(i));
-					real.push_back(float(j));
-					real.push_back(float(k));
-				}
-	four.clear();
-	for (int i = -3; i <= 3; i++)
-		for (int j = -3; j <= 3; j++)
-			for (int k = -3; k <= 3; k++) {
-				const int h2 = i * i + j * j + k * k;
-				if (h2 == 0 || h2 > 9) continue;
-				// keep one of each +-h pair: first non-zero component positive
-				const int first = i != 0 ? i : (j != 0 ? j : k);
-				if (first < 0) continue;
-				const double h[3] = {double(i), double(j), double(k)};
-				const double c0 = 2.0 * std::exp(-M_PI * M_PI * h2 / 4.0) / h2;  // x2: the -h partner
-				float row[38];
-				row[0] = float(i);
-				row[1] = float(j);
-				row[2] = float(k);
-				row[3] = float(-c0 / M_PI);
-				for (int a = 0; a < 3; a++) row[4 + a] = float(2.0 * h[a] * c0);
-				for (int t = 0; t < 6; t++) row[7 + t] = float(4.0 * M_PI * h[T2[t].i[0]] * h[T2[t].i[1]] * c0);
-				for (int t = 0; t < 10; t++)
-					row[13 + t] = float(-8.0 * M_PI * M_PI * h[T3[t].i[0]] * h[T3[t].i[1]] * h[T3[t].i[2]] * c0);
-				for (int t = 0; t < 15; t++)
-					row[23 + t] = float(-16.0 * M_PI * M_PI * M_PI * h[T4[t].i[0]] * h[T4[t].i[1]] * h[T4[t].i[2]] *
-							h[T4[t].i[3]] * c0);
-				four.insert(four.end(), row, row + 38);
-			}
-}
-
-void ewald_tables(std::vector<float>& real, std::vector<float>& four) {
-	build_tables(real, four);
-}
-
-
+// ---- the tabulated correction -------------------------------------------------------------
 static std::vector<const float*> device_table;  // the tabulated correction on each device
 
 const float* ewald_table_on(int device) {
@@ -80,14 +36,6 @@ hipError_t init_device_tables(int device) {
 	std::lock_guard<std::mutex> lock(mtx);
 	if ((int) done.size() <= device) done.resize(device + 1, 0);
 	if (done[device]) return hipSuccess;
-	std::vector<float> real, four;
-	build_tables(real, four);
-	if (real.size() != TG_NREAL * 3 || four.size() != TG_NFOUR * 38) return hipErrorInvalidValue;
-	// the image prefixes (ewald_images_upto) assume the table sorted by |n|^2
-	for (int i = 0; i < TG_NREAL; i++) {
-		const int n2 = int(std::lround(real[3 * i] * real[3 * i] + real[3 * i + 1] * real[3 * i + 1] + real[3 * i + 2] * real[3 * i + 2]));
-		if (i >= ewald_images_upto(n2) || i < ewald_images_upto(n2 - 1)) return hipErrorInvalidValue;
-	}
 	hipError_t e;
 	if ((e = hipSetDevice(device)) != hipSuccess) return e;
 	{

@@ -65,14 +65,11 @@ struct NodeArgs {
 
 enum Kind { PP_DIRECT = 0, PC_DIRECT, CC_DIRECT, CP_DIRECT, PP_EWALD, PC_EWALD, CC_EWALD, CP_EWALD, NKINDS };
 
-// one-time per-device setup of the Ewald lattice tables; returns hipSuccess or the error
+// one-time per-device setup of the tabulated Ewald correction; returns hipSuccess or the error
 hipError_t init_device_tables(int device);
 // the tabulated correction on a device (after init_device_tables), for other translation units
 const float* ewald_table_on(int device);
 
-// the Ewald lattice tables the kernels use (real-space vectors by |n|, 3 floats each; half-space
-// Fourier rows of 38 floats: h, then the 35 doubled coefficients), for other translation units
-void ewald_tables(std::vector<float>& real, std::vector<float>& four);
 hipError_t launch_pp_direct(const LeafArgs& a, bool do_phi, hipStream_t s);
 hipError_t launch_pc(const LeafArgs& a, bool ewald, hipStream_t s);
 hipError_t launch_pp_ewald(const LeafArgs& a, hipStream_t s);

@@ -208,47 +208,6 @@ __device__ __forceinline__ void m2p(float& phi, float* g, const float* M, const 
 }
 
 // ---- Ewald periodic correction ----------------------------------------------------------------
-// Lattice tables of the adjoint's direct sums (built by ewald_tables, tg_kernels.hip):
-//   real-space vectors n != 0 with |n|^2 <= 11 (170 of them); the reference sums |n|^2 <= 17 but the
-//   terms with |n|^2 >= 12 have |x - n| >= 2.6 for every wrapped x and contribute below 1e-10;
-//   Fourier vectors h != 0, |h|^2 <= 9, one of each +-h pair (61), with the 35 coefficient-tensor
-//   entries doubled to account for the partner (tigergrav/green.hpp:30-59).
-#define TG_NREAL 170
-#define TG_NFOUR 61
-
-// erfc for x >= 0 with relative error < 1.2e-7 everywhere (Numerical Recipes erfcc, Chebyshev
-// fit); e = exp(-x^2) is returned as a by-product.
-__device__ __forceinline__ float erfc_nr(float x, float& e) {
-	const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
-	float p = 0.17087277f;
-	p = fmaf(p, t, -0.82215223f);
-	p = fmaf(p, t, 1.48851587f);
-	p = fmaf(p, t, -1.13520398f);
-	p = fmaf(p, t, 0.27886807f);
-	p = fmaf(p, t, -0.18628806f);
-	p = fmaf(p, t, 0.09678418f);
-	p = fmaf(p, t, 0.37409196f);
-	p = fmaf(p, t, 1.00002368f);
-	p = fmaf(p, t, -1.26551223f);
-	const float L2E = 1.4426950408889634f;
-	e = __builtin_amdgcn_exp2f(-x * x * L2E);
-	return t * e * __builtin_amdgcn_exp2f(p * L2E);
-}
-
-// real-space images in the table (sorted by |n|^2: 6, 12, 8, 6, 24, 24, 0, 12, 30, 24, 24 with
-// |n|^2 = 1..11) with 0 < |n|^2 <= k
-__host__ __device__ constexpr int ewald_images_upto(int k) {
-	return k <= 0 ? 0 : k == 1 ? 6 : k == 2 ? 18 : k == 3 ? 26 : k == 4 ? 32 : k == 5 ? 56 : k <= 7 ? 80 : k == 8 ? 92 : k == 9 ? 122 : k == 10 ? 146 : 170;
-}
-// the images a separation x (minimum image, r = |x|) can need: |n| < 2.6 + r.  Every other image is
-// at least 2.6 away, where erfc(2 r') / r' and its derivatives to fourth order are below 1e-7 of
-// the correction's own (the forward's PP Ewald prefix uses the same bound)
-__device__ __forceinline__ int ewald_images_for(float r2) {
-	const float R = 2.6f + sqrtf(r2) + 1e-6f;
-	const float R2 = R * R;
-	return R2 > 11.0f ? TG_NREAL : ewald_images_upto(int(ceilf(R2)) - 1);
-}
-
 // The tabulated correction (tg_ewald_table.hpp): c_ewt points at the per-device table (defined
 // in tg_kernels.hip; another translation unit keeps its own copy of the pointer).
 extern __constant__ const float* c_ewt;
@@ -269,17 +228,19 @@ __device__ __forceinline__ void ewt_for_seq(std::integer_sequence<int, S...>, F&
 	(f(std::integral_constant<int, S>{}), ...);
 }
 
-// The first NOUT components (all of orders 0..1, 0..2 or 0..4: NOUT = 4, 10, 35) of the
-// periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x (|x_i| <= 1/2):
-// a third-order Taylor expansion about the point of `table` nearest |x| (FP64 values of the
-// correction, rounded to float, on a 1/64 grid), the sign of each component flipped by the
-// coordinates that are negative and occur an odd number of times.  Interpolation error below 4e-6
-// of each order's largest value (2e-6 RMS at order 4, 1.2e-7 at order 1).  At |x| <= 1e-6 the
-// reference's convention D = (2.8372975, 0, ..., 0) applies.
+// The first NOUT components (all of orders 0..1, 0..2, 0..4 or 0..5: NOUT = 4, 10, 35, 56) of
+// the periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x
+// (|x_i| <= 1/2): a third-order Taylor expansion about the point of `table` nearest |x| (FP64
+// values of the correction, rounded to float, on a 1/64 grid), the sign of each component flipped
+// by the coordinates that are negative and occur an odd number of times.  Interpolation error below
+// 4e-6 of each order's largest value (2e-6 RMS at order 4, 1.2e-7 at order 1).  Order-5 components
+// (the adjoint's) take a second-order expansion, the table ending at order 7: error about 1e-4 of
+// that order's largest value.  At |x| <= 1e-6 the reference's convention D = (2.8372975, 0, ..., 0)
+// applies.
 template <int NOUT>
 __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const float* table) {
-	constexpr int NLOAD = ewt_nload(NOUT);
-	static_assert(NOUT == 35 || NOUT == 10 || NOUT == 4, "orders 0..4, 0..2 or 0..1");
+	constexpr int NLOAD = NOUT == 56 ? TG_EWT_S : ewt_nload(NOUT);
+	static_assert(NOUT == 56 || NOUT == 35 || NOUT == 10 || NOUT == 4, "orders 0..5, 0..4, 0..2 or 0..1");
 	static_assert(NLOAD <= TG_EWT_S, "slots read");
 	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
 	float d[3], sg[3];
@@ -319,10 +280,12 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 	const bool z = !(r2 > 1e-12f);
 	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
 		constexpr int k = decltype(k_)::value;
+		// monomials of orders 1..3 (slots 1..19), or 1..2 (slots 1..9) where order 8 would be needed
+		constexpr int jn = ewt::ST.c[k].n0 + ewt::ST.c[k].n1 + ewt::ST.c[k].n2 + 3 <= TG_EWT_ORDER ? 19 : 9;
 		float v = t[k];
-		ewt_for_seq(std::make_integer_sequence<int, 19>{}, [&](auto j_) {
+		ewt_for_seq(std::make_integer_sequence<int, jn>{}, [&](auto j_) {
 			constexpr int j = decltype(j_)::value + 1;
-			static_assert(ewt::merge(k, j) < NLOAD, "slot read");
+			static_assert(ewt::merge(k, j) >= 0 && ewt::merge(k, j) < NLOAD, "slot read");
 			v = fmaf(t[ewt_cx<ewt::merge(k, j)>], w[j], v);
 		});
 		if constexpr (ewt::odd_in(k, 0)) v *= sg[0];
