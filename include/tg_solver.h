@@ -65,10 +65,13 @@ int tg_solver_solve(tg_solver* s);
  * local-expansion pass level by level (the reference's order, src/tree.cpp:400-401) instead of
  * per-leaf sums over the root path (0, the default); "gpu_walk" 1 (the default
  * for device solvers) builds the interaction lists on the device, 0 on the host (the same lists
- * either way: the walk of kick_fmm, src/tree.cpp:378-635; from the next tg_solver_set_particles) */
+ * either way: the walk of kick_fmm, src/tree.cpp:378-635; from the next tg_solver_set_particles);
+ * "overlap" 1 (the default) lets graph replays run the node kinds on a second stream beside the
+ * leaf kinds (the same results bit for bit), 0 keeps one stream */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);
 /* the same solve replayed from a HIP graph captured on its first call after set_particles (one
- * submission instead of nine launches; records no per-kind timings) */
+ * submission instead of nine launches; records no per-kind timings; with "overlap" the node and
+ * leaf kinds run concurrently) */
 int tg_solver_solve_graph(tg_solver* s);
 int tg_solver_synchronize(tg_solver* s);
 /* forces of the last solve, in input particle order (the reference's `force` per particle) */

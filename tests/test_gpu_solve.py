@@ -148,6 +148,30 @@ def test_graph_replay_matches_eager():
             np.testing.assert_array_equal(s.forces(), eager2)
 
 
+def test_overlap_knob_is_bit_identical():
+    """Graph replays with the node kinds on a second stream ("overlap" 1, the default) and on one
+    stream give the eager solve's forces bit for bit, with both local-pass forms."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    x = ics.zeldovich(27, 71)
+    n = x.shape[0]
+    print(f"overlap knob: N={n}", flush=True)
+    s = Solver(ewald=True, soft_len=ics.soft_len(n), m=1.0 / n, theta=0.6, parts_per_node=32, device=0)
+    s.set_particles(x)
+    for sweep in (0, 1):
+        s.tune("local_sweep", sweep)
+        s.solve()
+        s.synchronize()
+        eager = s.forces()
+        for ov in (1, 0, 1):
+            s.tune("overlap", ov)
+            for _ in range(2):
+                s.solve_graph()
+                s.synchronize()
+                np.testing.assert_array_equal(s.forces(), eager)
+
+
 def test_tuning_knobs():
     """The persistent queue-driven PP kernel evaluates each item exactly as the one-group-per-item
     kernel does (bit-identical forces, any residency); a different CC/CP chunk only regroups FP64

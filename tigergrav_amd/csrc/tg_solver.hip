@@ -119,11 +119,12 @@ __global__ __launch_bounds__(64) void k_local(const int4* __restrict__ leaves, i
 				double(float(double(centre_offset(p.z, c.z, wrap)) * POS_INV))};
 		double phi, g[3];
 		l2p(phi, g, Ls, d);
+		// added last, onto the leaf kinds' sums
 		double* o = out + 4 * size_t(i);
-		o[0] = phi - phi_self;
-		o[1] = g[0];
-		o[2] = g[1];
-		o[3] = g[2];
+		o[0] += phi - phi_self;
+		o[1] += g[0];
+		o[2] += g[1];
+		o[3] += g[2];
 	}
 }
 constexpr int LOCAL_LPG = 2;
@@ -174,11 +175,16 @@ __global__ __launch_bounds__(64) void k_l2p_parts(int64_t n, const int* __restri
 	for (int r = 0; r < 35; r++) Ls[r] = src[r];
 	double phi, g[3];
 	l2p(phi, g, Ls, d);
-	double* o = out + 4 * i;
-	o[0] = phi - phi_self;
-	o[1] = g[0];
-	o[2] = g[1];
-	o[3] = g[2];
+	double* o = out + 4 * i;  // added last, onto the leaf kinds' sums
+	o[0] += phi - phi_self;
+	o[1] += g[0];
+	o[2] += g[1];
+	o[3] += g[2];
+}
+
+__global__ __launch_bounds__(256) void k_zero_f64(double* __restrict__ p, int64_t n) {
+	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+	if (i < n) p[i] = 0.0;
 }
 
 static size_t local_lds_bytes(int maxdepth) { return size_t(LOCAL_LPG) * size_t(35 * maxdepth + 35) * sizeof(double); }
@@ -213,11 +219,14 @@ int Solver::init(int dev) {
 	gpu_walk = true;  // the lists on the device by default (bit-identical to the host walk)
 	SCHK(init_device_tables(dev));
 	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+	SCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+	SCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+	SCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 	SCHK(d_queue.alloc(256));
 	int cus = 0;
 	SCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 	resident_groups = cus * 4 * 6;  // six 64-lane work groups per SIMD
-	ev.resize(size_t(2) * TIMER_SPANS * TIMER_RING);
+	ev.resize(size_t(2) * TIMER_EVSPANS * TIMER_RING);
 	for (auto& e : ev) SCHK(hipEventCreate(&e));
 	return TG_OK;
 }
@@ -297,6 +306,12 @@ Solver::~Solver() {
 			w.list.release();
 			w.owner.release();
 		}
+		if (stream2) {
+			hipStreamSynchronize(stream2);
+			hipStreamDestroy(stream2);
+		}
+		if (ev_fork) hipEventDestroy(ev_fork);
+		if (ev_join) hipEventDestroy(ev_join);
 		hipStreamDestroy(stream);
 	}
 }
@@ -590,6 +605,11 @@ int Solver::upload() {
 }
 
 
+// One solve.  The leaf kinds (PC, PP) accumulate into the zeroed outputs; the node kinds (CC, CP),
+// their sums per node and the level-by-level L2L touch none of them, so in an untimed solve (a
+// graph capture) they run on a second stream beside the leaf kinds.  L2P adds last, after both.
+// A timed solve runs everything on one stream in the same order of additions, so both give the
+// same bits.
 int Solver::enqueue_solve(bool timed) {
 	SCHK(hipSetDevice(device));
 	hipStream_t s = stream;
@@ -601,9 +621,15 @@ int Solver::enqueue_solve(bool timed) {
 		if (rc != TG_OK) return rc;
 		solves_folded++;
 	}
-	hipEvent_t* evs = ev.data() + size_t(2) * TIMER_SPANS * size_t(solves_recorded % TIMER_RING);
+	hipEvent_t* evs = ev.data() + size_t(2) * TIMER_EVSPANS * size_t(solves_recorded % TIMER_RING);
 	if (timed) solves_recorded++;
 	auto mark = [&](int k) -> hipError_t { return timed ? hipEventRecord(evs[k], s) : hipSuccess; };
+	const bool fork = overlap && !timed;
+	hipStream_t sn = fork ? stream2 : s;  // the node side
+	if (fork) {
+		SCHK(hipEventRecord(ev_fork, s));
+		SCHK(hipStreamWaitEvent(stream2, ev_fork, 0));
+	}
 	NodeArgs na{};
 	na.sinkpos = d_nodepos.as<int4>();
 	na.srcs = d_parts.as<int4>();
@@ -619,35 +645,30 @@ int Solver::enqueue_solve(bool timed) {
 		na.nwaves = work[k].nitems;
 		SCHK(mark(2 * k));
 		if (k == CC_DIRECT || k == CC_EWALD)
-			SCHK(launch_cc(na, k == CC_EWALD, s));
+			SCHK(launch_cc(na, k == CC_EWALD, sn));
 		else
-			SCHK(launch_cp(na, k == CP_EWALD, s));
+			SCHK(launch_cp(na, k == CP_EWALD, sn));
 		SCHK(mark(2 * k + 1));
 	}
 	e = 2 * NKINDS;
 	SCHK(mark(e));
-	if (nn > 0 && n_leaf_items > 0) {
-		const double phi_self = (-315.0 / 128.0) * m / soft_len;
-		const int maxdepth = tree.max_level + 1;
-		hipLaunchKernelGGL(k_node_sum, dim3(unsigned((nn * 35 + 255) / 256)), dim3(256), 0, s,
+	const bool local_pass = nn > 0 && n_leaf_items > 0;
+	if (local_pass) {
+		hipLaunchKernelGGL(k_node_sum, dim3(unsigned((nn * 35 + 255) / 256)), dim3(256), 0, sn,
 				d_node_slot_ptr.as<int64_t>(), d_partial.as<double>(), d_dL.as<double>(), nn);
 		SCHK(hipGetLastError());
-		if (local_sweep) {
+		if (local_sweep)
 			for (size_t l = 0; l + 1 < level_begin.size(); l++) {
 				const int b = level_begin[l], e2 = level_begin[l + 1];
-				hipLaunchKernelGGL(k_l2l_level, dim3(unsigned((e2 - b + 63) / 64)), dim3(64), 0, s, b, e2, d_parent.as<int>(), d_nodepos.as<int4>(),
+				hipLaunchKernelGGL(k_l2l_level, dim3(unsigned((e2 - b + 63) / 64)), dim3(64), 0, sn, b, e2, d_parent.as<int>(), d_nodepos.as<int4>(),
 						d_dL.as<double>(), d_Lnode.as<double>());
 				SCHK(hipGetLastError());
 			}
-			hipLaunchKernelGGL(k_l2p_parts, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, n, d_pleaf.as<int>(), d_nodepos.as<int4>(),
-					d_parts.as<int4>(), d_Lnode.as<double>(), d_out.as<double>(), phi_self, ewald);
-			SCHK(hipGetLastError());
-		} else {
-			hipLaunchKernelGGL(k_local<LOCAL_LPG>, dim3(unsigned((n_leaf_items + LOCAL_LPG - 1) / LOCAL_LPG)), dim3(64), local_lds_bytes(maxdepth),
-					s, d_leaf_items.as<int4>(), n_leaf_items, d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(),
-					d_dL.as<double>(), d_out.as<double>(), phi_self, maxdepth, ewald);
-			SCHK(hipGetLastError());
-		}
+	}
+	if (n > 0) {
+		// a kernel, not a memset, so the launch sequence stays graph-safe
+		hipLaunchKernelGGL(k_zero_f64, dim3(unsigned((4 * n + 255) / 256)), dim3(256), 0, s, d_out.as<double>(), int64_t(4 * n));
+		SCHK(hipGetLastError());
 	}
 	SCHK(mark(e + 1));
 	LeafArgs la{};
@@ -678,6 +699,24 @@ int Solver::enqueue_solve(bool timed) {
 			SCHK(launch_pp_ewald(la, s));
 		SCHK(mark(2 * k + 1));
 	}
+	if (fork) {
+		SCHK(hipEventRecord(ev_join, stream2));
+		SCHK(hipStreamWaitEvent(s, ev_join, 0));
+	}
+	SCHK(mark(e + 2));
+	if (local_pass) {
+		const double phi_self = (-315.0 / 128.0) * m / soft_len;
+		const int maxdepth = tree.max_level + 1;
+		if (local_sweep)
+			hipLaunchKernelGGL(k_l2p_parts, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, n, d_pleaf.as<int>(), d_nodepos.as<int4>(),
+					d_parts.as<int4>(), d_Lnode.as<double>(), d_out.as<double>(), phi_self, ewald);
+		else
+			hipLaunchKernelGGL(k_local<LOCAL_LPG>, dim3(unsigned((n_leaf_items + LOCAL_LPG - 1) / LOCAL_LPG)), dim3(64), local_lds_bytes(maxdepth),
+					s, d_leaf_items.as<int4>(), n_leaf_items, d_path.as<int>(), d_level.as<int>(), d_nodepos.as<int4>(), d_parts.as<int4>(),
+					d_dL.as<double>(), d_out.as<double>(), phi_self, maxdepth, ewald);
+		SCHK(hipGetLastError());
+	}
+	SCHK(mark(e + 3));
 	return TG_OK;
 }
 
@@ -685,19 +724,23 @@ int Solver::enqueue_solve(bool timed) {
 int Solver::kernel_times(std::vector<float>& ms) {
 	ms.assign(TIMER_SPANS, 0.0f);
 	if (solves_recorded == 0) return TG_OK;
-	const hipEvent_t* evs = ev.data() + size_t(2) * TIMER_SPANS * size_t((solves_recorded - 1) % TIMER_RING);
-	SCHK(hipEventSynchronize(evs[2 * TIMER_SPANS - 1]));
-	for (int k = 0; k < TIMER_SPANS; k++) SCHK(hipEventElapsedTime(&ms[k], evs[2 * k], evs[2 * k + 1]));
+	const hipEvent_t* evs = ev.data() + size_t(2) * TIMER_EVSPANS * size_t((solves_recorded - 1) % TIMER_RING);
+	SCHK(hipEventSynchronize(evs[2 * TIMER_EVSPANS - 1]));
+	for (int k = 0; k < TIMER_EVSPANS; k++) {
+		float t = 0.0f;
+		SCHK(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]));
+		ms[std::min(k, TIMER_SPANS - 1)] += t;  // the two local spans report as one
+	}
 	return TG_OK;
 }
 
 int Solver::fold_set(int64_t solve) {
-	const hipEvent_t* evs = ev.data() + size_t(2) * TIMER_SPANS * size_t(solve % TIMER_RING);
-	SCHK(hipEventSynchronize(evs[2 * TIMER_SPANS - 1]));
-	for (int k = 0; k < TIMER_SPANS; k++) {
+	const hipEvent_t* evs = ev.data() + size_t(2) * TIMER_EVSPANS * size_t(solve % TIMER_RING);
+	SCHK(hipEventSynchronize(evs[2 * TIMER_EVSPANS - 1]));
+	for (int k = 0; k < TIMER_EVSPANS; k++) {
 		float t = 0.0f;
 		SCHK(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]));
-		acc_ms[k] += t;
+		acc_ms[size_t(std::min(k, TIMER_SPANS - 1))] += t;  // the two local spans report as one
 	}
 	return TG_OK;
 }
@@ -790,6 +833,8 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 	} else if (k == "resident") {
 		if (value < 1) return TG_EINVAL;
 		S.resident_groups = value;
+	} else if (k == "overlap") {
+		S.overlap = value != 0;
 	} else if (k == "local_sweep") {
 		S.local_sweep = value != 0;
 	} else if (k == "gpu_walk") {

@@ -41,6 +41,9 @@ struct Solver {
 	bool uniform_mass = true;
 	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64); applies from the next set_particles
 	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
+	bool overlap = true;         // graph replays run the node kinds on a second stream beside the leaf kinds
+	hipStream_t stream2 = nullptr;
+	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 	bool local_sweep = false;    // local pass level by level (k_l2l_level) instead of per-leaf path sums (k_local)
 	bool gpu_walk = false;       // build the interaction lists on the device (device_walk); on for device solvers
 	int resident_groups = 1024;  // work groups of the persistent form (set from the CU count)
@@ -83,8 +86,9 @@ struct Solver {
 	double t_tree = 0, t_multipoles = 0, t_walk = 0, t_upload = 0;
 	double t_detail[6] = {};  // walk: set-up, steps, assembly, census; upload: preparation, transfers
 	// per-solve event sets: a ring of TIMER_RING solves, each 2 events per timed span (8 kinds,
-	// then the local-expansion pass); sets are folded into acc_ms before they are reused
-	static constexpr int TIMER_RING = 32, TIMER_SPANS = NKINDS + 1;
+	// then the local-expansion pass in two spans: node sums and L2L before the leaf kinds, L2P
+	// after them); sets are folded into acc_ms (8 kinds, local) before they are reused
+	static constexpr int TIMER_RING = 32, TIMER_SPANS = NKINDS + 1, TIMER_EVSPANS = NKINDS + 2;
 	std::vector<hipEvent_t> ev;
 	int64_t solves_recorded = 0, solves_folded = 0;
 	std::vector<double> acc_ms = std::vector<double>(TIMER_SPANS, 0.0);
