@@ -240,3 +240,29 @@ def test_rebuild_is_bit_identical():
         out.append(s.forces())
         s.close()
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_autotune_picks_the_fastest_tried_configuration():
+    """The autotuner times graph replays of every configuration it tries and returns the fastest;
+    its choice applied to a fresh solver gives the same forces as the eager solve of that
+    configuration."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.autotune import apply, autotune
+    from tigergrav_amd.solver import Solver
+
+    x = ics.uniform(6000, 81)
+    n = x.shape[0]
+    h, m = ics.soft_len(n), 1.0 / n
+    print(f"autotune: N={n}", flush=True)
+    best, table = autotune(x, True, h, m, 0.7, leaves=(32, 64), chunks=(64,), solves=3)
+    assert len(table) == 2 * 1 * 2 * 2
+    assert all(r["ms"] > 0 for r in table) and best["ms"] == min(r["ms"] for r in table)
+    s = Solver(ewald=True, soft_len=h, m=m, theta=0.7, parts_per_node=best["leaf"], device=0)
+    apply(s, best)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    eager = s.forces()
+    s.solve_graph()
+    s.synchronize()
+    np.testing.assert_array_equal(s.forces(), eager)
