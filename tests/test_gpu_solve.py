@@ -24,6 +24,18 @@ CASES = {
 }
 
 
+EDGE_CASES = {
+    # name: (generator, ewald, theta, parts_per_node): one particle (zero force by symmetry), two,
+    # a single leaf, a ragged count with leaves of 16, and 40 coincident particles (r = 0 pairs)
+    "n1_periodic": (lambda ics: ics.uniform(1, 91), True, 0.7, 32),
+    "n2_open": (lambda ics: ics.uniform(2, 92), False, 0.7, 32),
+    "n63_periodic_one_leaf": (lambda ics: ics.uniform(63, 93), True, 0.7, 64),
+    "ragged_1001_periodic": (lambda ics: ics.uniform(1001, 94), True, 0.5, 16),
+    "coincident_cluster": (lambda ics: np.concatenate([np.repeat(ics.uniform(1, 95), 40, axis=0), ics.uniform(2000, 96)]),
+                           True, 0.7, 64),
+}
+
+
 def rms_rel(a, b):
     return math.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2))
 
@@ -57,6 +69,36 @@ def test_solve_parity(case):
     np.testing.assert_array_equal(s.forces(), f)
     ms = s.kernel_ms()
     assert all(v >= 0 for v in ms.values())
+
+
+@pytest.mark.parametrize("case", list(EDGE_CASES))
+def test_solve_edge_cases(case):
+    """Small, ragged and coincident particle sets against the oracle solve on the same lists:
+    every component within 2e-5 of the case's largest (a zero force stays zero)."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    gen, ewald, theta, ppn = EDGE_CASES[case]
+    x = np.ascontiguousarray(gen(ics))
+    N = x.shape[0]
+    h, m = ics.soft_len(N), 1.0 / N
+    print(f"edge case {case}: N={N} ewald={ewald} theta={theta} leaf={ppn}", flush=True)
+    s = Solver(ewald=ewald, soft_len=h, m=m, theta=theta, parts_per_node=ppn, device=0)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    f = s.forces()
+    T = s.tree_arrays()
+    ref_tree, _ = ob.oracle_solve({"ewald": int(ewald), "soft_len": h, "m": m}, T)
+    ref = np.empty_like(ref_tree)
+    ref[T["order"]] = ref_tree
+    gs = max(np.abs(ref[:, 1:]).max(), 1e-30)
+    ps = np.abs(ref[:, 0]).max()
+    assert np.abs(f[:, 1:] - ref[:, 1:]).max() <= 2e-5 * gs or np.abs(f[:, 1:]).max() == 0.0 == np.abs(ref[:, 1:]).max()
+    assert np.abs(f[:, 0] - ref[:, 0]).max() <= 2e-5 * ps
+    s.solve_graph()
+    s.synchronize()
+    np.testing.assert_array_equal(s.forces(), f)
 
 
 def test_solve_accuracy_vs_fp64():
