@@ -489,19 +489,29 @@ static uint64_t range_total(const hvec<int32_t>& v, int64_t b, int64_t e) {
 // sinks x sources
 void Tree::count_census() {
 	const Lists& Ls = lists;
-	Census C;
-	const size_t nn = Ls.cc_d_ptr.empty() ? 0 : Ls.cc_d_ptr.size() - 1;
-	for (size_t i = 0; i < nn; i++) {
-		const uint64_t ns = uint64_t(nodes[i].pend - nodes[i].pbegin);
-		C.CC_direct += uint64_t(Ls.cc_d_ptr[i + 1] - Ls.cc_d_ptr[i]);
-		C.CC_ewald += uint64_t(Ls.cc_e_ptr[i + 1] - Ls.cc_e_ptr[i]);
-		C.CP_direct += range_total(Ls.cp_d, Ls.cp_d_ptr[i], Ls.cp_d_ptr[i + 1]);
-		C.CP_ewald += range_total(Ls.cp_e, Ls.cp_e_ptr[i], Ls.cp_e_ptr[i + 1]);
-		C.PC_direct += ns * uint64_t(Ls.pc_d_ptr[i + 1] - Ls.pc_d_ptr[i]);
-		C.PC_ewald += ns * uint64_t(Ls.pc_e_ptr[i + 1] - Ls.pc_e_ptr[i]);
-		C.PP_direct += ns * range_total(Ls.pp_d, Ls.pp_d_ptr[i], Ls.pp_d_ptr[i + 1]);
-		C.PP_ewald += ns * range_total(Ls.pp_e, Ls.pp_e_ptr[i], Ls.pp_e_ptr[i + 1]);
+	const int64_t nn = Ls.cc_d_ptr.empty() ? 0 : int64_t(Ls.cc_d_ptr.size()) - 1;
+	uint64_t ccd = 0, cce = 0, cpd = 0, cpe = 0, pcd = 0, pce = 0, ppd = 0, ppe = 0;  // integer sums: any order
+#pragma omp parallel for schedule(static) reduction(+ : ccd, cce, cpd, cpe, pcd, pce, ppd, ppe)
+	for (int64_t i = 0; i < nn; i++) {
+		const uint64_t ns = uint64_t(nodes[size_t(i)].pend - nodes[size_t(i)].pbegin);
+		ccd += uint64_t(Ls.cc_d_ptr[size_t(i) + 1] - Ls.cc_d_ptr[size_t(i)]);
+		cce += uint64_t(Ls.cc_e_ptr[size_t(i) + 1] - Ls.cc_e_ptr[size_t(i)]);
+		cpd += range_total(Ls.cp_d, Ls.cp_d_ptr[size_t(i)], Ls.cp_d_ptr[size_t(i) + 1]);
+		cpe += range_total(Ls.cp_e, Ls.cp_e_ptr[size_t(i)], Ls.cp_e_ptr[size_t(i) + 1]);
+		pcd += ns * uint64_t(Ls.pc_d_ptr[size_t(i) + 1] - Ls.pc_d_ptr[size_t(i)]);
+		pce += ns * uint64_t(Ls.pc_e_ptr[size_t(i) + 1] - Ls.pc_e_ptr[size_t(i)]);
+		ppd += ns * range_total(Ls.pp_d, Ls.pp_d_ptr[size_t(i)], Ls.pp_d_ptr[size_t(i) + 1]);
+		ppe += ns * range_total(Ls.pp_e, Ls.pp_e_ptr[size_t(i)], Ls.pp_e_ptr[size_t(i) + 1]);
 	}
+	Census C;
+	C.CC_direct = ccd;
+	C.CC_ewald = cce;
+	C.CP_direct = cpd;
+	C.CP_ewald = cpe;
+	C.PC_direct = pcd;
+	C.PC_ewald = pce;
+	C.PP_direct = ppd;
+	C.PP_ewald = ppe;
 	census = C;
 }
 

@@ -8,8 +8,26 @@
 #include <cstring>
 
 #include "tg_gravity.h"
+#include "tg_walk.hpp"
 
 namespace tg {
+
+void walk_nodes(const Tree& T, WalkNode* out) {
+	const std::vector<float> span = T.sink_spans();
+	const int64_t nn = int64_t(T.nodes.size());
+#pragma omp parallel for schedule(static)
+	for (int64_t i = 0; i < nn; i++) {
+		const Node& nd = T.nodes[size_t(i)];
+		WalkNode& w = out[i];
+		w.pbegin = nd.pbegin;
+		w.pend = nd.pend;
+		w.c0 = nd.child[0];
+		w.c1 = nd.child[1];
+		for (int d = 0; d < 3; d++) w.x[d] = nd.x[d];
+		w.r = nd.r;
+		w.rmax = span[size_t(i)];
+	}
+}
 
 int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err) {
 	const int64_t nn = int64_t(T.nodes.size());

@@ -32,13 +32,6 @@
 namespace tg {
 namespace {
 
-struct WalkNode {
-	int32_t pbegin, pend, c0, c1;
-	int32_t x[3];
-	float r;
-	float rmax;  // span of the node's particle box (the sink rmax of the Ewald self test)
-};
-
 enum { MODE_FIRST = 0, MODE_DRAIN_D = 1, MODE_DRAIN_E = 2 };
 
 struct WalkUnit {
@@ -407,19 +400,8 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	} while (0)
 	WCHK(hipSetDevice(device));
 	{
-		const std::vector<float> span = T.sink_spans();
-		std::vector<WalkNode> wn(size_t(std::max<int64_t>(nn, 1)));
-		for (int64_t i = 0; i < nn; i++) {
-			const Node& nd = T.nodes[size_t(i)];
-			WalkNode& w = wn[size_t(i)];
-			w.pbegin = nd.pbegin;
-			w.pend = nd.pend;
-			w.c0 = nd.child[0];
-			w.c1 = nd.child[1];
-			for (int d = 0; d < 3; d++) w.x[d] = nd.x[d];
-			w.r = nd.r;
-			w.rmax = span[size_t(i)];
-		}
+		hvec<WalkNode> wn(size_t(std::max<int64_t>(nn, 1)));
+		walk_nodes(T, wn.data());
 		WCHK(hipMalloc(&dev_nodes, wn.size() * sizeof(WalkNode)));
 		WCHK(hipMemcpyAsync(dev_nodes, wn.data(), wn.size() * sizeof(WalkNode), hipMemcpyHostToDevice, s));
 	}

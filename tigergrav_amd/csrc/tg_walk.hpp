@@ -8,6 +8,17 @@
 
 namespace tg {
 
+// a node as the device walk reads it
+struct WalkNode {
+	int32_t pbegin, pend, c0, c1;
+	int32_t x[3];
+	float r;
+	float rmax;  // span of the node's particle box (the sink rmax of the Ewald self test)
+};
+
+// fills out[i] for every node of T (host, OpenMP; tg_upload.cpp)
+void walk_nodes(const Tree& T, WalkNode* out);
+
 // Tree::walk on the device: fills T.lists and T.census with exactly the lists the host walk
 // builds (same entries, same order).  Returns hipSuccess or the first HIP error; err names it.
 // t, if given, receives the seconds spent in set-up, the walk steps, list assembly with the copy
