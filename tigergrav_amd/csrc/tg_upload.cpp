@@ -4,6 +4,7 @@
 #include "tg_upload.hpp"
 
 #include <algorithm>
+#include <parallel/algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -99,25 +100,62 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 		}
 	hvec<int64_t> oent(size_t(4 * nn)), orun(size_t(4 * nn)), ochk(size_t(4 * nn)), oslot(size_t(4 * nn));
 	int64_t trun[4] = {}, tchk[4] = {}, slot = 0;
-	for (int64_t nd = 0; nd < nn; nd++) {
-		for (int q = 0; q < 4; q++) {
-			const KindSrc& k = nsrc[q];
-			const size_t i = size_t(4 * nd + q);
-			if (!k.ranges && (*k.ptr)[nd] != list_len[k.kind]) {
-				err = "node interaction list not in node order";
-				return TG_EINVAL;
-			}
-			const int64_t nch = (nent[i] + chunk - 1) / chunk;
-			oent[i] = list_len[k.kind];
-			list_len[k.kind] += nent[i];
-			orun[i] = trun[q];
-			trun[q] += nrun[i];
-			ochk[i] = tchk[q];
-			tchk[q] += nch;
-			oslot[i] = slot;
-			slot += nch;
+	{
+		// each node's offsets in node order (entries, runs, chunks per kind; slots over all kinds):
+		// per-block totals in parallel, their scan, then every block's running offsets from its start
+		constexpr int NC = 13;  // entries[4], runs[4], chunks[4], slots
+		const int64_t NB = std::min<int64_t>(nn, 1024);
+		auto first = [&](int64_t b) { return nn * b / NB; };
+		std::vector<int64_t> base(size_t(NB + 1) * NC, 0);
+#pragma omp parallel for schedule(static)
+		for (int64_t b = 0; b < NB; b++) {
+			int64_t* S = base.data() + NC * (b + 1);
+			for (int64_t nd = first(b); nd < first(b + 1); nd++)
+				for (int q = 0; q < 4; q++) {
+					const size_t i = size_t(4 * nd + q);
+					const int64_t nch = (nent[i] + chunk - 1) / chunk;
+					S[q] += nent[i];
+					S[4 + q] += nrun[i];
+					S[8 + q] += nch;
+					S[12] += nch;
+				}
 		}
-		slot_ptr[nd + 1] = slot;
+		for (int64_t b = 1; b <= NB; b++)
+			for (int c = 0; c < NC; c++) base[size_t(NC * b + c)] += base[size_t(NC * (b - 1) + c)];
+		int in_order = 1;
+#pragma omp parallel for schedule(static) reduction(min : in_order)
+		for (int64_t b = 0; b < NB; b++) {
+			int64_t R[NC];
+			for (int c = 0; c < NC; c++) R[c] = base[size_t(NC * b + c)];
+			for (int64_t nd = first(b); nd < first(b + 1); nd++) {
+				for (int q = 0; q < 4; q++) {
+					const KindSrc& k = nsrc[q];
+					const size_t i = size_t(4 * nd + q);
+					if (!k.ranges && (*k.ptr)[nd] != R[q]) in_order = 0;
+					const int64_t nch = (nent[i] + chunk - 1) / chunk;
+					oent[i] = R[q];
+					R[q] += nent[i];
+					orun[i] = R[4 + q];
+					R[4 + q] += nrun[i];
+					ochk[i] = R[8 + q];
+					R[8 + q] += nch;
+					oslot[i] = R[12];
+					R[12] += nch;
+				}
+				slot_ptr[nd + 1] = R[12];
+			}
+		}
+		if (!in_order) {
+			err = "node interaction list not in node order";
+			return TG_EINVAL;
+		}
+		const int64_t* tot = base.data() + NC * NB;
+		for (int q = 0; q < 4; q++) {
+			list_len[nsrc[q].kind] += tot[q];
+			trun[q] = tot[4 + q];
+			tchk[q] = tot[8 + q];
+		}
+		slot = tot[12];
 	}
 	for (int q = 0; q < 4; q++) {
 		runs[nsrc[q].kind].resize(size_t(trun[q]));
@@ -310,11 +348,11 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 		}
 	}
 	// heaviest leaf items first: the dispatcher hands out work groups in order
+	// (a stable sort: the result is the unique stable order, however the threads merge)
 	const int leaf_kinds[4] = {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD};
-#pragma omp parallel for schedule(dynamic, 1)
 	for (int q = 0; q < 4; q++) {
 		const int k = leaf_kinds[q];
-		std::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
+		__gnu_parallel::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
 			const int na = a.y & PP_COUNT_MASK, nb = b.y & PP_COUNT_MASK;
 			const int64_t ca = int64_t(a.w - a.z) * (1 << (na <= 1 ? 0 : 32 - __builtin_clz(na - 1)));
 			const int64_t cb = int64_t(b.w - b.z) * (1 << (nb <= 1 ? 0 : 32 - __builtin_clz(nb - 1)));
