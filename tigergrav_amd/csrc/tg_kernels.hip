@@ -24,16 +24,17 @@ namespace tg {
 __constant__ const float* c_ewt;
 
 // ---- the tabulated correction -------------------------------------------------------------
+static std::mutex table_mtx;                     // guards device_table and the per-device set-up
 static std::vector<const float*> device_table;  // the tabulated correction on each device
 
 const float* ewald_table_on(int device) {
+	std::lock_guard<std::mutex> lock(table_mtx);
 	return device >= 0 && device < (int) device_table.size() ? device_table[size_t(device)] : nullptr;
 }
 
 hipError_t init_device_tables(int device) {
-	static std::mutex mtx;
 	static std::vector<int> done;
-	std::lock_guard<std::mutex> lock(mtx);
+	std::lock_guard<std::mutex> lock(table_mtx);
 	if ((int) done.size() <= device) done.resize(device + 1, 0);
 	if (done[device]) return hipSuccess;
 	hipError_t e;
