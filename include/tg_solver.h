@@ -70,7 +70,7 @@ int tg_solver_solve(tg_solver* s);
  * leaf kinds (the same results bit for bit), 0 keeps one stream */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);
 /* the same solve replayed from a HIP graph captured on its first call after set_particles (one
- * submission instead of nine launches; records no per-kind timings; with "overlap" the node and
+ * submission instead of 11 or more launches; records no per-kind timings; with "overlap" the node and
  * leaf kinds run concurrently) */
 int tg_solver_solve_graph(tg_solver* s);
 int tg_solver_synchronize(tg_solver* s);
