@@ -33,6 +33,9 @@ EDGE_CASES = {
     "ragged_1001_periodic": (lambda ics: ics.uniform(1001, 94), True, 0.5, 16),
     "coincident_cluster": (lambda ics: np.concatenate([np.repeat(ics.uniform(1, 95), 40, axis=0), ics.uniform(2000, 96)]),
                            True, 0.7, 64),
+    # 100 coincident particles with leaves of 32: one 100-particle leaf at the depth limit
+    "coincident_beyond_leaf": (lambda ics: np.concatenate([np.repeat(ics.uniform(1, 97), 100, axis=0), ics.uniform(3000, 98)]),
+                               True, 0.7, 32),
 }
 
 

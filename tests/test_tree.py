@@ -138,6 +138,20 @@ def test_lists_partition_all_particles(ewald, gen, theta):
         assert info["PP_ewald"] == info["CC_ewald"] == info["PC_ewald"] == info["CP_ewald"] == 0
 
 
+def test_coincident_particles_beyond_leaf_size_stop_at_the_depth_limit():
+    """More coincident particles than a leaf holds cannot be separated: the build stops splitting
+    at level 60 and keeps them in one leaf, and every sink's lists still cover all particles once."""
+    from tigergrav_amd import ics
+
+    x = np.concatenate([np.repeat(ics.uniform(1, 5), 100, axis=0), ics.uniform(3000, 6)])
+    s = make_solver(x, ewald=True, ppn=32)
+    T = s.tree_arrays()
+    r, lv = T["node_range"], T["node_level"]
+    big = [lf for lf in T["leaves"] if r[lf][1] - r[lf][0] > 32]
+    assert len(big) == 1 and r[big[0]][1] - r[big[0]][0] == 100 and lv[big[0]] == 60 and lv.max() == 60
+    assert not check_partition(T, kinds=("d", "e"))
+
+
 def test_tiny_theta_is_all_pp():
     from tigergrav_amd import ics
 

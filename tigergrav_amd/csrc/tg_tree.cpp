@@ -91,7 +91,7 @@ void Tree::build(const int32_t* x, const float* masses, int64_t n, const TreePar
 		std::vector<int32_t> first(le - lb, -1);
 		size_t nc = le;
 		for (size_t i = lb; i < le; i++)
-			if (nodes[i].pend - nodes[i].pbegin > P.parts_per_node) {
+			if (nodes[i].pend - nodes[i].pbegin > P.parts_per_node && nodes[i].level < MAX_SPLIT_LEVEL) {
 				first[i - lb] = int32_t(nc);
 				nc += 2;
 			}

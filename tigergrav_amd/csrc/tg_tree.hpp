@@ -84,6 +84,12 @@ struct Census {
 	uint64_t PP_ewald = 0, PC_ewald = 0, CC_ewald = 0, CP_ewald = 0;
 };
 
+// Nodes at this level are not split further.  A box there is 2^-20 of the unit cell per side, so only
+// particles that (nearly) coincide reach it: more than parts_per_node of them at one position would
+// otherwise be split for ever (every split leaves them all on one side), in the reference too.  They
+// stay one leaf; the kernels take its sinks 64 at a time.
+constexpr int MAX_SPLIT_LEVEL = 60;
+
 class Tree {
 public:
 	TreeParams P;
