@@ -104,6 +104,27 @@ def test_solve_edge_cases(case):
     np.testing.assert_array_equal(s.forces(), f)
 
 
+def test_empty_particle_set():
+    """No particles: the solve, its graph form and the force readback all succeed and return nothing;
+    particles set afterwards on the same solver solve as usual."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    print("empty particle set", flush=True)
+    s = Solver(ewald=True, soft_len=1e-3, m=1.0, theta=0.7, parts_per_node=32, device=0)
+    s.set_particles(np.zeros((0, 3), np.int32))
+    s.solve()
+    s.solve_graph()
+    s.synchronize()
+    assert s.forces().shape == (0, 4)
+    x = ics.uniform(1000, 99)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    f = s.forces()
+    assert f.shape == (1000, 4) and np.all(np.isfinite(f))
+
+
 def test_solve_accuracy_vs_fp64():
     """Force error of the GPU solve against the FP64 periodic direct sum matches the expansion
     error of the reference algorithm (the oracle solve on the same lists)."""

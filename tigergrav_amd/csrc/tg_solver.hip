@@ -243,6 +243,7 @@ void Solver::drop_graph() {
 // solve reads is written earlier in the same solve.
 int Solver::solve_graph() {
 	SCHK(hipSetDevice(device));
+	if (n == 0) return TG_OK;  // no particles: nothing to launch, and no graph to capture
 	if (!graph_exec) {
 		SCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
 		const int rc = enqueue_solve(false);
