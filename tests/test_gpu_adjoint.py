@@ -142,3 +142,21 @@ def test_torch_autograd_function():
     pred = float(np.sum(pos.grad.numpy() * d))
     print(f"torch directional: fd {0.5 * (lp - lm):.6e} adjoint {pred:.6e}", flush=True)
     assert abs(0.5 * (lp - lm) - pred) <= 3e-3 * abs(pred) + 2e-5
+
+
+def test_adjoint_rejects_a_leaf_beyond_64_particles():
+    """A leaf at the tree's depth limit can hold more than 64 coincident particles; the adjoint,
+    which takes a leaf's sinks in one 64-lane group, refuses it with an error instead of leaving
+    the extra particles out."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    x = np.concatenate([np.repeat(ics.uniform(1, 61), 80, axis=0), ics.uniform(3000, 62)])
+    n = x.shape[0]
+    print(f"adjoint oversized leaf: N={n}", flush=True)
+    s = Solver(ewald=True, soft_len=ics.soft_len(n), m=1.0 / n, theta=0.7, parts_per_node=32, device=0)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    with pytest.raises(Exception, match="64 particles"):
+        s.adjoint(np.ones((n, 3), np.float32))

@@ -804,6 +804,12 @@ int Solver::adjoint(const float* w_in, double* grad_out, float* ms) {
 		err = "the adjoint handles leaves of at most 64 particles";
 		return TG_EINVAL;
 	}
+	for (int32_t lf : tree.leaves)
+		if (tree.nodes[size_t(lf)].pend - tree.nodes[size_t(lf)].pbegin > 64) {
+			// only more coincident particles than a leaf holds make one (the tree's depth limit)
+			err = "the adjoint handles leaves of at most 64 particles; a leaf at the tree's depth limit holds more";
+			return TG_EINVAL;
+		}
 	if (!adj_pp_symmetric) {
 		err = "the adjoint needs symmetric PP lists";
 		return TG_EINVAL;
