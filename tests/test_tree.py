@@ -314,3 +314,17 @@ def test_update_particles_keeps_tree_and_recomputes_moments():
         for lo, hi in ((0, 1), (1, 7), (7, 17)):  # each order against its own magnitude
             scale = np.abs(want[lo:hi]).max() + 1e-30
             assert np.max(np.abs(M[nd, lo:hi] - want[lo:hi])) <= 2e-5 * scale, (nd, lo, M[nd], want)
+
+
+def test_stage_times_are_reported():
+    """tg_solve_info splits the host build into stages; a host-only solver reports its tree,
+    multipole, walk and work-array preparation times (the device stages stay zero)."""
+    from tigergrav_amd import ics
+
+    s = make_solver(ics.uniform(5000, 17), ewald=True, ppn=32)
+    i = s.info()
+    assert i["t_tree"] > 0 and i["t_multipoles"] > 0 and i["t_walk"] > 0 and i["t_upload"] > 0
+    d = i["t_detail"]
+    assert set(d) == {"walk_setup", "walk_steps", "walk_assembly", "census", "prepare", "transfer"}
+    assert d["prepare"] > 0 and d["prepare"] <= i["t_upload"]
+    assert d["walk_steps"] == 0 and d["transfer"] == 0
