@@ -145,6 +145,31 @@ def device_free_bytes():
     return int(free.value)
 
 
+def target_leg(x, ewald, h, m, sinks, fref, theta=0.3, leaf=32, steps=5):
+    """The same particles at the opening angle whose force error meets RMS |da|/|a| <= 1e-3 on this
+    workload: graph-replay time per solve, interactions/s and the error on the accuracy sample."""
+    from tigergrav_amd.accuracy import error_stats
+    from tigergrav_amd.solver import Solver
+
+    s = Solver(ewald=ewald, soft_len=h, m=m, theta=theta, parts_per_node=leaf, device=0)
+    s.tune("local_sweep", 1)
+    s.set_particles(x)
+    for _ in range(2):
+        s.solve_graph()
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.solve_graph()
+    s.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    total = sum(s.info()["count"].values())
+    acc = error_stats(s.forces()[sinks], fref)
+    s.close()
+    return {"theta": theta, "leaf": leaf, "steps": steps, "ms_per_solve": ms, "interactions_per_s": total / (ms * 1e-3),
+            "rms": acc["rms"], "p99": acc["p99"], "meets_1e-3": bool(acc["rms"] <= 1e-3),
+            "sample": f"the accuracy sample above ({len(sinks)} sinks), same FP64 reference"}
+
+
 def adjoint_leg(s, x, ewald, h, m, forward_ms, steps):
     """The adjoint of the solve (tg_solver_adjoint) on the same particle set: warm device time per
     call against the forward's, a directional central difference of the frozen-tree solve, and the
@@ -203,6 +228,8 @@ def main():
                     help="build the interaction lists with the host walk instead of the device walk")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager solves (with per-kind event timers) instead of HIP-graph replays")
+    ap.add_argument("--no-accuracy-target", action="store_true",
+                    help="skip the theta 0.3 leg (the setting that meets RMS 1e-3 on the 2M workload)")
     ap.add_argument("--accuracy-sample", type=int, default=10000,
                     help="sinks checked against the FP64 direct sum (all of them if >= N)")
     ap.add_argument("--adjoint", action="store_true",
@@ -362,6 +389,9 @@ def main():
         fp["forward_plus_adjoint_bytes"] = free0 - device_free_bytes()
     fp["method"] = "hipMemGetInfo difference from before the solver was created (the accuracy checks free their buffers)"
     out["hbm_footprint"] = fp
+    if not args.no_accuracy_target and args.workload == "zeldovich128" and args.theta > 0.3 + 1e-9:
+        # this workload meets RMS 1e-3 only at theta 0.3: that operating point, timed the same way
+        out["accuracy_target"] = target_leg(x, ewald, h, m, sinks, fref)
     if not args.no_cpu:
         out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
         if f_cpu is not None:
