@@ -53,3 +53,29 @@ def test_solve_over_device_lists_is_bit_identical():
         s.solve()
         s.synchronize()
     assert np.array_equal(a.forces(), b.forces())
+
+
+# The device walk's own lists at the benchmark's size, checked against the reference's opening
+# criterion by the FP64 verifier (tests/mac_verify.py, which builds no lists) and for exact
+# coverage of every sink: the 128^3 Zel'dovich set at leaf 32, theta 0.7 (every entry) and theta
+# 0.3 (the entries of 3000 seeded sink leaves and their ancestors; coverage on 1000 of them).
+@pytest.mark.parametrize("theta,sample", [(0.7, None), (0.3, 3000)])
+def test_device_walk_lists_pass_mac_verifier_2m(theta, sample):
+    from mac_verify import MacVerifier, coverage_exact
+
+    x = ics.zeldovich(128, 3)
+    n = x.shape[0]
+    print(f"device walk 2M zeldovich theta={theta} leaf=32: verifying", flush=True)
+    s = _build(x, True, theta, 32, 0, True)
+    T = s.tree_arrays()
+    for name in LIST_NAMES:
+        T[name] = s.export(name)
+        T[name + "_ptr"] = s.export(name + "_ptr")
+    leaves = T["leaves"]
+    rng = np.random.default_rng(5)
+    sinks = None if sample is None else rng.choice(leaves, sample, replace=False)
+    bad = MacVerifier(T, theta, ics.soft_len(n), True).check(sink_leaves=sinks)
+    assert all(v == 0 for v in bad.values()), bad
+    cov = rng.choice(leaves, 1000 if sample else 4000, replace=False)
+    assert coverage_exact(T, cov, "d") == [] and coverage_exact(T, cov, "e") == []
+    assert sum(len(T[k]) for k in LIST_NAMES) > 0
