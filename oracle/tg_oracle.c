@@ -1114,11 +1114,31 @@ static tgo_multi_src* gather_multi(const int32_t* node_x, const float* node_m, c
 
 /* kinds: 0 cc_d, 1 cp_d, 2 cc_e, 3 cp_e, 4 pc_d, 5 pp_d, 6 pc_e, 7 pp_e.  Nodes are ordered with
  * parents before children.  With stride > 1 only every stride-th node's node-kind work and every
- * stride-th leaf's leaf-kind work is done (a timing sample; results are then partial).
+ * stride-th leaf's leaf-kind work is done (a timing sample; results are then partial).  With want
+ * != NULL only nodes with want[n] != 0 are evaluated (their L2L, node kinds and, for leaves, leaf
+ * kinds); marking a set of leaves and all their ancestors gives those leaves' exact results.
  * Returns the reference-weighted flop count. */
+static uint64_t solve_impl(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
+		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
+		const int64_t* const* ptr, const int32_t* const* lst, int stride, const uint8_t* want, tgo_force* out);
+
 uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
 		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
 		const int64_t* const* ptr, const int32_t* const* lst, int stride, tgo_force* out, uint64_t* counts) {
+	(void) counts;
+	return solve_impl(P, N, pos, nn, node_range, node_child, node_parent, node_x, node_m, ptr, lst, stride, NULL, out);
+}
+
+uint64_t tgo_solve_want(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
+		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
+		const int64_t* const* ptr, const int32_t* const* lst, const uint8_t* want, tgo_force* out) {
+	return solve_impl(P, N, pos, nn, node_range, node_child, node_parent, node_x, node_m, ptr, lst, 1, want, out);
+}
+
+static uint64_t solve_impl(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
+		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
+		const int64_t* const* ptr, const int32_t* const* lst, int stride, const uint8_t* want, tgo_force* out) {
+	(void) N;
 	double* L = calloc((size_t) nn * 35, sizeof(double));
 	uint64_t flop = 0, cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	/* node pass: levels are contiguous runs of nodes whose parents come earlier */
@@ -1130,6 +1150,7 @@ uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t n
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : flop)
 		for (int64_t n = start; n < end; n++) {
 			if (node_range[2 * n] == node_range[2 * n + 1]) continue;
+			if (want && !want[n]) continue;
 			double* Ln = L + 35 * n;
 			const int32_t p = node_parent[n];
 			if (p >= 0) {
@@ -1164,6 +1185,7 @@ uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t n
 		const int32_t pb = node_range[2 * n], pe = node_range[2 * n + 1];
 		if (pb == pe) continue;
 		if (stride > 1 && n % stride) continue;
+		if (want && !want[n]) continue;
 		const int64_t ns = pe - pb;
 		tgo_force* f = out + pb;
 		const int32_t* x = pos + 3 * (int64_t) pb;
@@ -1193,7 +1215,6 @@ uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t n
 		}
 	}
 	(void) cnt;
-	(void) counts;
 	free(L);
 	return flop;
 }

@@ -96,6 +96,11 @@ void tgo_l2p(const double* L, const double* dX, tgo_force* f);
 uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
 		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
 		const int64_t* const* ptr, const int32_t* const* lst, int stride, tgo_force* out, uint64_t* counts);
+/* the same solve restricted to the nodes with want[n] != 0 (a set of leaves and all their
+ * ancestors gives those leaves' exact results; the other particles' forces are left as they are) */
+uint64_t tgo_solve_want(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
+		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
+		const int64_t* const* ptr, const int32_t* const* lst, const uint8_t* want, tgo_force* out);
 #ifdef __cplusplus
 }
 #endif

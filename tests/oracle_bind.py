@@ -246,6 +246,32 @@ def oracle_solve(params, T, stride=1):
     return out, flop
 
 
+def oracle_solve_leaves(params, T, leaves):
+    """The oracle solve (tgo_solve_want) for the particles of `leaves` only: those leaves and every
+    ancestor are evaluated, nothing else.  Returns (forces in tree order, mask of evaluated particles)."""
+    L = lib()
+    L.tgo_solve_want.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [
+        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tgo_solve_want.restype = C.c_uint64
+    pos, nr, nc, npar, nx, nm, ptrs, lsts, keep = _solve_args(T)
+    want = np.zeros(nr.shape[0], np.uint8)
+    parent = np.asarray(T["node_parent"], np.int64)
+    a = np.unique(np.asarray(leaves, np.int64))
+    while len(a):
+        want[a] = 1
+        a = parent[a]
+        a = np.unique(a[a >= 0])
+    P = Params(int(params["ewald"]), float(params["soft_len"]), float(params["m"]))
+    out = np.zeros((pos.shape[0], 4), np.float64)
+    L.tgo_solve_want(C.byref(P), pos.shape[0], _p(pos), nr.shape[0], _p(nr), _p(nc), _p(npar), _p(nx), _p(nm), ptrs,
+                     lsts, _p(want), _p(out))
+    mask = np.zeros(pos.shape[0], bool)
+    for lf in np.unique(np.asarray(leaves, np.int64)):
+        b, e = nr[lf]
+        mask[b:e] = True
+    return out, mask
+
+
 def ref_solve(R, T, stride=1):
     """The same solve with the compiled reference's own L2L/L2P and kernels (tgref_solve)."""
     R.tgref_solve.argtypes = [C.c_int64, C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [C.c_void_p, C.c_void_p, C.c_int,
