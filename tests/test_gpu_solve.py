@@ -332,3 +332,58 @@ def test_autotune_picks_the_fastest_tried_configuration():
     s.solve_graph()
     s.synchronize()
     np.testing.assert_array_equal(s.forces(), eager)
+
+
+def test_update_moves_pair_inside_h_across_far_range():
+    """After update_particles, a particle moved to within h/4 of a particle of a leaf that its own
+    leaf lists as PP but classed far at the build (leaf boxes more than 2h apart) is still summed
+    with the softened pair: the frozen-tree solve equals the oracle solve (which softens every
+    listed pair) on the same lists, for both leaves' particles."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    n = 4096
+    x = ics.uniform(n, 41)
+    h, m = ics.soft_len(n), 1.0 / n
+    s = Solver(ewald=True, soft_len=h, m=m, theta=0.7, parts_per_node=32, device=0)
+    s.set_particles(x)
+    T = s.tree_arrays()
+    pos = T["pos"] * 2.0 ** -32 + 0.5
+    rng_, p, v = T["node_range"], s.export("pp_d_ptr"), s.export("pp_d").reshape(-1, 2)
+    box = {}
+    for lf in T["leaves"]:
+        b, e = rng_[lf]
+        if e > b:
+            box[b] = (pos[b:e].min(0), pos[b:e].max(0), lf)
+    pick = None
+    for A in T["leaves"]:
+        b, e = rng_[A]
+        if e <= b:
+            continue
+        for sb, se in v[p[A] // 2:p[A + 1] // 2]:
+            mnB, mxB, B = box[sb]
+            mnA, mxA = pos[b:e].min(0), pos[b:e].max(0)
+            gap = np.maximum(0.0, np.maximum(mnB - mxA, mnA - mxB))
+            if np.sqrt(np.sum(gap ** 2)) > 2 * h:
+                pick = (A, B, b, sb)
+                break
+        if pick:
+            break
+    assert pick is not None
+    A, B, i_tree, j_tree = pick
+    print(f"update: leaf {A} particle moved to h/4 of leaf {B}'s particle", flush=True)
+    order = T["order"]
+    x2 = x.copy()
+    off = np.array([int(h / 4 * 2 ** 32), 0, 0], np.int64)
+    x2[order[i_tree]] = (T["pos"][j_tree].astype(np.int64) + off).astype(np.int32)
+    s.update_particles(x2)
+    s.solve()
+    s.synchronize()
+    f = s.forces()
+    T2 = s.tree_arrays()
+    ref_tree, mask = ob.oracle_solve_leaves({"ewald": 1, "soft_len": h, "m": m}, T2, [A, B])
+    f_tree = f[T2["order"]]
+    eg = rms_rel(f_tree[mask, 1:], ref_tree[mask, 1:])
+    ei = np.linalg.norm(f_tree[i_tree, 1:] - ref_tree[i_tree, 1:]) / np.linalg.norm(ref_tree[i_tree, 1:])
+    print(f"  rms rel g over both leaves {eg:.2e}, moved particle {ei:.2e}", flush=True)
+    assert eg < 1e-5 and ei < 1e-5

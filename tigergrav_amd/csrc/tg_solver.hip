@@ -182,6 +182,12 @@ __global__ __launch_bounds__(64) void k_l2p_parts(int64_t n, const int* __restri
 	o[3] += g[2];
 }
 
+// PP-direct items: the near prefix extended to the whole list (softened pair on every tile)
+__global__ __launch_bounds__(256) void k_pp_near_all(int4* __restrict__ items, int n) {
+	const int i = int(blockIdx.x) * 256 + threadIdx.x;
+	if (i < n) items[i].y = pp_item_y(items[i].y & PP_COUNT_MASK, PP_NEAR_ALL);
+}
+
 __global__ __launch_bounds__(256) void k_zero_f64(double* __restrict__ p, int64_t n) {
 	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
 	if (i < n) p[i] = 0.0;
@@ -388,9 +394,22 @@ int Solver::update_particles(const int32_t* x, const float* mass) {
 		uniform_mass = false;
 	}
 	tree.update_moments(pt.data(), mass ? mt.data() : nullptr);
+	// The near prefix of each PP-direct item was classified from the leaf boxes of the build; moved
+	// particles may now come within h of a source range classed far.  Every PP-direct item of an
+	// updated tree therefore takes the softened pair on every tile (the adjoint reads the same
+	// flag), so the solve stays the spline-softened sum over every listed pair.
+	for (int4& it : host_pp_items) it.y = pp_item_y(it.y & PP_COUNT_MASK, PP_NEAR_ALL);
 	if (device < 0) return TG_OK;
 	SCHK(hipSetDevice(device));
-	return upload_sources();
+	const int rc = upload_sources();
+	if (rc) return rc;
+	KindWork& w = work[PP_DIRECT];
+	if (w.nitems > 0) {
+		hipLaunchKernelGGL(k_pp_near_all, dim3(unsigned((w.nitems + 255) / 256)), dim3(256), 0, stream, w.items.as<int4>(),
+				w.nitems);
+		SCHK(hipGetLastError());
+	}
+	return TG_OK;
 }
 
 // particle records and multipoles of the current tree to the device
