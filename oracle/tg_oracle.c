@@ -832,9 +832,14 @@ static inline void sep64(int ewald, const int32_t* X, const int32_t* Y, double* 
 	}
 }
 
+static void m2l64d(double* L, const double* M, const double* D);
 static void m2l64(double* L, const float* Mf, const double* D) {
 	double M[17];
 	for (int i = 0; i < 17; i++) M[i] = Mf[i];
+	m2l64d(L, M, D);
+}
+
+static void m2l64d(double* L, const double* M, const double* D) {
 	L[0] += M[0] * D[0];
 	for (int a = 0; a < 3; a++)
 		for (int b = a; b < 3; b++) {
@@ -864,9 +869,14 @@ static void m2l64(double* L, const float* Mf, const double* D) {
 				for (int d = c; d < 3; d++) L[E4(a, b, c, d)] += M[0] * D[E4(a, b, c, d)];
 }
 
+static void m2p64d(double* phi, double* g, const double* M, const double* D);
 static void m2p64(double* phi, double* g, const float* Mf, const double* D) {
 	double M[17];
 	for (int i = 0; i < 17; i++) M[i] = Mf[i];
+	m2p64d(phi, g, M, D);
+}
+
+static void m2p64d(double* phi, double* g, const double* M, const double* D) {
 	double f0 = M[0] * D[0];
 	for (int a = 0; a < 3; a++)
 		for (int b = a; b < 3; b++) f0 += M[M2(a, b)] * D[E2(a, b)] * EF64[E2(a, b)];
@@ -1217,4 +1227,159 @@ static uint64_t solve_impl(const tgo_params* P, int64_t N, const int32_t* pos, i
 	(void) cnt;
 	free(L);
 	return flop;
+}
+
+/* ========================================================================================== */
+/* FP64 frozen-tree solve: the function the batched solve evaluates on a kept tree (the adjoint's  */
+/* function), every step in FP64.  Moments about the kept centres node_x (no dipole slot, as the   */
+/* reference's multipole has none): leaves by P2M of their particles' offsets (wrap-around in      */
+/* periodic mode), other nodes by the raw-moment shift of their children (tigergrav/multipole.hpp: */
+/* 117-135); then the reference's solve sequence (src/tree.cpp:400-401,481-491,603-626) with the   */
+/* tgo64_ formulas, FP64 moments and unrounded L2P offsets.  Smooth in the positions, so central   */
+/* differences of it are the derivative to FP64 precision.                                         */
+static void shift_moments64(double* M, const double* Y) {
+	for (int p = 0; p < 3; p++)
+		for (int q = p; q < 3; q++)
+			for (int l = q; l < 3; l++)
+				M[M3(p, q, l)] += M[0] * Y[p] * Y[q] * Y[l] + M[M2(p, q)] * Y[l] + M[M2(q, l)] * Y[p] + M[M2(l, p)] * Y[q];
+	for (int p = 0; p < 3; p++)
+		for (int q = p; q < 3; q++) M[M2(p, q)] += M[0] * Y[p] * Y[q];
+}
+
+/* the spline-softened pair (src/gravity.cpp:231-287) in FP64: g += -m dx f(r), phi += -m p(r) */
+static void spline_pair64(double h, double m, const double* dx, tgo_force* f) {
+	const double r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+	double fr, pr;
+	if (r > h) {
+		fr = 1.0 / (r * r * r);
+		pr = 1.0 / r;
+	} else {
+		const double q = r / h;
+		if (r < 0.5 * h) {
+			fr = (32.0 / 3.0 - 192.0 / 5.0 * q * q + 32.0 * q * q * q) / (h * h * h);
+			pr = (14.0 / 5.0 - 16.0 / 3.0 * q * q + 48.0 / 5.0 * q * q * q * q - 32.0 / 5.0 * q * q * q * q * q) / h;
+		} else {
+			const double q3 = q * q * q;
+			fr = (64.0 / 3.0 * q3 - 48.0 * q3 * q + 192.0 / 5.0 * q3 * q * q - 32.0 / 3.0 * q3 * q3 - 1.0 / 15.0) / (r * r * r);
+			pr = (-1.0 / 15.0 + 16.0 / 5.0 * q - 32.0 / 3.0 * q3 + 16.0 * q3 * q - 48.0 / 5.0 * q3 * q * q + 32.0 / 15.0 * q3 * q3) / r;
+		}
+	}
+	for (int d = 0; d < 3; d++) f->g[d] -= dx[d] * m * fr;
+	f->phi -= m * pr;
+}
+
+void tgo64_frozen_solve(const tgo_params* P, int64_t N, const int32_t* pos, const double* mass, int64_t nn,
+		const int32_t* node_range, const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x,
+		const int64_t* const* ptr, const int32_t* const* lst, tgo_force* out) {
+	double* M = calloc((size_t) nn * 17, sizeof(double));
+	double* L = calloc((size_t) nn * 35, sizeof(double));
+#define MASS(i) (mass ? mass[i] : P->m)
+	/* moments, children before parents (nodes are numbered parents first) */
+	for (int64_t n = nn - 1; n >= 0; n--) {
+		double* Mn = M + 17 * n;
+		const int32_t pb = node_range[2 * n], pe = node_range[2 * n + 1];
+		if (pb == pe) continue;
+		if (node_child[2 * n] < 0) {
+			for (int32_t i = pb; i < pe; i++) {
+				double dx[3];
+				sep64(P->ewald, pos + 3 * (int64_t) i, node_x + 3 * n, dx);
+				const double mi = MASS(i);
+				Mn[0] += mi;
+				for (int a = 0; a < 3; a++)
+					for (int b = a; b < 3; b++) {
+						Mn[M2(a, b)] += mi * dx[a] * dx[b];
+						for (int c = b; c < 3; c++) Mn[M3(a, b, c)] += mi * dx[a] * dx[b] * dx[c];
+					}
+			}
+		} else {
+			for (int k = 0; k < 2; k++) {
+				const int32_t c = node_child[2 * n + k];
+				double Mc[17], Y[3];
+				memcpy(Mc, M + 17 * (int64_t) c, sizeof(Mc));
+				for (int d = 0; d < 3; d++) Y[d] = ((double) node_x[3 * (int64_t) c + d] - (double) node_x[3 * n + d]) * POS_INV;
+				shift_moments64(Mc, Y);
+				for (int q = 0; q < 17; q++) Mn[q] += Mc[q];
+			}
+		}
+	}
+	/* node pass, level by level */
+	int64_t start = 0;
+	while (start < nn) {
+		int64_t end = start + 1;
+		while (end < nn && node_parent[end] < start) end++;
+#pragma omp parallel for schedule(dynamic, 1)
+		for (int64_t n = start; n < end; n++) {
+			if (node_range[2 * n] == node_range[2 * n + 1]) continue;
+			double* Ln = L + 35 * n;
+			const int32_t p = node_parent[n];
+			if (p >= 0) {
+				memcpy(Ln, L + 35 * (int64_t) p, 35 * sizeof(double));
+				double dX[3];
+				for (int d = 0; d < 3; d++) dX[d] = ((double) node_x[3 * n + d] - (double) node_x[3 * (int64_t) p + d]) * POS_INV;
+				l2l_ref(Ln, dX);
+			}
+			for (int k = 0; k < 4; k++) {
+				const int ew = k >= 2;
+				for (int64_t q = ptr[k][n]; q < ptr[k][n + 1]; q += (k & 1) ? 2 : 1) {
+					if (k & 1) { /* CP: particle ranges */
+						for (int32_t j = lst[k][q]; j < lst[k][q + 1]; j++) {
+							double dx[3], D[35];
+							sep64(ew ? 1 : P->ewald, node_x + 3 * n, pos + 3 * (int64_t) j, dx);
+							if (ew) tgo64_green_ewald(dx, D); else tgo64_green_direct(dx, D);
+							const double mj = MASS(j);
+							for (int r = 0; r < 35; r++) Ln[r] += mj * D[r];
+						}
+					} else { /* CC: source nodes */
+						const int32_t c = lst[k][q];
+						double dx[3], D[35];
+						sep64(ew ? 1 : P->ewald, node_x + 3 * n, node_x + 3 * (int64_t) c, dx);
+						if (ew) tgo64_green_ewald(dx, D); else tgo64_green_direct(dx, D);
+						m2l64d(Ln, M + 17 * (int64_t) c, D);
+					}
+				}
+			}
+		}
+		start = end;
+	}
+	/* leaf pass */
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int64_t n = 0; n < nn; n++) {
+		if (node_child[2 * n] >= 0) continue;
+		const int32_t pb = node_range[2 * n], pe = node_range[2 * n + 1];
+		for (int32_t i = pb; i < pe; i++) {
+			double dX[3];
+			sep64(P->ewald, pos + 3 * (int64_t) i, node_x + 3 * n, dX);
+			l2p_ref(L + 35 * n, dX, &out[i]);
+			out[i].phi -= (-315.0 / 128.0) * P->m / P->soft_len;
+			for (int k = 4; k < 8; k++) {
+				const int ew = k >= 6;
+				for (int64_t q = ptr[k][n]; q < ptr[k][n + 1]; q += (k & 1) ? 2 : 1) {
+					if (k & 1) { /* PP: particle ranges */
+						for (int32_t j = lst[k][q]; j < lst[k][q + 1]; j++) {
+							double dx[3];
+							sep64(ew ? 1 : P->ewald, pos + 3 * (int64_t) i, pos + 3 * (int64_t) j, dx);
+							if (ew) {
+								double D[35];
+								tgo64_green_ewald(dx, D);
+								out[i].phi += MASS(j) * D[0];
+								for (int d = 0; d < 3; d++) out[i].g[d] -= MASS(j) * D[E1(d)];
+							} else {
+								spline_pair64(P->soft_len, MASS(j), dx, &out[i]);
+							}
+						}
+					} else { /* PC: source nodes */
+						const int32_t c = lst[k][q];
+						double dx[3], D[35];
+						sep64(ew ? 1 : P->ewald, pos + 3 * (int64_t) i, node_x + 3 * (int64_t) c, dx);
+						if (ew) tgo64_green_ewald(dx, D); else tgo64_green_direct(dx, D);
+						m2p64d(&out[i].phi, out[i].g, M + 17 * (int64_t) c, D);
+					}
+				}
+			}
+		}
+	}
+#undef MASS
+	(void) N;
+	free(M);
+	free(L);
 }

@@ -82,6 +82,14 @@ void tgo_direct64(int periodic, double h, const int32_t* xs, int64_t ns, const i
 /* periodic correction (potential, acceleration) of a unit mass at separation dx, FP64 */
 void tgo_ewald_corr64(const double dx[3], double* phi, double g[3]);
 
+/* FP64 frozen-tree solve (no float rounding anywhere): moments recomputed in FP64 about the kept
+ * centres node_x from the positions pos and masses (tree order; mass NULL = P->m for all), then
+ * the solve over the given lists with the tgo64_ formulas and per-particle masses; forces in tree
+ * order */
+void tgo64_frozen_solve(const tgo_params* P, int64_t N, const int32_t* pos, const double* mass, int64_t nn,
+		const int32_t* node_range, const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x,
+		const int64_t* const* ptr, const int32_t* const* lst, tgo_force* out);
+
 #ifdef __cplusplus
 }
 #endif

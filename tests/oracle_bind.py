@@ -272,6 +272,28 @@ def oracle_solve_leaves(params, T, leaves):
     return out, mask
 
 
+def frozen_solve64(params, T, pos_tree, mass_tree=None):
+    """The FP64 frozen-tree solve (tgo64_frozen_solve) of the tree and lists in T at positions
+    pos_tree (and masses mass_tree, None = params["m"] for all; tree order): moments recomputed in
+    FP64 about the kept centres, tgo64_ kernels, FP64 L2L/L2P.  Forces in tree order."""
+    L = lib()
+    L.tgo64_frozen_solve.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64] + [C.c_void_p] * 4 + [
+        C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tgo64_frozen_solve.restype = None
+    T = dict(T)
+    T["pos"] = pos_tree
+    pos, nr, nc, npar, nx, nm, ptrs, lsts, keep = _solve_args(T)
+    P = Params(int(params["ewald"]), float(params["soft_len"]), float(params["m"]))
+    out = np.zeros((pos.shape[0], 4), np.float64)
+    mp = None
+    if mass_tree is not None:
+        mass_tree = np.ascontiguousarray(mass_tree, np.float64)
+        mp = _p(mass_tree)
+    L.tgo64_frozen_solve(C.byref(P), pos.shape[0], _p(pos), mp, nr.shape[0], _p(nr), _p(nc), _p(npar), _p(nx), ptrs,
+                         lsts, _p(out))
+    return out
+
+
 def ref_solve(R, T, stride=1):
     """The same solve with the compiled reference's own L2L/L2P and kernels (tgref_solve)."""
     R.tgref_solve.argtypes = [C.c_int64, C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [C.c_void_p, C.c_void_p, C.c_int,
