@@ -77,8 +77,8 @@ double tg_ewald_far_separation(const double x[3], double r);
 
 /* green_ewald (tigergrav/green.hpp:189-286) in FP64, host code: the periodic correction with the
  * reference's truncation (real space |n|^2 <= 17 within r < 3.6, Fourier |h|^2 <= 9) and its
- * derivatives to sixth order, at n separations x (3 doubles each, |x_i| <= 1/2).  84 doubles per
- * point: orders 0..6 in turn, each order's sorted index tuples in lexicographic order (orders 0..4
+ * derivatives to seventh order, at n separations x (3 doubles each, |x_i| <= 1/2).  120 doubles per
+ * point: orders 0..7 in turn, each order's sorted index tuples in lexicographic order (orders 0..4
  * are the 35 of the expansion layout).  At x = 0 it gives the analytic limit, not the reference's
  * r -> 0 constant.  The device kernels interpolate a table of these values. */
 int tg_ewald_green64(const double* x, int64_t n, double* D);

@@ -5,9 +5,10 @@
   differences of the order below.
 * Symmetry: permuting coordinates permutes the components; flipping a coordinate's sign flips the
   components with an odd count of that index.
-* The kernels' interpolation (third-order Taylor about the nearest point of a 1/64 grid of
-  float-rounded values, signs by octant), restated here in FP64, against the generator at random
-  separations: the error bound the kernels document.  The kernels themselves are checked against
+* The kernels' interpolation (|x| sorted into the wedge u0 >= u1 >= u2, third-order Taylor about
+  the nearest point of a 1/64 grid of float-rounded values there, components permuted back, signs
+  by octant), restated here in FP64, against the generator at random separations: the error
+  bound the kernels document.  The kernels themselves are checked against
   the oracle by the GPU kernel tests.
 """
 import ctypes as C
@@ -95,10 +96,19 @@ def test_generator_symmetries():
 
 
 def interpolate(x, inv=64, npts=33):
-    """The kernels' scheme in FP64 from float-rounded table values."""
-    u = np.abs(x)
+    """The kernels' scheme in FP64 from float-rounded table values: |x| sorted into the wedge
+    u0 >= u1 >= u2 by the compare-exchanges (0,1), (1,2), (0,1); a third-order Taylor expansion about
+    the nearest grid point there; the exchanges undone on the components in reverse order; signs by
+    octant."""
+    u = np.abs(x).copy()
     sg = np.where(x < 0, -1.0, 1.0)
+    swaps = []
+    for a, b in ((0, 1), (1, 2), (0, 1)):
+        c = u[:, b] > u[:, a]
+        u[c, a], u[c, b] = u[c, b], u[c, a].copy()
+        swaps.append((a, b, c))
     gi = np.minimum(np.floor(u * inv + 0.5), npts - 1)
+    assert np.all(gi[:, 0] >= gi[:, 1]) and np.all(gi[:, 1] >= gi[:, 2])
     d = u - gi / inv
     T = g64(gi / inv).astype(np.float32).astype(np.float64)
     out = np.zeros((len(x), 35))
@@ -110,7 +120,16 @@ def interpolate(x, inv=64, npts=33):
             J = SLOTS[j]
             coef = 1.0 / (math.factorial(J[0]) * math.factorial(J[1]) * math.factorial(J[2]))
             v += T[:, SIDX[(c[0] + J[0], c[1] + J[1], c[2] + J[2])]] * coef * d[:, 0] ** J[0] * d[:, 1] ** J[1] * d[:, 2] ** J[2]
-        out[:, k] = v * np.prod(sg ** np.array(c), axis=1)
+        out[:, k] = v
+    for a, b, c in reversed(swaps):
+        for k in range(35):
+            n = list(SLOTS[k])
+            n[a], n[b] = n[b], n[a]
+            p = SIDX[tuple(n)]
+            if p > k:
+                out[c, k], out[c, p] = out[c, p], out[c, k].copy()
+    for k in range(35):
+        out[:, k] *= np.prod(sg ** np.array(SLOTS[k]), axis=1)
     return out
 
 

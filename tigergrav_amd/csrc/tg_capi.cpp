@@ -33,10 +33,15 @@ int set_hip_error(hipError_t e, const char* where) {
 		if (e_ != hipSuccess) return set_hip_error(e_, #call); \
 	} while (0)
 
-// growable device buffer
+// growable device buffer (freed by its owner, on the device it was allocated on)
 struct DevBuf {
 	void* p = nullptr;
 	size_t cap = 0;
+	void release() {
+		if (p) hipFree(p);
+		p = nullptr;
+		cap = 0;
+	}
 	hipError_t ensure(size_t bytes) {
 		if (bytes <= cap) return hipSuccess;
 		if (p) hipFree(p);
@@ -80,13 +85,16 @@ int get_ctx(const tg_params* p, ThreadCtx** out) {
 			g_err = "device ordinal out of range";
 			return TG_EINVAL;
 		}
+		if (T.device >= 0 && hipSetDevice(T.device) == hipSuccess) {
+			// the previous device's stream and staging buffers, released on that device
+			if (T.stream) hipStreamDestroy(T.stream);
+			for (DevBuf* b : {&T.sinks, &T.srcs, &T.multis, &T.items, &T.out}) b->release();
+		}
+		T = ThreadCtx();  // device -1 until the new device is set up
 		HIPCHK(hipSetDevice(p->device));
 		HIPCHK(tg::init_device_tables(p->device));
-		if (T.stream) hipStreamDestroy(T.stream);
-		T.stream = nullptr;
-		T = ThreadCtx();
-		T.device = p->device;
 		HIPCHK(hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking));
+		T.device = p->device;
 	} else {
 		HIPCHK(hipSetDevice(p->device));
 	}
@@ -231,6 +239,12 @@ bool bad_sizes(const void* f, const void* x, int64_t ns, const void* y, int64_t 
 }
 
 }  // namespace
+
+namespace tg {
+// the calling thread's tg_last_error() message, set from other translation units
+void set_capi_error(const char* where, hipError_t e) { set_hip_error(e, where); }
+void set_capi_message(const char* msg) { g_err = msg; }
+}  // namespace tg
 
 extern "C" {
 

@@ -154,31 +154,19 @@ void ewald_green64(const double* x, double* D) {
 }
 
 void ewald_table_build(std::vector<float>& T) {
-	const int N = TG_EWT_N;
-	T.assign(size_t(N) * N * N * TG_EWT_S, 0.0f);
+	T.assign(size_t(TG_EWT_P) * TG_EWT_S, 0.0f);
 	radial_terms();  // the term tables, before the threads read them
 	fourier_terms();
-	// the correction is symmetric under permutations of the coordinates: evaluate the points with
-	// i >= j >= l, then fill each permutation with the components' counts permuted alike
-#pragma omp parallel for schedule(dynamic, 4)
-	for (int i = 0; i < N; i++)
+	// the points with i >= j >= l; the kernels map every separation onto them by symmetry
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int i = 0; i < TG_EWT_N; i++)
 		for (int j = 0; j <= i; j++)
 			for (int l = 0; l <= j; l++) {
 				const double x[3] = {double(i) / TG_EWT_INV, double(j) / TG_EWT_INV, double(l) / TG_EWT_INV};
 				double D[TG_EWT_S];
 				ewald_green64(x, D);
-				const int idx[3] = {i, j, l};
-				static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-				for (const auto& pm : perms) {
-					// point y with y_a = x_(pm[a]); d/dy_a is d/dx_(pm[a])
-					const size_t p = (size_t(idx[pm[0]]) * N + size_t(idx[pm[1]])) * N + size_t(idx[pm[2]]);
-					for (int k = 0; k < TG_EWT_S; k++) {
-						const int n[3] = {ewt::ST.c[k].n0, ewt::ST.c[k].n1, ewt::ST.c[k].n2};
-						int m[3] = {0, 0, 0};
-						for (int a = 0; a < 3; a++) m[pm[a]] = n[a];
-						T[p * TG_EWT_S + size_t(k)] = float(D[ewt::slot_of(m[0], m[1], m[2])]);
-					}
-				}
+				float* row = T.data() + size_t(ewt::wedge_index(i, j, l)) * TG_EWT_S;
+				for (int k = 0; k < TG_EWT_S; k++) row[k] = float(D[k]);
 			}
 }
 

@@ -3,13 +3,15 @@
 // The correction phi_c(x) = sum_{n != 0} -erfc(2|x - n|)/|x - n| + erf(2r)/r
 //                           - sum_{h != 0} exp(-pi^2 h^2 / 4) / (pi h^2) cos(2 pi h.x) + pi/4
 // (the Ewald sum of -1/r minus its nearest image; real space |n|^2 <= 17, Fourier |h|^2 <= 9, as
-// tigergrav/green.hpp:3-59 truncates them) is smooth over the wrapped cell |x_i| <= 1/2 and even in
-// each coordinate.  Its derivatives to sixth order are tabulated in FP64, rounded to float, on the
-// grid {0, 1/64, ..., 1/2}^3 of the positive octant; a kernel takes the 35 derivatives of orders
-// 0..4 at x from a second-order Taylor expansion about the nearest grid point of |x| and flips the
-// sign of every component with an odd count of a negative coordinate's index.
+// tigergrav/green.hpp:3-59 truncates them) is smooth over the wrapped cell |x_i| <= 1/2, even in
+// each coordinate and symmetric under permutations of the coordinates.  Its derivatives to
+// seventh order are tabulated in FP64, rounded to float, on the points of the grid
+// {0, 1/64, ..., 1/2}^3 with i >= j >= l (6545 points, 3.1 MB: it stays in one XCD's L2).  A
+// kernel sorts |x| into that wedge, takes the derivatives it needs from a third-order Taylor
+// expansion about the nearest point, permutes the components back and flips the sign of every
+// component with an odd count of a negative coordinate's index.
 //
-// Symmetric tensors are stored by slot: orders 0..6 in turn, each order's sorted index tuples in
+// Symmetric tensors are stored by slot: orders 0..7 in turn, each order's sorted index tuples in
 // lexicographic order (orders 0..4 are the expansion layout of tigergrav/expansion.hpp:77-137).
 #pragma once
 #include <stdint.h>
@@ -20,6 +22,7 @@
 #define TG_EWT_INV 64  // inverse grid spacing
 #define TG_EWT_ORDER 7  // highest derivative order tabulated
 #define TG_EWT_S 120    // slots of orders 0..7
+#define TG_EWT_P 6545   // grid points with i >= j >= l: TG_EWT_N (TG_EWT_N + 1) (TG_EWT_N + 2) / 6
 
 namespace tg {
 namespace ewt {
@@ -49,6 +52,17 @@ constexpr int slot_of(int n0, int n1, int n2) {
 		if (ST.c[k].n0 == n0 && ST.c[k].n1 == n1 && ST.c[k].n2 == n2) return k;
 	return -1;
 }
+// slot k with indices a and b exchanged (a transposition of the coordinates)
+constexpr int swap_ab(int k, int a, int b) {
+	int n[3] = {ST.c[k].n0, ST.c[k].n1, ST.c[k].n2};
+	const int t = n[a];
+	n[a] = n[b];
+	n[b] = t;
+	return slot_of(n[0], n[1], n[2]);
+}
+// table row of the sorted grid point i >= j >= l
+constexpr int wedge_index(int i, int j, int l) { return i * (i + 1) * (i + 2) / 6 + j * (j + 1) / 2 + l; }
+static_assert(wedge_index(TG_EWT_N - 1, TG_EWT_N - 1, TG_EWT_N - 1) == TG_EWT_P - 1, "wedge size");
 // slot k with one more index a, with indices a and b
 constexpr int up1(int k, int a) { return slot_of(ST.c[k].n0 + (a == 0), ST.c[k].n1 + (a == 1), ST.c[k].n2 + (a == 2)); }
 constexpr int up2(int k, int a, int b) { return up1(up1(k, a), b); }
@@ -73,7 +87,7 @@ static_assert(slot_of(2, 0, 0) == 4 && slot_of(0, 1, 1) == 8 && slot_of(1, 1, 1)
 // FP64 value of the correction and its TG_EWT_S derivatives (orders 0..TG_EWT_ORDER) at x (host);
 // at x = 0 the analytic limit (the kernels apply the reference's r -> 0 convention themselves)
 void ewald_green64(const double* x, double* D);
-// the table: TG_EWT_N^3 points (x index slowest), TG_EWT_S floats each
+// the table: TG_EWT_P rows (the sorted grid points, row wedge_index(i, j, l)), TG_EWT_S floats each
 void ewald_table_build(std::vector<float>& T);
 
 }  // namespace tg

@@ -118,10 +118,12 @@ __device__ __forceinline__ void pp_pair(const int4& s, float h2u, float hinvu, f
 	}
 }
 
-// a pair known to be at least h apart
+// a pair known to be at least h apart, from its exact separation (position units).  The
+// separation of two distinct positions is at least one unit, so the floor of r2 changes no pair of
+// the list and keeps a zero-weight pad that lands on a sink finite.
 template <bool USE_W, bool PHI>
 __device__ __forceinline__ void pp_far(const int4& s, float& ax, float& ay, float& az, float& pp, float fx, float fy, float fz) {
-	const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+	const float r2 = fmaxf(fmaf(fz, fz, fmaf(fy, fy, fx * fx)), 1.0f);
 	const float ri = __builtin_amdgcn_rsqf(r2);
 	const float wri = USE_W ? __int_as_float(s.w) * ri : ri;
 	const float wri3 = wri * ri * ri;
@@ -131,9 +133,39 @@ __device__ __forceinline__ void pp_far(const int4& s, float& ax, float& ay, floa
 	if (PHI) pp += wri;
 }
 
-// one PP item (<= 64 sinks of a leaf against its source list), by one 64-lane work group
+// the same far pair from offsets to a common centre: sink offset (ox, oy, oz) minus the source's
+// offset (s.x, s.y, s.z), source weight s.w
+template <bool USE_W, bool PHI>
+__device__ __forceinline__ void pp_far_off(const float4& s, float ox, float oy, float oz, float& ax, float& ay, float& az,
+		float& pp) {
+	const float fx = ox - s.x, fy = oy - s.y, fz = oz - s.z;
+	const float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+	const float ri = __builtin_amdgcn_rsqf(r2);
+	const float wri = USE_W ? s.w * ri : ri;
+	const float wri3 = wri * ri * ri;
+	ax = fmaf(fx, wri3, ax);
+	ay = fmaf(fy, wri3, ay);
+	az = fmaf(fz, wri3, az);
+	if (PHI) pp += wri;
+}
+
+// offsets of at least 2^30 position units (a quarter box) from the item's centre: the difference
+// of two offsets below it is the wrapped separation itself
+__device__ __forceinline__ bool off_big(float ox, float oy, float oz) {
+	return fabsf(ox) >= 1073741824.0f || fabsf(oy) >= 1073741824.0f || fabsf(oz) >= 1073741824.0f;
+}
+
+// one PP item (<= 64 sinks of a leaf against its source list), by one 64-lane work group.
+// Tiles of the near prefix take separations as exact integer differences rounded once
+// (src/gravity.cpp:221-229) and the softened pair.  Far tiles take every separation as the
+// difference of two float offsets from the item's first sink: a source's offset is converted
+// once per tile when it is staged, so the pair costs one subtract per axis instead of an integer
+// subtract and a conversion.  The offsets are integer differences rounded once, so a separation
+// carries at most the rounding of the larger offset (2^-24 of it) beside its own.  Periodic tiles
+// with an offset of a quarter box or more (possible only when leaves span that much) keep the
+// exact form, so the wrap-around always picks the nearest image.
 template <bool WRAP, bool PHI, bool UNIFORM>
-__device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* tile) {
+__device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* tile, float4* ftile) {
 	const int lane = threadIdx.x;
 	const int ns = it.y & PP_COUNT_MASK;
 	const int near_len = int(unsigned(it.y) >> PP_NEAR_SHIFT);
@@ -145,9 +177,13 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 	const int si = lane & (nsp - 1);
 	const int ph = lane >> lg;
 	const int4 me = A.sinks[it.x + min(si, ns - 1)];
+	const int4 c0 = A.sinks[it.x];     // the item's centre for the far tiles
+	const float ox = sepu<WRAP>(me.x, c0.x), oy = sepu<WRAP>(me.y, c0.y), oz = sepu<WRAP>(me.z, c0.z);
+	const bool item_off = !WRAP || __ballot(off_big(ox, oy, oz)) == 0ull;
 	// source l of a tile goes to phase l % P, position l / P
 	const int dst = (lane & (P - 1)) * nsp + (lane >> lgP);
 	const int4* mine = tile + ph * nsp;
+	const float4* fmine = ftile + ph * nsp;
 	float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
 	double dax = 0.0, day = 0.0, daz = 0.0, dpp = 0.0;
 	// software pipeline: the next tile's source record (and the list index after it) are loaded
@@ -161,21 +197,40 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 		}
 		return v;
 	};
-	int4 next = source_at(index_at(it.z));
+	int next_idx0 = index_at(it.z);
+	int4 next = source_at(next_idx0);
+	bool next_pad = next_idx0 < 0;
 	int next_idx = index_at(it.z + 64);
 	for (int base = it.z; base < it.w; base += 64) {
 		const int n = min(64, it.w - base);
+		const bool near = base < near_end;
+		// the staged source as an offset from the centre; a pad sits out of reach (r2 overflows to
+		// infinity, its reciprocal root and weight are zero)
+		float4 f = make_float4(3.0e30f, 3.0e30f, 3.0e30f, 0.0f);
+		if (!next_pad) f = make_float4(sepu<WRAP>(next.x, c0.x), sepu<WRAP>(next.y, c0.y), sepu<WRAP>(next.z, c0.z),
+				__int_as_float(next.w));
+		const unsigned long long bigs = WRAP ? __ballot(!next_pad && off_big(f.x, f.y, f.z)) : 0ull;
+		const bool use_off = item_off && !near && bigs == 0ull;
 		__syncthreads();
 		tile[dst] = next;
+		ftile[dst] = f;
 		__syncthreads();
+		next_pad = next_idx < 0;
 		next = source_at(next_idx);
 		next_idx = index_at(base + 128);
 		const int trip = (n + P - 1) >> lgP;
-		if (base < near_end) {
+		if (near) {
 			for (int j = 0; j < trip; j++) {
 				const int4 s = mine[j];
 				pp_pair<true, PHI>(s, A.h2u, A.hinvu, A.hinv3u, ax, ay, az, pp, sepu<WRAP>(me.x, s.x), sepu<WRAP>(me.y, s.y),
 						sepu<WRAP>(me.z, s.z));
+			}
+		} else if (use_off) {
+			if (n == 64) {
+#pragma unroll 8
+				for (int j = 0; j < nsp; j++) pp_far_off<!UNIFORM, PHI>(fmine[j], ox, oy, oz, ax, ay, az, pp);
+			} else {
+				for (int j = 0; j < trip; j++) pp_far_off<true, PHI>(fmine[j], ox, oy, oz, ax, ay, az, pp);
 			}
 		} else if (n == 64) {
 #pragma unroll 8
@@ -216,7 +271,8 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 template <bool WRAP, bool PHI, bool UNIFORM>
 __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 	__shared__ int4 tile[64];
-	pp_item<WRAP, PHI, UNIFORM>(A, A.items[blockIdx.x], tile);
+	__shared__ float4 ftile[64];
+	pp_item<WRAP, PHI, UNIFORM>(A, A.items[blockIdx.x], tile, ftile);
 }
 
 // persistent form: as many work groups as the machine keeps resident, each taking the next
@@ -224,6 +280,7 @@ __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
 template <bool WRAP, bool PHI, bool UNIFORM>
 __global__ __launch_bounds__(64) void k_pp_direct_queue(LeafArgs A) {
 	__shared__ int4 tile[64];
+	__shared__ float4 ftile[64];
 	__shared__ int next;
 	for (;;) {
 		__syncthreads();
@@ -231,7 +288,7 @@ __global__ __launch_bounds__(64) void k_pp_direct_queue(LeafArgs A) {
 		__syncthreads();
 		const int k = next;
 		if (k >= A.nitems) break;
-		pp_item<WRAP, PHI, UNIFORM>(A, A.items[k], tile);
+		pp_item<WRAP, PHI, UNIFORM>(A, A.items[k], tile, ftile);
 	}
 }
 

@@ -228,31 +228,64 @@ __device__ __forceinline__ void ewt_for_seq(std::integer_sequence<int, S...>, F&
 	(f(std::integral_constant<int, S>{}), ...);
 }
 
+// exchanges components k and swap_ab(k, a, b) of D[0..NOUT) where cond holds: the tensor with
+// coordinates a and b exchanged
+template <int NOUT, int A, int B>
+__device__ __forceinline__ void ewt_transpose_if(float* D, bool cond) {
+	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
+		constexpr int k = decltype(k_)::value;
+		constexpr int p = ewt::swap_ab(k, A, B);
+		if constexpr (p > k && p < NOUT) {
+			const float a = D[k], b = D[p];
+			D[k] = cond ? b : a;
+			D[p] = cond ? a : b;
+		}
+	});
+}
+
 // The first NOUT components (all of orders 0..1, 0..2, 0..4 or 0..5: NOUT = 4, 10, 35, 56) of
 // the periodic-correction tensor (tigergrav/green.hpp:189-286) at a wrapped separation x
-// (|x_i| <= 1/2): a third-order Taylor expansion about the point of `table` nearest |x| (FP64
-// values of the correction, rounded to float, on a 1/64 grid), the sign of each component flipped
-// by the coordinates that are negative and occur an odd number of times.  Interpolation error below
-// 4e-6 of each order's largest value (2e-6 RMS at order 4, 1.2e-7 at order 1).  Order-5 components
-// (the adjoint's) take a second-order expansion, the table ending at order 7: error about 1e-4 of
-// that order's largest value.  At |x| <= 1e-6 the reference's convention D = (2.8372975, 0, ..., 0)
-// applies.
+// (|x_i| <= 1/2).  |x| is sorted into the wedge u0 >= u1 >= u2 by three compare-exchanges; the
+// components there come from a third-order Taylor expansion about the nearest grid point of
+// `table` (FP64 values of the correction, rounded to float, on a 1/64 grid of the wedge); the
+// exchanges are then undone on the components in reverse order, and the sign of each component is
+// flipped by the coordinates that are negative and occur an odd number of times.  Interpolation
+// error below 5e-6 of each order's largest value (1.7e-6 RMS over orders 0..4, 1.1e-7 at order 1).
+// Order-5 components (the adjoint's) take a second-order expansion, the table ending at order 7:
+// error about 1e-4 of that order's largest value.  At |x| <= 1e-6 the reference's convention
+// D = (2.8372975, 0, ..., 0) applies.
 template <int NOUT>
 __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const float* table) {
 	constexpr int NLOAD = NOUT == 56 ? TG_EWT_S : ewt_nload(NOUT);
 	static_assert(NOUT == 56 || NOUT == 35 || NOUT == 10 || NOUT == 4, "orders 0..5, 0..4, 0..2 or 0..1");
 	static_assert(NLOAD <= TG_EWT_S, "slots read");
 	const float r2 = fmaf(x[2], x[2], fmaf(x[1], x[1], x[0] * x[0]));
-	float d[3], sg[3];
+	float u[3], sg[3];
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		u[a] = fabsf(x[a]);
+		sg[a] = x[a] < 0.0f ? -1.0f : 1.0f;
+	}
+	// sort |x| descending: exchange (0,1), (1,2), (0,1) where the later coordinate is larger
+	auto cx = [](float& p, float& q, bool c) {
+		const float a = p, b = q;
+		p = c ? b : a;
+		q = c ? a : b;
+	};
+	const bool s1 = u[1] > u[0];
+	cx(u[0], u[1], s1);
+	const bool s2 = u[2] > u[1];
+	cx(u[1], u[2], s2);
+	const bool s3 = u[1] > u[0];
+	cx(u[0], u[1], s3);
+	float d[3];
 	int gi[3];
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
-		const float u = fabsf(x[a]);
-		sg[a] = x[a] < 0.0f ? -1.0f : 1.0f;
-		gi[a] = min(int(fmaf(u, float(TG_EWT_INV), 0.5f)), TG_EWT_N - 1);
-		d[a] = u - float(gi[a]) * (1.0f / TG_EWT_INV);
+		gi[a] = min(int(fmaf(u[a], float(TG_EWT_INV), 0.5f)), TG_EWT_N - 1);
+		d[a] = u[a] - float(gi[a]) * (1.0f / TG_EWT_INV);
 	}
-	const float4* tp = reinterpret_cast<const float4*>(table + size_t((gi[0] * TG_EWT_N + gi[1]) * TG_EWT_N + gi[2]) * TG_EWT_S);
+	const float4* tp = reinterpret_cast<const float4*>(table + size_t(ewt::wedge_index(gi[0], gi[1], gi[2])) * TG_EWT_S);
 	float t[NLOAD];
 #pragma unroll
 	for (int q = 0; q < NLOAD / 4; q++) {
@@ -277,7 +310,6 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 		for (int e = 0; e < n2; e++) m *= d[2];
 		w[j] = m;
 	});
-	const bool z = !(r2 > 1e-12f);
 	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
 		constexpr int k = decltype(k_)::value;
 		// monomials of orders 1..3 (slots 1..19), or 1..2 (slots 1..9) where order 8 would be needed
@@ -288,6 +320,16 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 			static_assert(ewt::merge(k, j) >= 0 && ewt::merge(k, j) < NLOAD, "slot read");
 			v = fmaf(t[ewt_cx<ewt::merge(k, j)>], w[j], v);
 		});
+		D[k] = v;
+	});
+	// back from the wedge to the separation's own axes
+	ewt_transpose_if<NOUT, 0, 1>(D, s3);
+	ewt_transpose_if<NOUT, 1, 2>(D, s2);
+	ewt_transpose_if<NOUT, 0, 1>(D, s1);
+	const bool z = !(r2 > 1e-12f);
+	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
+		constexpr int k = decltype(k_)::value;
+		float v = D[k];
 		if constexpr (ewt::odd_in(k, 0)) v *= sg[0];
 		if constexpr (ewt::odd_in(k, 1)) v *= sg[1];
 		if constexpr (ewt::odd_in(k, 2)) v *= sg[2];
