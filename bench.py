@@ -1,12 +1,17 @@
 """Benchmark: periodic force solve of 2,097,152 particles on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload zeldovich128] [--theta 0.7] [--leaf 0] [--no-graph]
+    python bench.py [--steps K] [--warmup W] [--workload zeldovich128] [--theta 0.3] [--leaf 0] [--no-graph]
 
 Workload (default): 128^3 particles displaced by a Zel'dovich field of a seeded Gaussian random
 field with P(k) ~ k^-2 (cut at the grid Nyquist), rms displacement 0.3 cell, periodic (Ewald),
-theta = 0.7, softening 0.02 N^(-1/3), equal masses 1/N.  By default the leaf size (32/64/128), the
-CC/CP chunk and the PP kernel form are autotuned on this workload's own device time (--leaf N
-fixes the leaf size instead), and the timed steps replay the solve from a captured HIP graph.
+softening 0.02 N^(-1/3), equal masses 1/N.  The timed solve runs at theta = 0.3: the largest
+opening angle whose force error meets RMS |da|/|a| <= 1e-3 against the FP64 direct sum on this
+workload (the `accuracy` field shows it).  The leaf size (16/32/64/128), the CC/CP chunk, the PP
+kernel form and the form of the local pass are autotuned on this workload's own graph-replay time
+(--leaf N with --node-chunk/--pp-persistent/--local-sweep fixes them instead), and the timed steps
+replay the solve from a captured HIP graph.  A second leg, `theta07`, times the same particles at
+theta = 0.7 (the workload's specified angle; about 10x faster, with RMS error ~0.15, which does not
+meet 1e-3).
 
 One step = one force solve on the device over the resident tree and interaction lists: the CC/CP
 node kernels, the local-expansion pass (reduce + L2L + L2P) and the PC/PP leaf kernels.  The tree,
@@ -14,6 +19,8 @@ multipoles and lists are built once on the host before the timed region (their t
 as the end-to-end part of `s_per_force_solve_end_to_end`).  Per-kind kernel times (and the
 roofline) come from HIP events on the solver's stream around each kind's kernels, over K eager
 solves run right after the graph-replayed ones (a captured graph carries no timers).
+`profile_cmd` is the command that reruns the timed configuration with its knobs fixed, under
+which the committed profiler summaries of the dominant kernel were taken.
 
 Prints one JSON line.  `value` is interactions per second (the reference's interaction census,
 src/tree.cpp:483-494,618-628, divided by the device time of one solve).
@@ -145,31 +152,6 @@ def device_free_bytes():
     return int(free.value)
 
 
-def target_leg(x, ewald, h, m, sinks, fref, theta=0.3, leaf=32, steps=5):
-    """The same particles at the opening angle whose force error meets RMS |da|/|a| <= 1e-3 on this
-    workload: graph-replay time per solve, interactions/s and the error on the accuracy sample."""
-    from tigergrav_amd.accuracy import error_stats
-    from tigergrav_amd.solver import Solver
-
-    s = Solver(ewald=ewald, soft_len=h, m=m, theta=theta, parts_per_node=leaf, device=0)
-    s.tune("local_sweep", 1)
-    s.set_particles(x)
-    for _ in range(2):
-        s.solve_graph()
-    s.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s.solve_graph()
-    s.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    total = sum(s.info()["count"].values())
-    acc = error_stats(s.forces()[sinks], fref)
-    s.close()
-    return {"theta": theta, "leaf": leaf, "steps": steps, "ms_per_solve": ms, "interactions_per_s": total / (ms * 1e-3),
-            "rms": acc["rms"], "p99": acc["p99"], "meets_1e-3": bool(acc["rms"] <= 1e-3),
-            "sample": f"the accuracy sample above ({len(sinks)} sinks), same FP64 reference"}
-
-
 def adjoint_leg(s, x, ewald, h, m, forward_ms, steps):
     """The adjoint of the solve (tg_solver_adjoint) on the same particle set: warm device time per
     call against the forward's, a directional central difference of the frozen-tree solve, and the
@@ -208,16 +190,121 @@ def adjoint_leg(s, x, ewald, h, m, forward_ms, steps):
             "loss": "sum_i w_i . g_i, w ~ N(0, I), seed 5"}
 
 
+def build_solver(x, ewald, h, m, theta, leaf, leaves, knobs, host_walk=False):
+    """A solver for the particle set at this opening angle, with the leaf size and knobs fixed
+    (leaf > 0) or autotuned over `leaves` on graph-replay time (leaf == 0).  Returns the solver
+    (particles set), the tuned configuration, the tuning table and the host build time (s)."""
+    from tigergrav_amd.autotune import apply, autotune
+    from tigergrav_amd.solver import Solver
+
+    best, table = None, None
+    if leaf == 0:
+        best, table = autotune(x, ewald, h, m, theta, leaves=leaves)
+        leaf = best["leaf"]
+    else:
+        best = {"leaf": leaf}
+        best.update({k: v for k, v in knobs.items() if v is not None})
+    s = Solver(ewald=ewald, soft_len=h, m=m, theta=theta, parts_per_node=leaf, device=0)
+    if host_walk:
+        s.tune("gpu_walk", 0)
+    apply(s, best)
+    t0 = time.time()
+    s.set_particles(x)
+    return s, best, table, time.time() - t0
+
+
+def time_solves(s, steps, warmup, graph):
+    """ms per solve over `steps` back-to-back solves (graph replays, or eager), after `warmup`
+    untimed ones; then the per-kind device means over as many eager solves with event timers."""
+    step = s.solve_graph if graph else s.solve
+    for _ in range(warmup):
+        step()
+    s.synchronize()
+    s.collect_ms()  # drop the warmup solves' event times
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    s.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ms_eager = None
+    if graph:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.solve()
+        s.synchronize()
+        ms_eager = (time.perf_counter() - t0) * 1e3 / steps
+    kms, nsolves = s.collect_ms()
+    assert nsolves == steps
+    return ms, ms_eager, kms
+
+
+def profile_command(workload, theta, best, steps=3):
+    """The bench command that reruns exactly this configuration with every knob fixed (what the
+    committed rocprofv3 summaries of the dominant kernel were taken from)."""
+    cmd = f"python3 bench.py --workload {workload} --theta {theta} --leaf {best['leaf']}"
+    for k, flag in (("node_chunk", "--node-chunk"), ("pp_persistent", "--pp-persistent"), ("local_sweep", "--local-sweep")):
+        if best.get(k) is not None:
+            cmd += f" {flag} {best[k]}"
+    return cmd + f" --steps {steps} --warmup 1 --no-cpu --no-theta07 --accuracy-sample 1000"
+
+
+def roofline_of(workload, theta, best, count, kms):
+    """Roofline of the dominant kernel: algorithmic flops per interaction x the interactions of one
+    launch / its mean launch time (HIP events on the solver's stream), against the FP32 vector peak;
+    HBM bytes per launch from the committed profiler summary of the same configuration."""
+    dom = max(KINDS, key=lambda k: kms[k])
+    achieved = count[dom] * ALG_FLOPS[dom] / (kms[dom] / 1e3) / 1e12
+    roofline = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "flops_per_interaction": ALG_FLOPS[dom], "interactions": count[dom], "kernel_ms": kms[dom]}
+    # the ceilings this part sustains for the same instruction mix, measured now on this device
+    import ctypes as C
+    from tigergrav_amd._lib import lib
+
+    L = lib()
+    L.tg_calibrate.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    fma_tf, pair_rate = C.c_double(), C.c_double()
+    if L.tg_calibrate(0, 3, C.byref(fma_tf), C.byref(pair_rate)) == 0:
+        roofline["measured_fp32_fma_tflops"] = fma_tf.value
+        roofline["frac_of_measured_fma"] = achieved / fma_tf.value
+        if dom == "PP_direct":
+            rate = count[dom] / (kms[dom] / 1e3)
+            roofline["pp_pairs_per_s"] = rate
+            roofline["pp_pair_loop_ceiling_per_s"] = pair_rate.value
+            roofline["frac_of_pair_loop_ceiling"] = rate / pair_rate.value
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            tr = json.load(open(traffic_file))
+            key = f"{workload}:theta{theta:g}:leaf{best['leaf']}:{dom}"
+            if key in tr:
+                roofline["traffic"] = tr[key]["bytes"]
+                roofline["traffic_source"] = tr[key]["source"]
+        except Exception:
+            pass
+    return roofline
+
+
+def accuracy_of(f, sinks, fref, ns):
+    from tigergrav_amd.accuracy import error_stats
+
+    acc = error_stats(f[sinks], fref)
+    acc.update({"meets_1e-3": bool(acc["rms"] <= 1e-3), "sample": f"{ns} sinks (seed 7)"})
+    return acc
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="zeldovich128")
-    ap.add_argument("--theta", type=float, default=0.7)
+    ap.add_argument("--theta", type=float, default=0.3,
+                    help="opening angle of the timed solve (0.3: the largest that meets RMS 1e-3 on the 2M workload)")
     ap.add_argument("--leaf", type=int, default=0,
-                    help="particles per leaf; 0 autotunes leaf (32, 64, 128), CC/CP chunk and the PP kernel form")
-    ap.add_argument("--wide-autotune", action="store_true", help="autotune over leaves of 8 and 16 as well")
+                    help="particles per leaf; 0 autotunes leaf (16, 32, 64, 128), CC/CP chunk, PP kernel form and "
+                         "local pass")
+    ap.add_argument("--wide-autotune", action="store_true", help="autotune over leaves of 8 as well")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pp-persistent", type=int, default=None, choices=(0, 1),
                     help="with a fixed --leaf: the persistent (1) or one-group-per-item (0) PP kernel")
@@ -228,8 +315,8 @@ def main():
                     help="build the interaction lists with the host walk instead of the device walk")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager solves (with per-kind event timers) instead of HIP-graph replays")
-    ap.add_argument("--no-accuracy-target", action="store_true",
-                    help="skip the theta 0.3 leg (the setting that meets RMS 1e-3 on the 2M workload)")
+    ap.add_argument("--no-theta07", action="store_true",
+                    help="skip the theta 0.7 leg (the 2M workload's specified opening angle, which does not meet 1e-3)")
     ap.add_argument("--accuracy-sample", type=int, default=10000,
                     help="sinks checked against the FP64 direct sum (all of them if >= N)")
     ap.add_argument("--adjoint", action="store_true",
@@ -256,106 +343,31 @@ def main():
         return
 
     from tigergrav_amd import ics
-    from tigergrav_amd.solver import Solver
+    from tigergrav_amd.accuracy import direct_forces
 
     x, ewald, desc = make_workload(args.workload)
     N = x.shape[0]
     h, m = ics.soft_len(N), 1.0 / N
-    tuning, best = None, None
-    if args.leaf == 0:
-        # autotune leaf size, CC/CP chunk and PP kernel form on this very workload's device time
-        from tigergrav_amd.autotune import autotune
-
-        # the adjoint handles leaves of at most 64 particles
-        # the workload's own candidate set is leaf 32/64/128; --wide-autotune adds 8 and 16
-        leaves = (8, 16, 32, 64, 128) if args.wide_autotune else (32, 64, 128)
-        best, tuning = autotune(x, ewald, h, m, args.theta, leaves=tuple(l for l in leaves if l <= 64 or not args.adjoint))
-        args.leaf = best["leaf"]
+    # the adjoint handles leaves of at most 64 particles
+    leaves = tuple(l for l in ((8,) if args.wide_autotune else ()) + (16, 32, 64, 128) if l <= 64 or not args.adjoint)
+    knobs = {"node_chunk": args.node_chunk, "pp_persistent": args.pp_persistent, "local_sweep": args.local_sweep}
     free0 = device_free_bytes()
-    s = Solver(ewald=ewald, soft_len=h, m=m, theta=args.theta, parts_per_node=args.leaf, device=0)
-    if args.host_walk:
-        s.tune("gpu_walk", 0)
-    if best is not None:
-        from tigergrav_amd.autotune import apply
-
-        apply(s, best)
-    else:
-        if args.pp_persistent is not None:
-            s.tune("pp_persistent", args.pp_persistent)
-        if args.node_chunk is not None:
-            s.tune("node_chunk", args.node_chunk)
-        if args.local_sweep is not None:
-            s.tune("local_sweep", args.local_sweep)
-    t0 = time.time()
-    s.set_particles(x)
-    t_host = time.time() - t0
+    s, best, tuning, t_host = build_solver(x, ewald, h, m, args.theta, args.leaf, leaves, knobs, args.host_walk)
     info = s.info()
     count = info["count"]
     total = sum(count.values())
-
-    step = s.solve if args.no_graph else s.solve_graph
-    for _ in range(args.warmup):
-        step()
-    s.synchronize()
-    s.collect_ms()  # drop the warmup solves' event times
+    graph = not args.no_graph
     # timed region: K solves back to back on the resident data (graph replays by default)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    s.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    ms, ms_eager, kms = time_solves(s, args.steps, args.warmup, graph)
     value = total / (ms / 1e3)
-    ms_eager = None
-    if not args.no_graph:
-        # the same K solves eagerly, each recording HIP events around each kind's kernels
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            s.solve()
-        s.synchronize()
-        ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
-    kms, nsolves = s.collect_ms()  # mean per-solve device time of each kind over the eager solves
-    assert nsolves == args.steps
-
-    dom = max(KINDS, key=lambda k: kms[k])
-    achieved = count[dom] * ALG_FLOPS[dom] / (kms[dom] / 1e3) / 1e12
-    roofline = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                "flops_per_interaction": ALG_FLOPS[dom], "interactions": count[dom], "kernel_ms": kms[dom]}
-    # the ceilings this part sustains for the same instruction mix, measured now on this device
-    import ctypes as C
-    from tigergrav_amd._lib import lib
-
-    L = lib()
-    L.tg_calibrate.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    fma_tf, pair_rate = C.c_double(), C.c_double()
-    if L.tg_calibrate(0, 3, C.byref(fma_tf), C.byref(pair_rate)) == 0:
-        roofline["measured_fp32_fma_tflops"] = fma_tf.value
-        roofline["frac_of_measured_fma"] = achieved / fma_tf.value
-        if dom == "PP_direct":
-            rate = count[dom] / (kms[dom] / 1e3)
-            roofline["pp_pairs_per_s"] = rate
-            roofline["pp_pair_loop_ceiling_per_s"] = pair_rate.value
-            roofline["frac_of_pair_loop_ceiling"] = rate / pair_rate.value
-    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(traffic_file):
-        try:
-            tr = json.load(open(traffic_file))
-            key = f"{args.workload}:leaf{args.leaf}:{dom}"
-            if key in tr:
-                roofline["traffic"] = tr[key]
-        except Exception:
-            pass
 
     out = {
         "metric": METRIC, "value": value, "unit": "interactions/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": desc, "N": N, "theta": args.theta, "leaf": args.leaf, "ewald": ewald, "softening": h,
-                   "pp_persistent": best["pp_persistent"] if best else args.pp_persistent,
-                   "node_chunk": best["node_chunk"] if best else args.node_chunk,
-                   "local_sweep": best.get("local_sweep", 0) if best else args.local_sweep,
-                   "accumulation": "f64", "parallelism": "single GPU"},
+        "config": {"workload": desc, "N": N, "theta": args.theta, "leaf": best["leaf"], "ewald": ewald, "softening": h,
+                   "pp_persistent": best.get("pp_persistent"), "node_chunk": best.get("node_chunk"),
+                   "local_sweep": best.get("local_sweep"), "accumulation": "f64", "parallelism": "single GPU"},
         "s_per_force_solve": ms / 1e3,
         "step": "eager solves" if args.no_graph else "HIP-graph replays",
         "ms_per_step_eager": ms_eager,
@@ -365,38 +377,46 @@ def main():
                          "detail": info["t_detail"]},
         "interactions": count, "kernel_ms": kms,
         "tuning": {"best": best, "table": tuning} if tuning else None,
-        "roofline": roofline,
+        "roofline": roofline_of(args.workload, args.theta, best, count, kms),
+        "profile_cmd": profile_command(args.workload, args.theta, best),
     }
     # force accuracy of the solve: FP64 direct sum (converged Ewald when periodic) on the GPU for a
     # seeded sample of sinks, after the timed region
-    from tigergrav_amd.accuracy import direct_forces, error_stats
-
     f = s.forces()
     ns = min(N, args.accuracy_sample)
     sinks = np.sort(np.random.default_rng(7).choice(N, ns, replace=False)) if ns < N else np.arange(N)
     t0 = time.perf_counter()
     fref = direct_forces(x, sinks, ewald, h, m)
-    acc = error_stats(f[sinks], fref)
-    acc.update({"reference": "FP64 direct sum over all N sources" + (", converged Ewald sum" if ewald else ""),
-                "sample": f"{ns} sinks (seed 7)", "s": time.perf_counter() - t0})
-    out["accuracy"] = acc
+    out["accuracy"] = accuracy_of(f, sinks, fref, ns)
+    out["accuracy"].update({"reference": "FP64 direct sum over all N sources" + (", converged Ewald sum" if ewald else ""),
+                            "s": time.perf_counter() - t0})
     # device memory held by the solver (all of it stays allocated between solves): resident data
     # after the forward, and after the adjoint's buffers are added
     fp = {"forward_bytes": free0 - device_free_bytes()}
     if args.adjoint:
-        out["adjoint"] = adjoint_leg(s, x, ewald, h, m, ms, args.steps) if args.leaf <= 64 else {
+        out["adjoint"] = adjoint_leg(s, x, ewald, h, m, ms, args.steps) if best["leaf"] <= 64 else {
             "skipped": "the adjoint handles leaves of at most 64 particles"}
         fp["forward_plus_adjoint_bytes"] = free0 - device_free_bytes()
     fp["method"] = "hipMemGetInfo difference from before the solver was created (the accuracy checks free their buffers)"
     out["hbm_footprint"] = fp
-    if not args.no_accuracy_target and args.workload == "zeldovich128" and args.theta > 0.3 + 1e-9:
-        # this workload meets RMS 1e-3 only at theta 0.3: that operating point, timed the same way
-        out["accuracy_target"] = target_leg(x, ewald, h, m, sinks, fref)
     if not args.no_cpu:
-        out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, args.leaf)
+        out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, best["leaf"])
         if f_cpu is not None:
             # the same solve done on the host by the CPU baseline: its error on the same sample
-            out["cpu_baseline"]["accuracy_rms"] = error_stats(f_cpu[sinks], fref)["rms"]
+            out["cpu_baseline"]["accuracy_rms"] = accuracy_of(f_cpu, sinks, fref, ns)["rms"]
+    s.close()
+    if not args.no_theta07 and args.workload == "zeldovich128" and abs(args.theta - 0.7) > 1e-9:
+        # the workload's specified opening angle, 0.7: faster, but its error is the reference
+        # algorithm's own truncation on near-lattice forces, far above 1e-3 (autotuned on the
+        # workload's leaf set 32/64/128, timed the same way, same accuracy sample)
+        s7, best7, _, t7 = build_solver(x, ewald, h, m, 0.7, 0, (32, 64, 128), knobs)
+        ms7, _, kms7 = time_solves(s7, args.steps, args.warmup, graph)
+        c7 = s7.info()["count"]
+        out["theta07"] = {"theta": 0.7, "best": best7, "ms_per_solve": ms7,
+                          "interactions_per_s": sum(c7.values()) / (ms7 / 1e3), "kernel_ms": kms7,
+                          "s_per_force_solve_end_to_end": t7 + ms7 / 1e3,
+                          "accuracy": accuracy_of(s7.forces(), sinks, fref, ns)}
+        s7.close()
     print(json.dumps(out), flush=True)
 
 

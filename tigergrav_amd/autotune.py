@@ -37,7 +37,8 @@ def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(
 
 
 def apply(solver, best):
-    """Set a tuned configuration's knobs on a solver built with best["leaf"] (before set_particles)."""
-    solver.tune("node_chunk", best["node_chunk"])
-    solver.tune("pp_persistent", best["pp_persistent"])
-    solver.tune("local_sweep", best.get("local_sweep", 0))
+    """Set a tuned configuration's knobs on a solver built with best["leaf"] (before set_particles);
+    knobs missing from `best` keep their defaults."""
+    for knob in ("node_chunk", "pp_persistent", "local_sweep"):
+        if best.get(knob) is not None:
+            solver.tune(knob, best[knob])
