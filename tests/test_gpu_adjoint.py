@@ -160,3 +160,66 @@ def test_adjoint_rejects_a_leaf_beyond_64_particles():
     s.synchronize()
     with pytest.raises(Exception, match="64 particles"):
         s.adjoint(np.ones((n, 3), np.float32))
+
+
+FP64_CASES = {
+    # name: (generator, ewald, theta, leaf)
+    "uniform_periodic": (lambda: ics.uniform(1024, 181), True, 0.7, 16),
+    "box_edge_periodic": (lambda: _edge(ics.uniform(1024, 184), k=16), True, 0.7, 16),
+    "zeldovich_periodic": (lambda: ics.zeldovich(8, 183), True, 0.7, 8),
+    "plummer_open": (lambda: ics.plummer_spheres(8192, 182), False, 0.7, 64),
+}
+
+
+@pytest.mark.parametrize("case", list(FP64_CASES))
+def test_adjoint_against_fp64_frozen_tree(case):
+    """The adjoint's directional derivatives against central differences of the oracle's FP64
+    frozen-tree solve (tgo64_frozen_solve: the same tree, lists and expansion centres, moments
+    recomputed in FP64 about them, the FP64 kernels and FP64 L2L/L2P) -- the function the adjoint
+    differentiates, free of single-precision rounding.  Positions: Richardson-extrapolated central
+    differences at steps s and 2s (s = 2^12 position units per coordinate step); masses: the loss is
+    linear in them.  Bound: 1e-4 relative, no absolute floor."""
+    import oracle_bind as ob
+    from tigergrav_amd.solver import Solver
+
+    gen, ewald, theta, leaf = FP64_CASES[case]
+    x = gen()
+    n = x.shape[0]
+    h, m = ics.soft_len(n), 1.0 / n
+    print(f"adjoint vs FP64 frozen tree {case}: N={n} ewald={ewald} theta={theta} leaf={leaf}", flush=True)
+    s = Solver(ewald=ewald, soft_len=h, m=m, theta=theta, parts_per_node=leaf, device=0)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    rng = np.random.default_rng(len(case) + 200)
+    w = rng.standard_normal((n, 3)).astype(np.float32)
+    grad = s.adjoint(w)
+    T = s.tree_arrays()
+    order = T["order"]
+    P = {"ewald": int(ewald), "soft_len": h, "m": m}
+    wt = w[order].astype(np.float64)
+
+    def loss64(pos_tree, mass_tree=None):
+        return float(np.sum(wt * ob.frozen_solve64(P, T, pos_tree, mass_tree)[:, 1:]))
+
+    d = rng.integers(-1, 2, size=x.shape).astype(np.int64)
+    base = T["pos"].astype(np.int64)
+    fd = []
+    for st in (2 ** 12, 2 ** 13):
+        lp = loss64((base + st * d[order]).astype(np.int32))
+        lm = loss64((base - st * d[order]).astype(np.int32))
+        fd.append((lp - lm) / (2 * st))
+    fd_r = (4 * fd[0] - fd[1]) / 3  # per position unit
+    pred = float(np.sum(grad[:, :3] * d)) * 2.0 ** -32
+    rel = abs(fd_r - pred) / abs(pred)
+    print(f"  positions: fd64 {fd_r:.9e} (steps s, 2s: {fd[0]:.9e}, {fd[1]:.9e}) adjoint {pred:.9e} rel {rel:.2e}",
+          flush=True)
+    assert rel <= 1e-4
+    dm = m * rng.uniform(-1e-2, 1e-2, n)
+    lp = loss64(T["pos"], (m + dm)[order])
+    lm = loss64(T["pos"], (m - dm)[order])
+    fdm = 0.5 * (lp - lm)
+    predm = float(np.sum(grad[:, 3] * dm))
+    relm = abs(fdm - predm) / abs(predm)
+    print(f"  masses: fd64 {fdm:.9e} adjoint {predm:.9e} rel {relm:.2e}", flush=True)
+    assert relm <= 1e-4
