@@ -83,6 +83,21 @@ double tg_ewald_far_separation(const double x[3], double r);
  * r -> 0 constant.  The device kernels interpolate a table of these values. */
 int tg_ewald_green64(const double* x, int64_t n, double* D);
 
+/* ---- no allocation ------------------------------------------------------------------------ */
+/* tg_gravity_PP_direct without any allocation (no counterpart in the reference): for callers that
+ * cannot allocate -- code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, results (bit for bit), flop count and errors as tg_gravity_PP_direct;
+ * its staging lives in a fixed device array of 8.4 MiB that every device holds from the
+ * library's load until the process ends, so sizes are limited: ns <= TG_NOALLOC_MAX_SINKS and
+ * nj <= TG_NOALLOC_MAX_SOURCES, larger calls return TG_EINVAL before any device work.  The array
+ * is shared by every call on one device and keeps the last call's contents: calls that overlap on
+ * one device race (serialise them, e.g. on one host thread); calls on different devices do not.
+ * Work runs on the calling thread's per-thread default stream. */
+#define TG_NOALLOC_MAX_SINKS 16384
+#define TG_NOALLOC_MAX_SOURCES 262144
+int tg_gravity_PP_direct_noalloc(const tg_params* p, tg_force* f, const int32_t* x, int64_t ns, const int32_t* y, int64_t nj,
+		int do_phi, uint64_t* flop);
+
 /* ---- library status ----------------------------------------------------------------------- */
 const char* tg_last_error(void);   /* message for the calling thread's last TG_EHIP / TG_ENOMEM */
 int tg_device_count(void);
