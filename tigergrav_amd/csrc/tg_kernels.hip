@@ -156,6 +156,11 @@ __device__ __forceinline__ bool off_big(float ox, float oy, float oz) {
 }
 
 // one PP item (<= 64 sinks of a leaf against its source list), by one 64-lane work group.
+// Lane = (sink si, phase ph) with P = 64 / ns phases (lanes from P ns on idle), so a leaf of ns
+// sinks keeps P ns of the 64 lanes busy.  A 64-source tile is staged phase-major in LDS: phase ph
+// takes sources ph, ph + P, ... from K = ceil(64 / P) consecutive slots (slots no source fills hold
+// zero-weight pads), so every lane walks its slots with a wave-uniform trip count.  Per-tile partial
+// sums are float, the running sums FP64; the phases of a sink are added in phase order through LDS.
 // Tiles of the near prefix take separations as exact integer differences rounded once
 // (src/gravity.cpp:221-229) and the softened pair.  Far tiles take every separation as the
 // difference of two float offsets from the item's first sink: a source's offset is converted
@@ -164,33 +169,43 @@ __device__ __forceinline__ bool off_big(float ox, float oy, float oz) {
 // carries at most the rounding of the larger offset (2^-24 of it) beside its own.  Periodic tiles
 // with an offset of a quarter box or more (possible only when leaves span that much) keep the
 // exact form, so the wrap-around always picks the nearest image.
+struct PPShared {
+	int4 tile[128];
+	float4 ftile[128];
+	double red[4][64];
+};
+
 template <bool WRAP, bool PHI, bool UNIFORM>
-__device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* tile, float4* ftile) {
+__device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, PPShared& S) {
 	const int lane = threadIdx.x;
-	const int ns = it.y & PP_COUNT_MASK;
+	const int ns = max(it.y & PP_COUNT_MASK, 1);
 	const int near_len = int(unsigned(it.y) >> PP_NEAR_SHIFT);
 	const int near_end = near_len == PP_NEAR_ALL ? INT_MAX : it.z + near_len;
-	const int lg = ceil_log2_64(ns);
-	const int nsp = 1 << lg;           // sinks padded to a power of two
-	const int lgP = 6 - lg;            // phases P = 64 / nsp
-	const int P = 1 << lgP;
-	const int si = lane & (nsp - 1);
-	const int ph = lane >> lg;
-	const int4 me = A.sinks[it.x + min(si, ns - 1)];
+	const int P = 64 / ns;             // phases
+	const int K = (64 + P - 1) / P;    // slots per phase
+	const int ph = min(lane / ns, P - 1);
+	const int si = lane - (lane / ns) * ns;
+	const int4 me = A.sinks[it.x + si];
 	const int4 c0 = A.sinks[it.x];     // the item's centre for the far tiles
 	const float ox = sepu<WRAP>(me.x, c0.x), oy = sepu<WRAP>(me.y, c0.y), oz = sepu<WRAP>(me.z, c0.z);
 	const bool item_off = !WRAP || __ballot(off_big(ox, oy, oz)) == 0ull;
-	// source l of a tile goes to phase l % P, position l / P
-	const int dst = (lane & (P - 1)) * nsp + (lane >> lgP);
-	const int4* mine = tile + ph * nsp;
-	const float4* fmine = ftile + ph * nsp;
+	// source l of a tile goes to phase l % P, slot l / P; slots past the tile's sources are pads
+	const int dst = (lane % P) * K + lane / P;
+	const int4 pad = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);
+	const float4 fpad = make_float4(3.0e30f, 3.0e30f, 3.0e30f, 0.0f);
+	S.tile[lane] = pad;
+	S.tile[lane + 64] = pad;
+	S.ftile[lane] = fpad;
+	S.ftile[lane + 64] = fpad;
+	const int4* mine = S.tile + ph * K;
+	const float4* fmine = S.ftile + ph * K;
 	float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
 	double dax = 0.0, day = 0.0, daz = 0.0, dpp = 0.0;
 	// software pipeline: the next tile's source record (and the list index after it) are loaded
 	// while the current tile is evaluated
 	auto index_at = [&](int b) { return b + lane < it.w ? (A.list ? A.list[b + lane] : b + lane) : -1; };
 	auto source_at = [&](int j) {
-		int4 v = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);  // zero weight
+		int4 v = pad;  // zero weight
 		if (j >= 0) {
 			v = A.srcs[j];
 			if (UNIFORM) v.w = __float_as_int(1.0f);
@@ -206,19 +221,19 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 		const bool near = base < near_end;
 		// the staged source as an offset from the centre; a pad sits out of reach (r2 overflows to
 		// infinity, its reciprocal root and weight are zero)
-		float4 f = make_float4(3.0e30f, 3.0e30f, 3.0e30f, 0.0f);
+		float4 f = fpad;
 		if (!next_pad) f = make_float4(sepu<WRAP>(next.x, c0.x), sepu<WRAP>(next.y, c0.y), sepu<WRAP>(next.z, c0.z),
 				__int_as_float(next.w));
 		const unsigned long long bigs = WRAP ? __ballot(!next_pad && off_big(f.x, f.y, f.z)) : 0ull;
 		const bool use_off = item_off && !near && bigs == 0ull;
 		__syncthreads();
-		tile[dst] = next;
-		ftile[dst] = f;
+		S.tile[dst] = next;
+		S.ftile[dst] = f;
 		__syncthreads();
 		next_pad = next_idx < 0;
 		next = source_at(next_idx);
 		next_idx = index_at(base + 128);
-		const int trip = (n + P - 1) >> lgP;
+		const int trip = n == 64 ? K : (n + P - 1) / P;
 		if (near) {
 			for (int j = 0; j < trip; j++) {
 				const int4 s = mine[j];
@@ -226,18 +241,9 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 						sepu<WRAP>(me.z, s.z));
 			}
 		} else if (use_off) {
-			if (n == 64) {
+			// pads add exact zeros with or without their weight
 #pragma unroll 8
-				for (int j = 0; j < nsp; j++) pp_far_off<!UNIFORM, PHI>(fmine[j], ox, oy, oz, ax, ay, az, pp);
-			} else {
-				for (int j = 0; j < trip; j++) pp_far_off<true, PHI>(fmine[j], ox, oy, oz, ax, ay, az, pp);
-			}
-		} else if (n == 64) {
-#pragma unroll 8
-			for (int j = 0; j < nsp; j++) {
-				const int4 s = mine[j];
-				pp_far<!UNIFORM, PHI>(s, ax, ay, az, pp, sepu<WRAP>(me.x, s.x), sepu<WRAP>(me.y, s.y), sepu<WRAP>(me.z, s.z));
-			}
+			for (int j = 0; j < trip; j++) pp_far_off<!UNIFORM, PHI>(fmine[j], ox, oy, oz, ax, ay, az, pp);
 		} else {
 			for (int j = 0; j < trip; j++) {
 				const int4 s = mine[j];
@@ -250,16 +256,22 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 		dpp += pp;
 		ax = ay = az = pp = 0.f;
 	}
-	for (int off = nsp; off < 64; off <<= 1) {
-		dax += wave_xor(dax, off);
-		day += wave_xor(day, off);
-		daz += wave_xor(daz, off);
-		dpp += wave_xor(dpp, off);
-	}
-	if (ph == 0 && si < ns) {
+	// the phases of each sink, added in phase order
+	S.red[0][lane] = dax;
+	S.red[1][lane] = day;
+	S.red[2][lane] = daz;
+	S.red[3][lane] = dpp;
+	__syncthreads();
+	if (lane < ns) {
+		for (int q = 1; q < P; q++) {
+			dax += S.red[0][q * ns + lane];
+			day += S.red[1][q * ns + lane];
+			daz += S.red[2][q * ns + lane];
+			dpp += S.red[3][q * ns + lane];
+		}
 		// position units: g = -sum w dx r^-3 = -2^64 sum (w du ru^-3), phi = -2^32 sum (w / ru)
 		const double ms = UNIFORM ? double(A.m) : 1.0;
-		double* o = A.out + 4 * (size_t) (it.x + si);
+		double* o = A.out + 4 * (size_t) (it.x + lane);
 		if (PHI) o[0] += -4294967296.0 * ms * dpp;
 		o[1] += -18446744073709551616.0 * ms * dax;
 		o[2] += -18446744073709551616.0 * ms * day;
@@ -270,17 +282,15 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, int4* 
 // one work group per item; the dispatcher hands items out in array order (heaviest first)
 template <bool WRAP, bool PHI, bool UNIFORM>
 __global__ __launch_bounds__(64) void k_pp_direct(LeafArgs A) {
-	__shared__ int4 tile[64];
-	__shared__ float4 ftile[64];
-	pp_item<WRAP, PHI, UNIFORM>(A, A.items[blockIdx.x], tile, ftile);
+	__shared__ PPShared S;
+	pp_item<WRAP, PHI, UNIFORM>(A, A.items[blockIdx.x], S);
 }
 
 // persistent form: as many work groups as the machine keeps resident, each taking the next
 // item from a queue counter (A.queue, zero at launch) until the items run out
 template <bool WRAP, bool PHI, bool UNIFORM>
 __global__ __launch_bounds__(64) void k_pp_direct_queue(LeafArgs A) {
-	__shared__ int4 tile[64];
-	__shared__ float4 ftile[64];
+	__shared__ PPShared S;
 	__shared__ int next;
 	for (;;) {
 		__syncthreads();
@@ -288,7 +298,7 @@ __global__ __launch_bounds__(64) void k_pp_direct_queue(LeafArgs A) {
 		__syncthreads();
 		const int k = next;
 		if (k >= A.nitems) break;
-		pp_item<WRAP, PHI, UNIFORM>(A, A.items[k], tile, ftile);
+		pp_item<WRAP, PHI, UNIFORM>(A, A.items[k], S);
 	}
 }
 
