@@ -5,9 +5,10 @@ Two references on identical inputs (tests/kernel_cases.py, both kernel-parameter
   * tgo64_ -- the same formulas in FP64.
 Metric: rms relative error over a case set, sqrt(sum |d|^2 / sum |ref|^2), taken separately for
 accelerations and potentials (PP/PC) or over all 35 coefficients (CC/CP).  Tolerance: 1e-5.
-For the Ewald kinds the reference forms the correction in float with a cancellation at close
-range, so the scale is the total the correction is added to (correction + the matching direct
-kernel's term); the correction alone is also compared with tgo64_ where it is not cancelling.
+For the Ewald kinds the comparison with tgo64_ takes the correction alone (its own norm as the
+scale).  The reference's float evaluation (tgo_) forms the correction with a cancellation at close
+range that costs it about 1e-5 of its own, so against tgo_ the scale is the total the correction is
+added to (correction + the matching direct kernel's term).
 """
 import math
 
@@ -78,15 +79,15 @@ def test_kernel_parity(tg, cfg, kind):
             if key == "p" and kind == "PP_direct" and not c["do_phi"]:
                 continue
             a, b32, b64 = gpu[..., sl[0]], r32[..., sl[0]], r64[..., sl[0]]
-            den = np.sum(b64 ** 2) + (np.sum(extra[..., sl[0]] ** 2) if extra is not None else 0.0)
+            den64 = np.sum(b64 ** 2)
+            den32 = den64 + (np.sum(extra[..., sl[0]] ** 2) if extra is not None else 0.0)
             acc[key + "32"][0] += np.sum((a - b32) ** 2)
-            acc[key + "32"][1] += den
+            acc[key + "32"][1] += den32
             acc[key + "64"][0] += np.sum((a - b64) ** 2)
-            acc[key + "64"][1] += den
-    for key, (num, den) in acc.items():
-        if den == 0:
-            continue
-        e = rel(num, den)
+            acc[key + "64"][1] += den64
+    errs = {key: rel(num, den) for key, (num, den) in acc.items() if den > 0}
+    print(f"{kind} {cfg}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for key, e in errs.items():
         assert e < TOL, f"{kind} {cfg} {key}: rms rel {e:.3e}"
 
 
@@ -103,7 +104,8 @@ def test_ewald_correction_alone(tg, kind):
         sl = slice(1, 4) if kind[:2] == "PC" else slice(0, 35)
         num += np.sum((gpu[..., sl] - r64[..., sl]) ** 2)
         den += np.sum(r64[..., sl] ** 2)
-    assert rel(num, den) < 2e-5
+    print(f"{kind} correction alone vs FP64: rms rel {rel(num, den):.2e}", flush=True)
+    assert rel(num, den) < TOL
 
 
 def test_pp_ewald_close_pairs_beat_reference_float(tg):
@@ -120,7 +122,8 @@ def test_pp_ewald_close_pairs_beat_reference_float(tg):
         e_gpu += np.sum((gpu - r64) ** 2)
         e_ref += np.sum((r32 - r64) ** 2)
         den += np.sum(r64 ** 2)
-    assert rel(e_gpu, den) < 2e-5
+    print(f"PP_ewald vs FP64: GPU {rel(e_gpu, den):.2e}, reference float {rel(e_ref, den):.2e}", flush=True)
+    assert rel(e_gpu, den) < TOL
     assert e_gpu < e_ref
 
 
