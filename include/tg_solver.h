@@ -44,6 +44,10 @@ int tg_solver_destroy(tg_solver* s);
  * construction) on the device or, with "gpu_walk" 0 or a host-only solver, on the host; makes
  * them resident on the device */
 int tg_solver_set_particles(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
+/* tg_solver_set_particles with x (and mass, if not NULL) in device memory on the solver's device:
+ * the tree is built there ("gpu_tree" 1, the default), so a time step needs no host copy of the
+ * particles (the reference keeps them in its own memory, src/part_vect.cpp) */
+int tg_solver_set_particles_device(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
 /* new positions (and masses, or NULL to keep them) for the same particles, in input order, with
  * the tree, the interaction lists and the expansion centres of the last tg_solver_set_particles
  * kept: the moments are recomputed about the kept centres.  The next solve evaluates that frozen
@@ -76,6 +80,8 @@ int tg_solver_solve_graph(tg_solver* s);
 int tg_solver_synchronize(tg_solver* s);
 /* forces of the last solve, in input particle order (the reference's `force` per particle) */
 int tg_solver_get_forces(tg_solver* s, tg_force* f);
+/* the same forces into device memory f (n tg_force, input order), for kick (tg_step.h) */
+int tg_solver_get_forces_device(tg_solver* s, tg_force* f);
 /* device time (ms) of each kind's kernels in the last solve: 8 kinds in tg_solve_info order, then
  * the local-expansion pass (reduce + L2L + L2P) */
 int tg_solver_kernel_ms(tg_solver* s, double* ms /* [9] */);

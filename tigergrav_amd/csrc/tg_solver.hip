@@ -17,6 +17,7 @@
 #include "tg_gravity.h"
 #include "tg_local.hpp"
 #include "tg_solver.hpp"
+#include "tg_devtree.hpp"
 #include "tg_upload.hpp"
 #include "tg_walk.hpp"
 #include "tg_solver.h"
@@ -323,8 +324,13 @@ Solver::~Solver() {
 	}
 }
 
-int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
+int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bool on_device) {
+	if (on_device && !(gpu_tree && device >= 0)) {
+		err = "particles in device memory need the device tree build (\"gpu_tree\" 1)";
+		return TG_EINVAL;
+	}
 	drop_graph();  // the captured launches point at buffers that may be reallocated
+	order_on_device = false;
 	n = count;
 	TreeParams P;
 	P.ewald = ewald;
@@ -333,10 +339,19 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count) {
 	P.parts_per_node = parts_per_node;
 	double t0 = now();
 	uniform_mass = (mass == nullptr);
-	tree.build(x, mass, count, P, m);
-	double t1 = now();
-	tree.compute_multipoles();
-	double t2 = now();
+	double t1, t2;
+	if (gpu_tree && device >= 0) {
+		double tt[2] = {0.0, 0.0};
+		const hipError_t e = device_build_tree(tree, x, mass, count, P, m, device, stream, err, tt, on_device);
+		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
+		t1 = t0 + tt[0];
+		t2 = now();
+	} else {
+		tree.build(x, mass, count, P, m);
+		t1 = now();
+		tree.compute_multipoles();
+		t2 = now();
+	}
 	if (gpu_walk) {
 		// the CC lists stay on the device as the work lists (the host reads only their pointers);
 		// host_lists() copies them back when something asks for the values
@@ -778,6 +793,29 @@ int Solver::collect_times(double* ms_sum, int64_t* count) {
 	return TG_OK;
 }
 
+__global__ void k_forces_input_order(const double* __restrict__ out, const int64_t* __restrict__ order, int64_t n,
+		double* __restrict__ f) {
+	const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int64_t o = order[i];
+	for (int c = 0; c < 4; c++) f[4 * o + c] = out[4 * i + c];
+}
+
+int Solver::forces_device(double* out4_dev) {
+	SCHK(hipSetDevice(device));
+	if (n == 0) return TG_OK;
+	if (!order_on_device) {
+		SCHK(d_order.alloc(size_t(n) * sizeof(int64_t)));
+		SCHK(hipMemcpyAsync(d_order.p, tree.order.data(), size_t(n) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+		order_on_device = true;
+	}
+	hipLaunchKernelGGL(k_forces_input_order, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, d_out.as<const double>(),
+			d_order.as<const int64_t>(), n, out4_dev);
+	SCHK(hipGetLastError());
+	SCHK(hipStreamSynchronize(stream));
+	return TG_OK;
+}
+
 int Solver::forces(double* out4) {
 	SCHK(hipSetDevice(device));
 	std::vector<double> h(n * 4);
@@ -836,6 +874,16 @@ int tg_solver_set_particles(tg_solver* h, const int32_t* x, const float* mass, i
 	return fail(h, h->s.set_particles(x, mass, n));
 }
 
+int tg_solver_set_particles_device(tg_solver* h, const int32_t* x, const float* mass, int64_t n) {
+	if (!h || n < 0 || (n > 0 && !x) || n > (int64_t(1) << 31) - 64) return TG_EINVAL;
+	return fail(h, h->s.set_particles(x, mass, n, true));
+}
+
+int tg_solver_get_forces_device(tg_solver* h, tg_force* f) {
+	if (!h || h->s.device < 0 || (h->s.n > 0 && !f)) return TG_EINVAL;
+	return fail(h, h->s.forces_device(reinterpret_cast<double*>(f)));
+}
+
 int tg_solver_solve(tg_solver* h) {
 	if (!h || h->s.device < 0) return TG_EINVAL;
 	return fail(h, h->s.enqueue_solve(true));
@@ -860,6 +908,9 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 	} else if (k == "gpu_walk") {
 		if (S.device < 0 && value != 0) return TG_EINVAL;
 		S.gpu_walk = value != 0;
+	} else if (k == "gpu_tree") {
+		if (S.device < 0 && value != 0) return TG_EINVAL;
+		S.gpu_tree = value != 0;
 	} else {
 		g_solver_err = "unknown tuning knob " + k;
 		return TG_EINVAL;
@@ -965,7 +1016,7 @@ int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capa
 	else if (nm == "mass") put(T.mass.data(), T.mass.size() * 4);
 	else if (nm == "leaves") put(T.leaves.data(), T.leaves.size() * 4);
 	else if (nm == "node_range" || nm == "node_child" || nm == "node_parent" || nm == "node_level" || nm == "node_x" ||
-	         nm == "node_m" || nm == "node_r" || nm == "node_boxid") {
+	         nm == "node_m" || nm == "node_r" || nm == "node_boxid" || nm == "node_box" || nm == "node_n") {
 		std::vector<char> v;
 		for (size_t i = 0; i < nn; i++) {
 			const tg::Node& d = T.nodes[i];
@@ -973,6 +1024,7 @@ int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capa
 			size_t b = 0;
 			int32_t tmp[2];
 			uint64_t bid[2];
+			double box[6];
 			if (nm == "node_range") { tmp[0] = d.pbegin; tmp[1] = d.pend; p = (const char*) tmp; b = 8; }
 			else if (nm == "node_child") { tmp[0] = d.child[0]; tmp[1] = d.child[1]; p = (const char*) tmp; b = 8; }
 			else if (nm == "node_parent") { p = (const char*) &d.parent; b = 4; }
@@ -980,6 +1032,13 @@ int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capa
 			else if (nm == "node_x") { p = (const char*) d.x; b = 12; }
 			else if (nm == "node_m") { p = (const char*) d.m; b = 68; }
 			else if (nm == "node_r") { p = (const char*) &d.r; b = 4; }
+			else if (nm == "node_n") { p = (const char*) &d.N; b = 8; }
+			else if (nm == "node_box") {
+				std::memcpy(box, d.rmin, 24);
+				std::memcpy(box + 3, d.rmax, 24);
+				p = (const char*) box;
+				b = 48;
+			}
 			else { bid[0] = uint64_t(d.boxid); bid[1] = uint64_t(d.boxid >> 64); p = (const char*) bid; b = 16; }
 			v.insert(v.end(), p, p + b);
 		}

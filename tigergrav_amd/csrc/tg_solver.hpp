@@ -46,6 +46,7 @@ struct Solver {
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 	bool local_sweep = false;    // local pass level by level (k_l2l_level) instead of per-leaf path sums (k_local)
 	bool gpu_walk = false;       // build the interaction lists on the device (device_walk); on for device solvers
+	bool gpu_tree = false;       // build the tree and multipoles on the device (device_build_tree); on for device solvers
 	int resident_groups = 1024;  // work groups of the persistent form (set from the CU count)
 	Tree tree;
 	int64_t n = 0;
@@ -95,7 +96,7 @@ struct Solver {
 	std::string err;
 
 	int init(int dev);
-	int set_particles(const int32_t* x, const float* mass, int64_t count);
+	int set_particles(const int32_t* x, const float* mass, int64_t count, bool on_device = false);
 	int upload();
 	int host_lists();  // copies list values the device walk kept on the device into tree.lists
 	int upload_sources();
@@ -112,6 +113,9 @@ struct Solver {
 	int collect_times(double* ms_sum, int64_t* n);  // sums over the timed solves since the last collect
 	int fold_set(int64_t solve);
 	int forces(double* out4);  // N x (phi, gx, gy, gz), input order
+	int forces_device(double* out4_dev);  // the same into device memory, on the solver's stream (waits for it)
+	DeviceArray d_order;  // input index of each tree-order particle, uploaded by forces_device
+	bool order_on_device = false;
 	~Solver();
 };
 

@@ -170,7 +170,7 @@ class Solver:
 
     _DT = {"pos": (np.int32, 3), "order": (np.int64, 1), "mass": (np.float32, 1), "leaves": (np.int32, 1),
            "node_range": (np.int32, 2), "node_child": (np.int32, 2), "node_parent": (np.int32, 1),
-           "node_level": (np.int32, 1), "node_x": (np.int32, 3), "node_m": (np.float32, 17), "node_r": (np.float32, 1),
+           "node_level": (np.int32, 1), "node_x": (np.int32, 3), "node_m": (np.float32, 17), "node_r": (np.float32, 1), "node_box": (np.float64, 6), "node_n": (np.uint64, 1),
            "node_boxid": (np.uint64, 2), "pp_d_items": (np.int32, 4), "pp_d_list": (np.int32, 1)}
 
     def export(self, name):
