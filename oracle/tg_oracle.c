@@ -1383,3 +1383,118 @@ void tgo64_frozen_solve(const tgo_params* P, int64_t N, const int32_t* pos, cons
 	free(M);
 	free(L);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* time step: part_vect_kick (src/part_vect.cpp:341-436) and part_vect_drift (src/part_vect.cpp:
+ * 493-540) over one particle range, in particle order, with the time grid of tigergrav/time.hpp.
+ * Every particle's force is given (f[i]); the reference compacts them to the active ones, which
+ * reads the same values.  float here is the reference's vect<float>, double its vect<double>. */
+static double tgo_rung_to_dt(double t_max, int rung) { return t_max / (double) (1 << rung); } /* time.hpp:36-39 */
+
+static int tgo_dt_to_rung(double t_max, double dt) { /* time.hpp:41-48; -1 past rung 30 */
+	int rung = 0;
+	while (tgo_rung_to_dt(t_max, rung) > dt) {
+		rung++;
+		if (rung > 30) return -1;
+	}
+	return rung;
+}
+
+int tgo_kick(const tgo_step_params* P, int64_t n, const tgo_force* f, float* v, int8_t* rung, int min_rung, int do_out,
+		double* stats /* g[3], p[3], pot, kin, max_rung */) {
+	const double eps = 10.0 * 2.2250738585072014e-308; /* 10 numeric_limits<double>::min() */
+	const double scale = P->scale;
+	const double a3inv = 1.0 / (scale * scale * scale);
+	const double m = P->m;
+	const double sgn = P->glass ? -1.0 : 1.0;
+	const double glass_drag = 1.0;
+	double g[3] = {0, 0, 0}, p[3] = {0, 0, 0}, pot = 0.0, kin = 0.0;
+	int rc_rung = 0, bad = 0;
+	for (int64_t i = 0; i < n; i++) {
+		float* vi = v + 3 * i;
+		/* p = p + v / problem_size / (scale scale): vect<float> / float, / float, then to double */
+		for (int d = 0; d < 3; d++) p[d] = p[d] + (double) ((vi[d] / (float) P->problem_size) / (float) (scale * scale));
+		float dot = vi[0] * vi[0];
+		dot += vi[1] * vi[1];
+		dot += vi[2] * vi[2];
+		kin += 0.5 * m * (double) dot / (scale * scale);
+		const int r = rung[i];
+		if (r >= min_rung || do_out) {
+			if (r >= min_rung) {
+				if (r != 0) {
+					const double dt = tgo_rung_to_dt(P->t_max, r);
+					const double dt1 = P->cosmic ? P->kick_dt1[r] : 0.5 * dt;
+					for (int d = 0; d < 3; d++) vi[d] = vi[d] + (float) (f[i].g[d] * (dt1 * sgn * P->G));
+					if (P->glass)
+						for (int d = 0; d < 3; d++) vi[d] = vi[d] / (float) (1.0 + glass_drag * dt1);
+				}
+				double gG[3];
+				for (int d = 0; d < 3; d++) gG[d] = f[i].g[d] * P->G;
+				double gg = gG[0] * gG[0];
+				gg += gG[1] * gG[1];
+				gg += gG[2] * gG[2];
+				const double a = sqrt(gg) * a3inv;
+				double dt = P->eta * sqrt(P->soft_len * (1.0 / 2.8) / (a + eps));
+				if (P->dt_max < dt) dt = P->dt_max; /* std::min(dt_max, dt) */
+				int nr = tgo_dt_to_rung(P->t_max, dt);
+				if (nr < 0) {
+					bad = 1;
+					nr = 30;
+				}
+				if (nr < min_rung) nr = min_rung;
+				if (nr > rc_rung) rc_rung = nr;
+				dt = tgo_rung_to_dt(P->t_max, nr);
+				int st = nr > r - 1 ? nr : r - 1;
+				if (st < 1) st = 1;
+				rung[i] = (int8_t) st;
+				const double dt2 = P->ewald ? P->kick_dt2[st] : 0.5 * dt;
+				for (int d = 0; d < 3; d++) vi[d] = vi[d] + (float) (f[i].g[d] * (dt2 * sgn * P->G));
+				if (P->glass)
+					for (int d = 0; d < 3; d++) vi[d] = vi[d] / (float) (1.0 + glass_drag * dt2);
+			}
+			if (do_out) {
+				for (int d = 0; d < 3; d++) g[d] = g[d] + f[i].g[d] * P->G / P->problem_size / (scale * scale * scale);
+				pot += 0.5 * m * f[i].phi * P->G / scale;
+			}
+		}
+	}
+	for (int d = 0; d < 3; d++) {
+		stats[d] = g[d];
+		stats[3 + d] = p[d];
+	}
+	stats[6] = pot;
+	stats[7] = kin;
+	stats[8] = rc_rung;
+	return bad ? -1 : 0;
+}
+
+static int32_t tgo_double_to_pos(double x) { /* tigergrav/position.hpp:30-37 */
+	const double POS_MAX = 4294967296.0;
+	const double y = x - 0.5;
+	long z = lround(y * POS_MAX);
+	if (z >= POS_MAX / 2.0) z -= (long) POS_MAX;
+	return (int32_t) z;
+}
+
+double tgo_drift(const tgo_step_params* P, int64_t n, int32_t* x, const float* v, int max_rung, double drift_dt, double a1) {
+	const double m = P->m;
+	const double a1inv2 = 1.0 / (a1 * a1);
+	const double dt = P->cosmic ? drift_dt : tgo_rung_to_dt(P->t_max, max_rung);
+	double ekin = 0.0;
+	for (int64_t j = 0; j < n; j++) {
+		const float* vj = v + 3 * j;
+		float dot = vj[0] * vj[0];
+		dot += vj[1] * vj[1];
+		dot += vj[2] * vj[2];
+		ekin += 0.5 * m * (double) dot * a1inv2;
+		for (int d = 0; d < 3; d++) {
+			const double dx = (double) (vj[d] * (float) dt); /* vect<float> * float */
+			double xd = (double) x[3 * j + d] * POS_INV + 0.5;
+			xd += dx;
+			while (xd >= 1.0) xd -= 1.0;
+			while (xd < 0.0) xd += 1.0;
+			x[3 * j + d] = tgo_double_to_pos(xd);
+		}
+	}
+	return ekin;
+}

@@ -109,6 +109,20 @@ uint64_t tgo_solve(const tgo_params* P, int64_t N, const int32_t* pos, int64_t n
 uint64_t tgo_solve_want(const tgo_params* P, int64_t N, const int32_t* pos, int64_t nn, const int32_t* node_range,
 		const int32_t* node_child, const int32_t* node_parent, const int32_t* node_x, const float* node_m,
 		const int64_t* const* ptr, const int32_t* const* lst, const uint8_t* want, tgo_force* out);
+
+/* ---- time step: part_vect_kick / part_vect_drift (src/part_vect.cpp:341-436,493-540) ---- */
+typedef struct tgo_step_params {
+	double G, eta, soft_len, dt_max, t_max, m, problem_size;
+	int glass, cosmic, ewald;
+	double scale;
+	double kick_dt1[31], kick_dt2[31];
+} tgo_step_params;
+/* stats: g[3], p[3], pot, kin, max rung; returns -1 if a step falls below rung 30 */
+int tgo_kick(const tgo_step_params* P, int64_t n, const tgo_force* f, float* v, int8_t* rung, int min_rung, int do_out,
+		double* stats);
+/* returns the kinetic energy before the move */
+double tgo_drift(const tgo_step_params* P, int64_t n, int32_t* x, const float* v, int max_rung, double drift_dt, double a1);
+
 #ifdef __cplusplus
 }
 #endif

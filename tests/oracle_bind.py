@@ -304,3 +304,46 @@ def ref_solve(R, T, stride=1):
     flop = R.tgref_solve(pos.shape[0], _p(pos), nr.shape[0], _p(nr), _p(nc), _p(npar), _p(nx), _p(nm), ptrs, lsts,
                          int(stride), _p(out))
     return out, flop
+
+
+class StepParams(C.Structure):
+    """tgo_step_params (oracle/tg_oracle.h)."""
+    _fields_ = [("G", C.c_double), ("eta", C.c_double), ("soft_len", C.c_double), ("dt_max", C.c_double),
+                ("t_max", C.c_double), ("m", C.c_double), ("problem_size", C.c_double), ("glass", C.c_int),
+                ("cosmic", C.c_int), ("ewald", C.c_int), ("scale", C.c_double), ("kick_dt1", C.c_double * 31),
+                ("kick_dt2", C.c_double * 31)]
+
+
+def step_params(P):
+    """The oracle's parameter block from the product's (tigergrav_amd.step.StepParams)."""
+    Q = StepParams()
+    for name, _ in StepParams._fields_:
+        if name.startswith("kick_dt"):
+            for r in range(31):
+                getattr(Q, name)[r] = getattr(P, name)[r]
+        else:
+            setattr(Q, name, getattr(P, name))
+    return Q
+
+
+def kick(P, f, v, rung, min_rung, do_out):
+    """tgo_kick on host arrays (v float32 [n, 3] and rung int8 [n] updated in place); returns
+    (status, stats dict)."""
+    L = lib()
+    L.tgo_kick.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    L.tgo_kick.restype = C.c_int
+    Q = step_params(P)
+    f = np.ascontiguousarray(f, np.float64)
+    st = np.zeros(9)
+    rc = L.tgo_kick(C.byref(Q), v.shape[0], _p(f), _p(v), _p(rung), int(min_rung), int(do_out), _p(st))
+    return rc, {"g": st[0:3].copy(), "p": st[3:6].copy(), "pot": st[6], "kin": st[7], "max_rung": int(st[8])}
+
+
+def drift(P, x, v, max_rung, drift_dt=0.0, a1=1.0):
+    """tgo_drift on host arrays (x int32 [n, 3] updated in place); returns the kinetic energy."""
+    L = lib()
+    L.tgo_drift.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double]
+    L.tgo_drift.restype = C.c_double
+    Q = step_params(P)
+    v = np.ascontiguousarray(v, np.float32)
+    return L.tgo_drift(C.byref(Q), x.shape[0], _p(x), _p(v), int(max_rung), float(drift_dt), float(a1))
