@@ -90,3 +90,37 @@ def zeldovich(ngrid, seed=3, n_s=-2.0, rms_cells=0.3, return_psi=False):
 def soft_len(n):
     """The reference's default softening length 0.02 N^(-1/3) (src/options.cpp:86-88)."""
     return 0.02 * float(n) ** (-1.0 / 3.0)
+
+
+def load_ngenic(path):
+    """An N-GenIC / Gadget-1 file (include/tg_load.h, src/load.cpp:34-96): (x int32 [N, 3],
+    v float32 [N, 3], header dict).  Positions shifted by half the mean spacing and wrapped,
+    velocities times (1 + z)^(-3/2), as the reference loads them."""
+    import ctypes as C
+
+    from ._lib import TGError, lib
+
+    class Header(C.Structure):
+        _fields_ = [("npart", C.c_uint32 * 6), ("mass", C.c_double * 6), ("time", C.c_double), ("redshift", C.c_double),
+                    ("flag_sfr", C.c_int32), ("flag_feedback", C.c_int32), ("npartTotal", C.c_uint32 * 6),
+                    ("flag_cooling", C.c_int32), ("num_files", C.c_int32), ("BoxSize", C.c_double), ("Omega0", C.c_double),
+                    ("OmegaLambda", C.c_double), ("HubbleParam", C.c_double), ("flag_stellarage", C.c_int32),
+                    ("flag_metals", C.c_int32), ("hashtabsize", C.c_int32), ("fill", C.c_char * 84)]
+
+    assert C.sizeof(Header) == 256
+    L = lib()
+    L.tg_ngenic_read_header.argtypes = [C.c_char_p, C.POINTER(Header)]
+    L.tg_ngenic_load.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+    h = Header()
+    p = str(path).encode()
+    if L.tg_ngenic_read_header(p, C.byref(h)) != 0:
+        raise TGError(L.tg_last_error().decode(errors="replace"))
+    n = C.c_int64()
+    npart = int(h.npart[1])
+    x = np.zeros((npart, 3), np.int32)
+    v = np.zeros((npart, 3), np.float32)
+    if L.tg_ngenic_load(p, npart, x.ctypes.data, v.ctypes.data, C.byref(n)) != 0:
+        raise TGError(L.tg_last_error().decode(errors="replace"))
+    hd = {name: (list(getattr(h, name)) if name in ("npart", "mass", "npartTotal") else getattr(h, name))
+          for name, _ in Header._fields_ if name != "fill"}
+    return x, v, hd
