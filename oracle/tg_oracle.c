@@ -1498,3 +1498,46 @@ double tgo_drift(const tgo_step_params* P, int64_t n, int32_t* x, const float* v
 	}
 	return ekin;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* friends-of-friends: part_vect_find_groups (src/part_vect.cpp:84-120) over all pairs, repeated
+ * until no particle changes (the reference's groups_execute_finders loop, src/main.cpp:41-45).
+ * pos is tree order; group[i] starts at DEFAULT_GROUP; a first link gives a particle its own index,
+ * then each link takes the smaller of the two ids.  O(n^2) per sweep: small n only.  A pair whose
+ * coordinate difference wraps (across the box edge) is not linked: the reference searches
+ * unwrapped ranges there (src/range.cpp:80-116), which leaves such pairs to the kd boxes' reach. */
+void tgo_fof(int64_t n, const int32_t* pos, uint64_t L2, uint64_t* group) {
+	const uint64_t NONE = 0xFFFFFFFFFFFFFFull;
+	for (int64_t i = 0; i < n; i++) group[i] = NONE;
+	int changed = 1;
+	while (changed) {
+		changed = 0;
+		for (int64_t i = 0; i < n; i++) {
+			for (int64_t j = 0; j < n; j++) {
+				if (j == i) continue;
+				uint64_t d2 = 0;
+				int wraps = 0;
+				for (int d = 0; d < 3; d++) {
+					const int32_t dx = (int32_t) ((uint32_t) pos[3 * i + d] - (uint32_t) pos[3 * j + d]);
+					const uint64_t u = (uint64_t) (int64_t) dx;
+					d2 += u * u;
+					wraps |= (int64_t) pos[3 * i + d] - (int64_t) pos[3 * j + d] != (int64_t) dx;
+				}
+				if (d2 < L2 && d2 != 0 && !wraps) {
+					uint64_t id = group[i];
+					if (id == NONE) {
+						id = (uint64_t) i;
+						group[i] = id;
+						changed = 1;
+					}
+					const uint64_t other = group[j];
+					if (id != other) {
+						const uint64_t g = id < other ? id : other;
+						if (group[i] != g) changed = 1;
+						group[i] = g;
+					}
+				}
+			}
+		}
+	}
+}

@@ -123,6 +123,10 @@ int tgo_kick(const tgo_step_params* P, int64_t n, const tgo_force* f, float* v, 
 /* returns the kinetic energy before the move */
 double tgo_drift(const tgo_step_params* P, int64_t n, int32_t* x, const float* v, int max_rung, double drift_dt, double a1);
 
+
+/* ---- friends-of-friends (src/part_vect.cpp:84-120), all pairs, to convergence ---- */
+void tgo_fof(int64_t n, const int32_t* pos_tree, uint64_t L2, uint64_t* group);
+
 #ifdef __cplusplus
 }
 #endif

@@ -347,3 +347,20 @@ def drift(P, x, v, max_rung, drift_dt=0.0, a1=1.0):
     Q = step_params(P)
     v = np.ascontiguousarray(v, np.float32)
     return L.tgo_drift(C.byref(Q), x.shape[0], _p(x), _p(v), int(max_rung), float(drift_dt), float(a1))
+
+
+def fof(pos_tree, L2):
+    """tgo_fof: the reference's friends-of-friends relabelling to convergence (tree-order ids)."""
+    L = lib()
+    L.tgo_fof.argtypes = [C.c_int64, C.c_void_p, C.c_uint64, C.c_void_p]
+    L.tgo_fof.restype = None
+    pos = np.ascontiguousarray(pos_tree, np.int32)
+    g = np.zeros(pos.shape[0], np.uint64)
+    L.tgo_fof(pos.shape[0], _p(pos), int(L2), _p(g))
+    return g
+
+
+def fof_link2(n, link_len):
+    """The reference's integer squared linking length (src/part_vect.cpp:86-88)."""
+    L = int((n ** (-1.0 / 3.0) * link_len) * 4294967295.0)
+    return L * L

@@ -18,6 +18,7 @@
 #include "tg_local.hpp"
 #include "tg_solver.hpp"
 #include "tg_devtree.hpp"
+#include "tg_fof.hpp"
 #include "tg_upload.hpp"
 #include "tg_walk.hpp"
 #include "tg_solver.h"
@@ -331,6 +332,7 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 	}
 	drop_graph();  // the captured launches point at buffers that may be reallocated
 	order_on_device = false;
+	fof_group.clear();
 	n = count;
 	TreeParams P;
 	P.ewald = ewald;
@@ -882,6 +884,32 @@ int tg_solver_set_particles_device(tg_solver* h, const int32_t* x, const float* 
 int tg_solver_get_forces_device(tg_solver* h, tg_force* f) {
 	if (!h || h->s.device < 0 || (h->s.n > 0 && !f)) return TG_EINVAL;
 	return fail(h, h->s.forces_device(reinterpret_cast<double*>(f)));
+}
+
+int tg_solver_find_groups(tg_solver* h, double link_len, uint64_t* group, int64_t* n_groups) {
+	if (!h || h->s.device < 0 || (h->s.n > 0 && !group) || !(link_len > 0.0)) return TG_EINVAL;
+	tg::Solver& S = h->s;
+	const int64_t n = S.n;
+	S.fof_group.assign(size_t(n), TG_FOF_NONE);
+	int64_t cand = 0;
+	const hipError_t e = tg::fof_device(S.tree, S.d_parts.as<const int4>(), link_len, S.device, S.stream, S.fof_group.data(), &cand, S.err);
+	if (e != hipSuccess) return fail(h, e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP);
+	int64_t ng = 0;
+	for (int64_t i = 0; i < n; i++) {
+		group[S.tree.order[size_t(i)]] = S.fof_group[size_t(i)];
+		ng += S.fof_group[size_t(i)] == uint64_t(i);
+	}
+	if (n_groups) *n_groups = ng;
+	return TG_OK;
+}
+
+int tg_solver_group_catalog(tg_solver* h, const float* v, double a, int32_t min_n, tg_group* out, int64_t capacity, int64_t* n_out) {
+	if (!h || !n_out || !(a > 0.0) || (h->s.n > 0 && !v) || capacity < 0) return TG_EINVAL;
+	if (h->s.fof_group.size() != size_t(h->s.n)) {
+		g_solver_err = "tg_solver_group_catalog before tg_solver_find_groups";
+		return TG_EINVAL;
+	}
+	return tg::group_catalog(h->s.tree, h->s.fof_group, v, a, min_n, out, capacity, n_out);
 }
 
 int tg_solver_solve(tg_solver* h) {

@@ -79,6 +79,7 @@ struct Solver {
 	// host-only solvers (device < 0) keep the PP-direct items and flattened source list
 	hvec<int4> host_pp_items;
 	std::vector<int> host_pp_list;
+	std::vector<uint64_t> fof_group;  // friends-of-friends group per particle, tree order (find_groups)
 	int n_leaf_items = 0;
 	int64_t n_slots = 0;
 	int64_t list_bytes = 0;
