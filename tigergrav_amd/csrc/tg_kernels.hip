@@ -159,7 +159,9 @@ __device__ __forceinline__ bool off_big(float ox, float oy, float oz) {
 // Lane = (sink si, phase ph) with P = 64 / ns phases (lanes from P ns on idle), so a leaf of ns
 // sinks keeps P ns of the 64 lanes busy.  A 64-source tile is staged phase-major in LDS: phase ph
 // takes sources ph, ph + P, ... from K = ceil(64 / P) consecutive slots (slots no source fills hold
-// zero-weight pads), so every lane walks its slots with a wave-uniform trip count.  Per-tile partial
+// zero-weight pads), so every lane walks its slots with a wave-uniform trip count.  The phases
+// start an odd number of 16-byte slots apart, so the P addresses one read instruction broadcasts
+// lie in different LDS banks.  Per-tile partial
 // sums are float, the running sums FP64; the phases of a sink are added in phase order through LDS.
 // Tiles of the near prefix take separations as exact integer differences rounded once
 // (src/gravity.cpp:221-229) and the softened pair.  Far tiles take every separation as the
@@ -183,6 +185,9 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, PPShar
 	const int near_end = near_len == PP_NEAR_ALL ? INT_MAX : it.z + near_len;
 	const int P = 64 / ns;             // phases
 	const int K = (64 + P - 1) / P;    // slots per phase
+	// phase stride in the tile: odd, so the phases' broadcast reads in one instruction fall on
+	// different 4-bank groups of the LDS (conflict-free for up to 8 phases)
+	const int KS = K | 1;
 	const int ph = min(lane / ns, P - 1);
 	const int si = lane - (lane / ns) * ns;
 	const int4 me = A.sinks[it.x + si];
@@ -190,15 +195,15 @@ __device__ __forceinline__ void pp_item(const LeafArgs& A, const int4 it, PPShar
 	const float ox = sepu<WRAP>(me.x, c0.x), oy = sepu<WRAP>(me.y, c0.y), oz = sepu<WRAP>(me.z, c0.z);
 	const bool item_off = !WRAP || __ballot(off_big(ox, oy, oz)) == 0ull;
 	// source l of a tile goes to phase l % P, slot l / P; slots past the tile's sources are pads
-	const int dst = (lane % P) * K + lane / P;
+	const int dst = (lane % P) * KS + lane / P;
 	const int4 pad = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);
 	const float4 fpad = make_float4(3.0e30f, 3.0e30f, 3.0e30f, 0.0f);
 	S.tile[lane] = pad;
 	S.tile[lane + 64] = pad;
 	S.ftile[lane] = fpad;
 	S.ftile[lane + 64] = fpad;
-	const int4* mine = S.tile + ph * K;
-	const float4* fmine = S.ftile + ph * K;
+	const int4* mine = S.tile + ph * KS;
+	const float4* fmine = S.ftile + ph * KS;
 	float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
 	double dax = 0.0, day = 0.0, daz = 0.0, dpp = 0.0;
 	// software pipeline: the next tile's source record (and the list index after it) are loaded
