@@ -40,13 +40,13 @@ typedef struct tg_solve_info {
 int tg_solver_create(const tg_params* p, double theta, int parts_per_node, tg_solver** out);
 int tg_solver_destroy(tg_solver* s);
 /* particles (int32 x3 fixed point, n of them; mass NULL = all p->m): builds the tree and
- * multipoles on the host (tree::refine, compute_multipoles) and the lists (kick_fmm's list
- * construction) on the device or, with "gpu_walk" 0 or a host-only solver, on the host; makes
- * them resident on the device */
+ * multipoles (tree::refine, compute_multipoles) and the lists (kick_fmm's list construction) on
+ * the device or, with "gpu_tree" / "gpu_walk" 0 or a host-only solver, on the host; makes them
+ * resident on the device */
 int tg_solver_set_particles(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
 /* tg_solver_set_particles with x (and mass, if not NULL) in device memory on the solver's device:
- * the tree is built there ("gpu_tree" 1, the default), so a time step needs no host copy of the
- * particles (the reference keeps them in its own memory, src/part_vect.cpp) */
+ * the tree is built there (needs "gpu_tree" 1, the default), so a time step needs no host copy of the particles
+ * (the reference keeps them in its own memory, src/part_vect.cpp) */
 int tg_solver_set_particles_device(tg_solver* s, const int32_t* x, const float* mass, int64_t n);
 /* new positions (and masses, or NULL to keep them) for the same particles, in input order, with
  * the tree, the interaction lists and the expansion centres of the last tg_solver_set_particles
@@ -64,12 +64,16 @@ int tg_solver_adjoint(tg_solver* s, const float* w, double* grad, double* ms);
 /* enqueue one force solve on the solver's stream (replaces tree::kick_fmm's kernel calls) */
 int tg_solver_solve(tg_solver* s);
 /* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent
- * queue-driven kernel; "node_chunk" 1..64 sources per CC/CP partial slot (applies from the next
+ * queue-driven kernel; "node_chunk" 1..512 sources per CC/CP partial slot (applies from the next
  * tg_solver_set_particles); "resident" work groups of the persistent form; "local_sweep" 1 runs the
  * local-expansion pass level by level (the reference's order, src/tree.cpp:400-401) instead of
  * per-leaf sums over the root path (0, the default); "gpu_walk" 1 (the default
  * for device solvers) builds the interaction lists on the device, 0 on the host (the same lists
  * either way: the walk of kick_fmm, src/tree.cpp:378-635; from the next tg_solver_set_particles);
+ * with the lists on the device, the work of every kind is built from them there and the host
+ * receives a copy only when something asks for it (tg_solver_export, the adjoint);
+ * "gpu_tree" 1 (the default for device solvers) builds the tree and multipoles on the device, 0 on
+ * the host (the same tree bit for bit; from the next tg_solver_set_particles);
  * "overlap" 1 (the default) lets graph replays run the node kinds on a second stream beside the
  * leaf kinds (the same results bit for bit), 0 keeps one stream */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);
@@ -91,10 +95,14 @@ int tg_solver_collect_ms(tg_solver* s, double* ms_sum /* [9] */, int64_t* solves
 int tg_solver_info(tg_solver* s, tg_solve_info* info);
 /* copy a named host array of the last build (tree order positions "pos", "order", "mass",
  * "leaves", per node "node_range" "node_child" "node_parent" "node_level" "node_x" "node_m"
- * "node_r" "node_boxid", and the lists "cc_d" "cp_d" "cc_e" "cp_e" "pc_d" "pp_d" "pc_e" "pp_e"
- * with their "_ptr" CSR offsets; for a host-only solver also "pp_d_items", the PP-direct work items
- * (first sink, count | near-prefix length << 7, list begin, list end), and "pp_d_list", their
- * flattened source indices).  Returns the size in bytes; dst may be NULL to query it. */
+ * "node_r" "node_boxid" "node_box" "node_n", and the lists "cc_d" "cp_d" "cc_e" "cp_e" "pc_d"
+ * "pp_d" "pc_e" "pp_e" with their "_ptr" CSR offsets; for a host-only solver also "pp_d_items",
+ * the PP-direct work items (first sink, count | near-prefix length << 7, list begin, list end),
+ * and "pp_d_list", their flattened source indices; for a device solver the device work of kind K
+ * (tg_solve_info order) "work_items_K" (leaf kinds: int4 items as above; node kinds: int2 waves of
+ * list entries), "work_list_K", "work_owner_K" (node kinds: the partial slot of each entry), and
+ * "slot_node", "slot_ptr" (each slot's node; each node's first slot)).  Returns the size in bytes;
+ * dst may be NULL to query it. */
 int64_t tg_solver_export(tg_solver* s, const char* name, void* dst, int64_t capacity);
 const char* tg_solver_last_error(void);
 

@@ -74,6 +74,40 @@ def test_solve_parity(case):
     assert all(v >= 0 for v in ms.values())
 
 
+@pytest.mark.parametrize("chunk", [64, 256, 500])
+def test_solve_parity_long_chunks(chunk):
+    """CC/CP partial slots of more than 64 list entries: one wave per slot whose lanes each add
+    several entries in order before the store.  Periodic Zel'dovich set at theta 0.3 (node lists of
+    hundreds of entries) against the oracle solve on the same lists, at the parity tolerance."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    x = ics.zeldovich(24, 27)
+    N = x.shape[0]
+    h, m = ics.soft_len(N), 1.0 / N
+    print(f"long chunks {chunk}: N={N} theta=0.3 leaf=16", flush=True)
+    s = Solver(ewald=True, soft_len=h, m=m, theta=0.3, parts_per_node=16, device=0)
+    s.tune("node_chunk", chunk)
+    s.set_particles(x)
+    s.solve()
+    s.synchronize()
+    f = s.forces()
+    T = s.tree_arrays()
+    cc = np.diff(T["cc_d_ptr"])
+    assert cc.max() > chunk or chunk == 500  # slots longer than a wave occur
+    ref_tree, _ = ob.oracle_solve({"ewald": 1, "soft_len": h, "m": m}, T)
+    ref = np.empty_like(ref_tree)
+    ref[T["order"]] = ref_tree
+    eg = rms_rel(f[:, 1:], ref[:, 1:])
+    dphi = math.sqrt(np.mean((f[:, 0] - ref[:, 0]) ** 2))
+    print(f"  rms rel g {eg:.2e}  rms |dphi| {dphi:.2e} (total mass 1); longest CC list {cc.max()}", flush=True)
+    # the periodic potential's rounding is bounded in absolute terms, as for the full-size test
+    assert eg < 1e-5 and dphi < 2e-6
+    s.solve_graph()
+    s.synchronize()
+    np.testing.assert_array_equal(s.forces(), f)
+
+
 @pytest.mark.parametrize("case", list(EDGE_CASES))
 def test_solve_edge_cases(case):
     """Small, ragged and coincident particle sets against the oracle solve on the same lists:
@@ -264,13 +298,14 @@ def test_tuning_knobs():
         s.solve_graph()
         s.synchronize()
         np.testing.assert_array_equal(s.forces(), base)
-    s2 = Solver(ewald=True, soft_len=h, m=m, theta=0.7, parts_per_node=64, device=0)
-    s2.tune("node_chunk", 16)
-    s2.set_particles(x)
-    s2.solve()
-    s2.synchronize()
-    f2 = s2.forces()
-    assert rms_rel(f2[:, 1:], base[:, 1:]) < 1e-6
+    for chunk in (16, 200):
+        s2 = Solver(ewald=True, soft_len=h, m=m, theta=0.7, parts_per_node=64, device=0)
+        s2.tune("node_chunk", chunk)
+        s2.set_particles(x)
+        s2.solve()
+        s2.synchronize()
+        f2 = s2.forces()
+        assert rms_rel(f2[:, 1:], base[:, 1:]) < 1e-6
     # the level-by-level local pass shifts the running sum node by node (the reference's order):
     # the same expansion up to FP64 rounding, and graph replays of it bit-identical to eager solves
     s.tune("pp_persistent", 0)
@@ -321,7 +356,7 @@ def test_autotune_picks_the_fastest_tried_configuration():
     h, m = ics.soft_len(n), 1.0 / n
     print(f"autotune: N={n}", flush=True)
     best, table = autotune(x, True, h, m, 0.7, leaves=(32, 64), chunks=(64,), solves=3)
-    assert len(table) == 2 * 1 * 2 * 2
+    assert len(table) == 2 * 1 * 2 * 2 * 2  # leaves x chunks x PP forms x local passes x overlap
     assert all(r["ms"] > 0 for r in table) and best["ms"] == min(r["ms"] for r in table)
     s = Solver(ewald=True, soft_len=h, m=m, theta=0.7, parts_per_node=best["leaf"], device=0)
     apply(s, best)

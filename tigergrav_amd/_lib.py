@@ -35,6 +35,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise TGError(f"{LIB_PATH} is not built; run `python -m tigergrav_amd.build` (or __graft_entry__.build())")
+    # One HIP runtime per process.  torch ships its own libamdhip64 (soname libamdhip64.so.7, but
+    # its libraries ask for it as libamdhip64.so): loaded after this library's /opt/rocm runtime it
+    # would be a second runtime, which cannot initialise the GPU the first one holds.  Loaded
+    # first, its runtime is the one this library's libamdhip64.so.7 resolves to.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i64, ci, u64p = C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_uint64)
     P = C.POINTER(Params)

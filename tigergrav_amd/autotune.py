@@ -1,17 +1,19 @@
-"""Autotuner of the batched solve: picks the leaf size, the CC/CP chunk per partial slot, the
-PP-direct kernel form (one work group per item, or persistent queue-driven) and the form of the
-local-expansion pass (per-leaf path sums, or level by level) on the time per solve of graph
-replays of the caller's own particle set (what the benchmark measures: with the node kinds beside
-the leaf kinds, so the sum of per-kind kernel times would misjudge it)."""
+"""Autotuner of the batched solve: picks the leaf size, the CC/CP chunk per partial slot (chunks
+longer than 64 entries are one wave each, its lanes adding several entries), the PP-direct kernel
+form (one work group per item, or persistent queue-driven), the form of the local-expansion pass
+(per-leaf path sums, or level by level) and whether graph replays run the node kinds on a second
+stream.  The criterion is the time per solve of graph replays of the caller's own particle set
+(what the benchmark measures: with the node kinds beside the leaf kinds, so the sum of per-kind
+kernel times would misjudge it)."""
 import itertools
 import time
 
 from .solver import Solver
 
 
-def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(32, 64), persistent=(0, 1), sweep=(0, 1),
-             solves=10, device=0):
-    """Returns (best, table): best = {"leaf", "node_chunk", "pp_persistent", "local_sweep", "ms"};
+def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(64, 256), persistent=(0, 1), sweep=(0, 1),
+             overlap=(1, 0), solves=10, device=0):
+    """Returns (best, table): best = {"leaf", "node_chunk", "pp_persistent", "local_sweep", "overlap", "ms"};
     table lists every configuration tried with its ms per solve over `solves` graph replays
     (host clock, the device synchronised before and after)."""
     table = []
@@ -19,9 +21,10 @@ def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(
         s = Solver(ewald=ewald, soft_len=soft_len, m=m, theta=theta, parts_per_node=leaf, device=device)
         s.tune("node_chunk", chunk)
         s.set_particles(x)
-        for p, w in itertools.product(persistent, sweep):
+        for p, w, o in itertools.product(persistent, sweep, overlap):
             s.tune("pp_persistent", p)
             s.tune("local_sweep", w)
+            s.tune("overlap", o)
             for _ in range(3):  # capture and warm
                 s.solve_graph()
             s.synchronize()
@@ -30,7 +33,7 @@ def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(
                 s.solve_graph()
             s.synchronize()
             ms = (time.perf_counter() - t0) * 1e3 / solves
-            table.append({"leaf": leaf, "node_chunk": chunk, "pp_persistent": p, "local_sweep": w, "ms": ms})
+            table.append({"leaf": leaf, "node_chunk": chunk, "pp_persistent": p, "local_sweep": w, "overlap": o, "ms": ms})
         s.close()
     best = min(table, key=lambda r: r["ms"])
     return dict(best), table
@@ -39,6 +42,6 @@ def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(
 def apply(solver, best):
     """Set a tuned configuration's knobs on a solver built with best["leaf"] (before set_particles);
     knobs missing from `best` keep their defaults."""
-    for knob in ("node_chunk", "pp_persistent", "local_sweep"):
+    for knob in ("node_chunk", "pp_persistent", "local_sweep", "overlap"):
         if best.get(knob) is not None:
             solver.tune(knob, best[knob])

@@ -515,64 +515,71 @@ __device__ __forceinline__ void node_segment_store(const float* L, int slot, int
 	}
 }
 
+// A wave takes a run of list entries: up to 64 entries of one or more partial slots (one entry per
+// lane), or the entries of one slot beyond 64 (a chunk longer than a wave), which the lanes take
+// 64 apart, each adding its entries' terms in order before the store.
 template <bool EWALD, bool WRAP>
 __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 	const int2 wv = A.waves[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int j = wv.x + lane;
 	float L[35];
 #pragma unroll
 	for (int c = 0; c < 35; c++) L[c] = 0.f;
 	int slot = -1;
+	int j = wv.x + lane;
 	if (j < wv.y) {
-		slot = A.owner ? A.owner[j] : int(blockIdx.x);
+		slot = A.owner ? A.owner[j] : int(blockIdx.x);  // the plain drop-in: one slot per wave, one sink
 		const int4 X = A.sinkpos[A.slot_node ? A.slot_node[slot] : 0];
-		const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[j] : j));
-		float v[20];
+		for (; j < wv.y; j += 64) {
+			const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[j] : j));
+			float v[20];
 #pragma unroll
-		for (int q = 0; q < 5; q++) {
-			const float4 t = src[q];
-			v[4 * q] = t.x;
-			v[4 * q + 1] = t.y;
-			v[4 * q + 2] = t.z;
-			v[4 * q + 3] = t.w;
+			for (int q = 0; q < 5; q++) {
+				const float4 t = src[q];
+				v[4 * q] = t.x;
+				v[4 * q + 1] = t.y;
+				v[4 * q + 2] = t.z;
+				v[4 * q + 3] = t.w;
+			}
+			const float x[3] = {sepu<EWALD || WRAP>(X.x, __float_as_int(v[17])) * POS_INVF,
+					sepu<EWALD || WRAP>(X.y, __float_as_int(v[18])) * POS_INVF,
+					sepu<EWALD || WRAP>(X.z, __float_as_int(v[19])) * POS_INVF};
+			float D[35];
+			if (EWALD)
+				green_ewald(D, x);
+			else
+				green_direct(D, x);
+			m2l(L, v, D);
 		}
-		const float x[3] = {sepu<EWALD || WRAP>(X.x, __float_as_int(v[17])) * POS_INVF,
-				sepu<EWALD || WRAP>(X.y, __float_as_int(v[18])) * POS_INVF,
-				sepu<EWALD || WRAP>(X.z, __float_as_int(v[19])) * POS_INVF};
-		float D[35];
-		if (EWALD)
-			green_ewald(D, x);
-		else
-			green_direct(D, x);
-		m2l(L, v, D);
 	}
-	node_segment_store(L, slot, wv.y - wv.x, A.slot_node, A.partial);
+	node_segment_store(L, slot, min(wv.y - wv.x, 64), A.slot_node, A.partial);
 }
 
 template <bool EWALD, bool WRAP>
 __global__ __launch_bounds__(64) void k_cp(NodeArgs A) {
 	const int2 wv = A.waves[blockIdx.x];
 	const int lane = threadIdx.x;
-	const int j = wv.x + lane;
 	float L[35];
 #pragma unroll
 	for (int c = 0; c < 35; c++) L[c] = 0.f;
 	int slot = -1;
+	int j = wv.x + lane;
 	if (j < wv.y) {
-		slot = A.owner ? A.owner[j] : int(blockIdx.x);
+		slot = A.owner ? A.owner[j] : int(blockIdx.x);  // the plain drop-in: one slot per wave, one sink
 		const int4 X = A.sinkpos[A.slot_node ? A.slot_node[slot] : 0];
-		const int4 s = A.srcs[A.list ? A.list[j] : j];
-		const float x[3] = {sepu<EWALD || WRAP>(X.x, s.x) * POS_INVF, sepu<EWALD || WRAP>(X.y, s.y) * POS_INVF,
-				sepu<EWALD || WRAP>(X.z, s.z) * POS_INVF};
-		float D[35];
-		if (EWALD)
-			green_ewald(D, x);
-		else
-			green_direct(D, x);
-		p2l(L, __int_as_float(s.w), D);
+		for (; j < wv.y; j += 64) {
+			const int4 s = A.srcs[A.list ? A.list[j] : j];
+			const float x[3] = {sepu<EWALD || WRAP>(X.x, s.x) * POS_INVF, sepu<EWALD || WRAP>(X.y, s.y) * POS_INVF,
+					sepu<EWALD || WRAP>(X.z, s.z) * POS_INVF};
+			float D[35];
+			if (EWALD)
+				green_ewald(D, x);
+			else
+				green_direct(D, x);
+			p2l(L, __int_as_float(s.w), D);
+		}
 	}
-	node_segment_store(L, slot, wv.y - wv.x, A.slot_node, A.partial);
+	node_segment_store(L, slot, min(wv.y - wv.x, 64), A.slot_node, A.partial);
 }
 
 hipError_t launch_cc(const NodeArgs& a, bool ewald, hipStream_t s) {

@@ -7,10 +7,11 @@
 // y = count | near_len << PP_NEAR_SHIFT, with near_len == PP_NEAR_ALL meaning the whole list.
 // Results are added to out[4 * sink + {0:phi, 1..3:g}] (FP64; the item owns its sinks, so no
 // atomics are needed while kernels of one stream run in order).
-// Node kinds (CC/CP: the sink is an expansion centre): a wave takes a run of <= 64 consecutive list
-// entries, waves[w] = [begin, end); entry j belongs to partial slot owner[j], whose expansion centre
-// is sinkpos[slot_node[slot]].  Every slot's entries lie inside one wave, so each slot's 35 FP64
-// coefficients are written exactly once (no atomics, deterministic).
+// Node kinds (CC/CP: the sink is an expansion centre): a wave takes a run of consecutive list
+// entries, waves[w] = [begin, end): at most 64 of one or more partial slots, or any number of one
+// slot (lane l then takes entries begin + l + 64 k); entry j belongs to partial slot owner[j],
+// whose expansion centre is sinkpos[slot_node[slot]].  Every slot's entries lie inside one wave,
+// so each slot's 35 FP64 coefficients are written exactly once (no atomics, deterministic).
 // `list` holds source indices (particles for PP/CP, multipoles for PC/CC); NULL means the
 // identity list.  For node kinds owner == NULL means one slot per wave (slot = wave index) and
 // slot_node == NULL means every slot's centre is sinkpos[0].

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "tg_gravity.h"
@@ -26,7 +27,7 @@
 namespace tg {
 
 hipError_t DeviceArray::alloc(size_t n) {
-	if (n <= bytes && p) return hipSuccess;
+	if (n <= bytes && p && owned) return hipSuccess;
 	release();
 	if (n == 0) return hipSuccess;
 	hipError_t e = hipMalloc(&p, n);
@@ -35,9 +36,10 @@ hipError_t DeviceArray::alloc(size_t n) {
 }
 
 void DeviceArray::release() {
-	if (p) hipFree(p);
+	if (p && owned) hipFree(p);
 	p = nullptr;
 	bytes = 0;
+	owned = true;
 }
 
 // ---- kernels ---------------------------------------------------------------------------------
@@ -225,6 +227,7 @@ int Solver::init(int dev) {
 	}
 	SCHK(hipSetDevice(dev));
 	gpu_walk = true;  // the lists on the device by default (bit-identical to the host walk)
+	gpu_tree = true;  // the tree and multipoles too (bit-identical to the host build)
 	SCHK(init_device_tables(dev));
 	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 	SCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
@@ -315,6 +318,7 @@ Solver::~Solver() {
 			w.list.release();
 			w.owner.release();
 		}
+		dlists.release();
 		if (stream2) {
 			hipStreamSynchronize(stream2);
 			hipStreamDestroy(stream2);
@@ -354,34 +358,18 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 		tree.compute_multipoles();
 		t2 = now();
 	}
+	// the work lists of the last build may be views of the walk's lists, released below
+	for (auto& w : work)
+		if (!w.list.owned) w.list.release();
+	lists_on_device = false;
 	if (gpu_walk) {
-		// the CC lists stay on the device as the work lists (the host reads only their pointers);
-		// host_lists() copies them back when something asks for the values
-		int32_t* kept[8] = {};
-		const hipError_t e = device_walk(tree, device, stream, err, t_detail, (1u << 0) | (1u << 2), kept);
-		const int kinds[2] = {CC_DIRECT, CC_EWALD};
-		cc_on_device = false;
-		for (int q = 0; q < 2; q++) {
-			DeviceArray& d = work[kinds[q]].list;
-			d.release();
-			const hvec<int64_t>& ptr = q ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
-			if (e == hipSuccess && kept[2 * q] && !ptr.empty()) {
-				d.p = kept[2 * q];
-				// at least 1024 entries are allocated, so claiming 16 bytes for a shorter list is safe
-				d.bytes = std::max<size_t>(size_t(ptr.back()) * 4, 16);
-			} else if (kept[2 * q]) {
-				(void) hipFree(kept[2 * q]);
-			}
-		}
+		// the lists stay on the device; the work is built from them there (upload_lists_device)
+		const hipError_t e = device_walk(tree, device, stream, err, t_detail, &dlists);
 		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
-		if (!work[CC_DIRECT].list.p || !work[CC_EWALD].list.p) {
-			err = "device walk returned no CC lists";
-			return TG_EHIP;
-		}
-		cc_on_device = true;
+		lists_on_device = true;
 	} else {
+		dlists.release();
 		tree.walk();
-		cc_on_device = false;
 	}
 	double t3 = now();
 	t_tree = t1 - t0;
@@ -498,17 +486,21 @@ static hipError_t put(DeviceArray& d, const std::vector<T, A>& h, hipStream_t s)
 }
 
 int Solver::host_lists() {
-	if (!cc_on_device) return TG_OK;
+	if (!lists_on_device) return TG_OK;
 	SCHK(hipSetDevice(device));
-	const int kinds[2] = {CC_DIRECT, CC_EWALD};
-	for (int q = 0; q < 2; q++) {
-		hvec<int32_t>& v = q ? tree.lists.cc_e : tree.lists.cc_d;
-		const hvec<int64_t>& ptr = q ? tree.lists.cc_e_ptr : tree.lists.cc_d_ptr;
-		v.resize(ptr.empty() ? 0 : size_t(ptr.back()));
-		if (!v.empty()) SCHK(hipMemcpyAsync(v.data(), work[kinds[q]].list.p, v.size() * 4, hipMemcpyDeviceToHost, stream));
+	Lists& Ls = tree.lists;
+	hvec<int64_t>* ptrs[8] = {&Ls.cc_d_ptr, &Ls.cp_d_ptr, &Ls.cc_e_ptr, &Ls.cp_e_ptr, &Ls.pc_d_ptr, &Ls.pp_d_ptr, &Ls.pc_e_ptr, &Ls.pp_e_ptr};
+	hvec<int32_t>* vals[8] = {&Ls.cc_d, &Ls.cp_d, &Ls.cc_e, &Ls.cp_e, &Ls.pc_d, &Ls.pp_d, &Ls.pc_e, &Ls.pp_e};
+	const int64_t np = dlists.nn + 1;
+	for (int k = 0; k < 8; k++) {
+		ptrs[k]->resize(size_t(np));
+		vals[k]->resize(size_t(dlists.len[k]));
+		SCHK(hipMemcpyAsync(ptrs[k]->data(), dlists.ptr(k), size_t(np) * 8, hipMemcpyDeviceToHost, stream));
+		if (dlists.len[k] > 0)
+			SCHK(hipMemcpyAsync(vals[k]->data(), dlists.val[k], size_t(dlists.len[k]) * 4, hipMemcpyDeviceToHost, stream));
 	}
 	SCHK(hipStreamSynchronize(stream));
-	cc_on_device = false;
+	lists_on_device = false;
 	return TG_OK;
 }
 
@@ -518,8 +510,10 @@ int Solver::upload() {
 	const int64_t nn = int64_t(T.nodes.size());
 	HostWork W;
 	const double tp0 = now();
+	const bool device_lists = lists_on_device && device >= 0;
 	{
-		const int rc = prepare_work(T, n, ewald, soft_len, node_chunk, W, err);
+		// with the lists on the device only the tree's arrays are made here
+		const int rc = device_lists ? prepare_tree_work(T, n, W, err) : prepare_work(T, n, ewald, soft_len, node_chunk, W, err);
 		if (rc != TG_OK) return rc;
 	}
 	t_detail[4] = now() - tp0;
@@ -536,10 +530,8 @@ int Solver::upload() {
 	auto* waves = W.waves;
 	const int64_t* list_len = W.list_len;
 	const hvec<int32_t>* walk_list[NKINDS] = {};  // CC kinds: the walk's flat list, node order
-	bool on_device[NKINDS] = {};                   // the device walk left it in work[k].list
 	walk_list[CC_DIRECT] = &T.lists.cc_d;
 	walk_list[CC_EWALD] = &T.lists.cc_e;
-	on_device[CC_DIRECT] = on_device[CC_EWALD] = cc_on_device;
 	auto& parts = W.parts;
 	auto& multis = W.multis;
 	auto& npos = W.npos;
@@ -559,11 +551,9 @@ int Solver::upload() {
 	SCHK(put(d_parts, parts, s));
 	SCHK(put(d_multis, multis, s));
 	SCHK(put(d_nodepos, npos, s));
-	SCHK(put(d_node_slot_ptr, slot_ptr, s));
 	SCHK(put(d_leaf_items, leaf_items, s));
 	adj_checked = false;
 	adj_lists_ready = false;
-	SCHK(put(d_slot_node, slot_node, s));
 	SCHK(put(d_path, path, s));
 	SCHK(put(d_level, level, s));
 	{
@@ -585,6 +575,21 @@ int Solver::upload() {
 		SCHK(put(d_pleaf, pleaf, s));
 		SCHK(d_Lnode.alloc(std::max<size_t>(size_t(nn) * 35 * sizeof(double), 16)));
 	}
+	if (device_lists) {
+		SCHK(hipStreamSynchronize(s));  // the host arrays above are freed on return
+		const double tp1 = now();
+		const int rc = upload_lists_device();
+		if (rc != TG_OK) return rc;
+		t_detail[5] = tp1 - tp0 - t_detail[4];  // transfers: the tree's arrays
+		t_detail[4] += now() - tp1;            // preparation: the tree's arrays (host), the list work (device)
+		SCHK(d_partial.alloc(std::max<size_t>(n_slots * 35 * sizeof(double), 16)));
+		SCHK(d_dL.alloc(std::max<size_t>(T.nodes.size() * 35 * sizeof(double), 16)));
+		SCHK(d_out.alloc(std::max<size_t>(n * 4 * sizeof(double), 16)));
+		SCHK(hipStreamSynchronize(s));
+		return TG_OK;
+	}
+	SCHK(put(d_node_slot_ptr, slot_ptr, s));
+	SCHK(put(d_slot_node, slot_node, s));
 	list_bytes = 0;
 	// run tables stay allocated until the stream has drained below
 	std::vector<DeviceArray> run_tables;
@@ -614,7 +619,7 @@ int Solver::upload() {
 			list_bytes += int64_t(items[k].size() * sizeof(int4));
 		}
 		if (e == hipSuccess) e = work[k].list.alloc(std::max<size_t>(lbytes, 16));
-		if (e == hipSuccess && lbytes > 0 && !on_device[k]) {
+		if (e == hipSuccess && lbytes > 0) {
 			if (walk_list[k])
 				e = hipMemcpyAsync(work[k].list.p, walk_list[k]->data(), lbytes, hipMemcpyHostToDevice, s);
 			else if (!lists[k].empty())
@@ -924,7 +929,7 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 	if (k == "pp_persistent") {
 		S.pp_persistent = value != 0;
 	} else if (k == "node_chunk") {
-		if (value < 1 || value > 64) return TG_EINVAL;
+		if (value < 1 || value > tg::NODE_CHUNK_MAX) return TG_EINVAL;
 		S.node_chunk = value;
 	} else if (k == "resident") {
 		if (value < 1) return TG_EINVAL;
@@ -951,6 +956,8 @@ int tg_solver_adjoint(tg_solver* h, const float* w, double* grad, double* ms) {
 	if (!h || h->s.device < 0 || (h->s.n > 0 && (!w || !grad))) return TG_EINVAL;
 	tg::Solver& S = h->s;
 	if (!S.adj_checked) {
+		const int rl = S.host_lists();  // the symmetry check and the adjoint's transposes read the lists
+		if (rl != TG_OK) return fail(h, rl);
 		S.adj_pp_symmetric = S.pp_lists_symmetric();
 		S.adj_checked = true;
 	}
@@ -1083,6 +1090,47 @@ int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capa
 	TG_EXPORT_LIST(pp_d)
 	TG_EXPORT_LIST(pc_e)
 	TG_EXPORT_LIST(pp_e)
+	else if (nm.rfind("work_", 0) == 0 || nm == "slot_node" || nm == "slot_ptr") {
+		// the device work of the last build: work_items_K, work_list_K, work_owner_K (K a kind index)
+		tg::Solver& S = h->s;
+		const tg::DeviceArray* a = nullptr;
+		size_t bytes = 0;
+		if (S.device < 0) return TG_EINVAL;
+		if (nm == "slot_node") {
+			a = &S.d_slot_node;
+			bytes = size_t(S.n_slots) * 4;
+		} else if (nm == "slot_ptr") {
+			a = &S.d_node_slot_ptr;
+			bytes = (nn + 1) * 8;
+		} else {
+			const size_t us = nm.rfind('_');
+			const int k = us == std::string::npos ? -1 : std::atoi(nm.c_str() + us + 1);
+			if (k < 0 || k >= tg::NKINDS) return TG_EINVAL;
+			const std::string what = nm.substr(0, us);
+			const bool node_kind = k == tg::CC_DIRECT || k == tg::CP_DIRECT || k == tg::CC_EWALD || k == tg::CP_EWALD;
+			const tg::KindWork& w = S.work[k];
+			if (what == "work_items") {
+				a = &w.items;
+				bytes = size_t(w.nitems) * (node_kind ? sizeof(int2) : sizeof(int4));
+			} else if (what == "work_list") {
+				a = &w.list;
+				bytes = size_t(w.nlist) * 4;
+			} else if (what == "work_owner" && node_kind) {
+				a = &w.owner;
+				bytes = size_t(w.nlist) * 4;
+			} else {
+				return TG_EINVAL;
+			}
+		}
+		buf.resize(bytes);
+		if (bytes) {
+			if (hipSetDevice(S.device) != hipSuccess || !a->p ||
+					hipMemcpy(buf.data(), a->p, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+				g_solver_err = "tg_solver_export: device copy failed";
+				return TG_EHIP;
+			}
+		}
+	}
 	else if (nm == "pp_d_items") put(h->s.host_pp_items.data(), h->s.host_pp_items.size() * sizeof(int4));
 	else if (nm == "pp_d_list") put(h->s.host_pp_list.data(), h->s.host_pp_list.size() * 4);
 #undef TG_EXPORT_LIST

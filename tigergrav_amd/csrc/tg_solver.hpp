@@ -9,12 +9,14 @@
 #include "tg_kernels.hpp"
 #include "tg_tree.hpp"
 #include "tg_upload.hpp"
+#include "tg_walk.hpp"
 
 namespace tg {
 
 struct DeviceArray {
 	void* p = nullptr;
 	size_t bytes = 0;
+	bool owned = true;  // false: a view of memory another object frees (release only forgets it)
 	hipError_t alloc(size_t n);
 	void release();
 	template <class T>
@@ -39,7 +41,7 @@ struct Solver {
 	double soft_len = 0.0, m = 0.0, theta = 0.7;
 	int parts_per_node = 64;
 	bool uniform_mass = true;
-	int node_chunk = 64;  // sources per CC/CP partial slot (<= 64); applies from the next set_particles
+	int node_chunk = 64;  // sources per CC/CP partial slot (<= NODE_CHUNK_MAX); applies from the next set_particles
 	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
 	bool overlap = true;         // graph replays run the node kinds on a second stream beside the leaf kinds
 	hipStream_t stream2 = nullptr;
@@ -83,8 +85,12 @@ struct Solver {
 	int n_leaf_items = 0;
 	int64_t n_slots = 0;
 	int64_t list_bytes = 0;
+	// the device walk's lists (gpu_walk): the work of every kind is built from them on the device
+	// (upload_lists_device), the CC and PC work lists are views of them, and the host copy in
+	// tree.lists is made only on demand (host_lists)
+	DeviceLists dlists;
+	bool lists_on_device = false;  // tree.lists holds no copy of dlists yet
 	// timings (seconds) of the last build and solve
-	bool cc_on_device = false;  // the CC list values are only in work[CC_*].list (device walk)
 	double t_tree = 0, t_multipoles = 0, t_walk = 0, t_upload = 0;
 	double t_detail[6] = {};  // walk: set-up, steps, assembly, census; upload: preparation, transfers
 	// per-solve event sets: a ring of TIMER_RING solves, each 2 events per timed span (8 kinds,
@@ -99,7 +105,8 @@ struct Solver {
 	int init(int dev);
 	int set_particles(const int32_t* x, const float* mass, int64_t count, bool on_device = false);
 	int upload();
-	int host_lists();  // copies list values the device walk kept on the device into tree.lists
+	int upload_lists_device();  // every kind's work from dlists, on the device (tg_prep.hip)
+	int host_lists();  // copies the device walk's lists into tree.lists
 	int upload_sources();
 	int update_particles(const int32_t* x, const float* mass);
 	// gradients of sum_i w_i . g_i (w: 3 per particle, input order) with respect to positions and

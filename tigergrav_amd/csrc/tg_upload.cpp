@@ -30,7 +30,7 @@ void walk_nodes(const Tree& T, WalkNode* out) {
 	}
 }
 
-int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err) {
+int prepare_tree_work(const Tree& T, int64_t n, HostWork& W, std::string& err) {
 	const int64_t nn = int64_t(T.nodes.size());
 	// particles (x, y, z, mass bits), tree order
 	hvec<int4> parts(static_cast<size_t>(n));
@@ -49,8 +49,47 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 		std::memcpy(multis[i].x, nd.x, sizeof(nd.x));
 		npos[i] = make_int4(nd.x[0], nd.x[1], nd.x[2], 0);
 	}
+	// non-empty leaves in node order: (node, first particle, end, root-path offset), and the
+	// root-to-leaf paths of the local-expansion pass
+	hvec<int4> leaf_items;
+	for (int32_t lf : T.leaves)
+		if (!T.nodes[lf].empty()) leaf_items.push_back(make_int4(lf, T.nodes[lf].pbegin, T.nodes[lf].pend, 0));
+	const int64_t nl = int64_t(leaf_items.size());
+	hvec<int> path, level(static_cast<size_t>(nn));
+	for (int64_t i = 0; i < nn; i++) level[size_t(i)] = T.nodes[size_t(i)].level;
+	{
+		std::vector<int64_t> poff(size_t(nl) + 1, 0);
+		for (int64_t i = 0; i < nl; i++) poff[size_t(i) + 1] = poff[size_t(i)] + T.nodes[leaf_items[size_t(i)].x].level + 1;
+		if (poff[size_t(nl)] > int64_t(INT32_MAX)) {
+			err = "root-to-leaf paths exceed 2^31 entries";
+			return TG_EINVAL;
+		}
+		path.resize(size_t(poff[size_t(nl)]));
+#pragma omp parallel for schedule(static)
+		for (int64_t i = 0; i < nl; i++) {
+			leaf_items[size_t(i)].w = int(poff[size_t(i)]);
+			int64_t p = poff[size_t(i) + 1];
+			for (int a = leaf_items[size_t(i)].x; a >= 0; a = T.nodes[a].parent) path[size_t(--p)] = a;
+		}
+	}
+	W.parts = std::move(parts);
+	W.multis = std::move(multis);
+	W.npos = std::move(npos);
+	W.leaf_items = std::move(leaf_items);
+	W.path = std::move(path);
+	W.level = std::move(level);
+	return TG_OK;
+}
+
+int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err) {
+	const int64_t nn = int64_t(T.nodes.size());
+	{
+		const int rc = prepare_tree_work(T, n, W, err);
+		if (rc != TG_OK) return rc;
+	}
 	// node kinds: each node's list is cut into chunks of <= node_chunk entries, one partial slot per
-	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64.
+	// chunk (a node's slots are contiguous); consecutive chunks are packed into waves of <= 64
+	// entries, and a chunk longer than 64 is a wave of its own.
 	// Source ranges (CP, PP) are not flattened here: each becomes a run that the device expands
 	// (k_expand_runs), as do the partial-slot owners of the node kinds' entries.  Node and leaf
 	// work are built in three passes: per-node (per-leaf) counts in parallel, offsets by a serial
@@ -65,7 +104,7 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 	hvec<int> slot_node;
 	hvec<int64_t> slot_ptr(static_cast<size_t>(nn + 1));
 	slot_ptr[0] = 0;
-	const int chunk = std::min(64, std::max(1, node_chunk));
+	const int chunk = std::min(NODE_CHUNK_MAX, std::max(1, node_chunk));
 	struct KindSrc {
 		int kind;
 		const hvec<int64_t>* ptr;
@@ -188,10 +227,12 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 		}
 	hvec<int2> waves[NKINDS];
 	for (int k : {CC_DIRECT, CP_DIRECT, CC_EWALD, CP_EWALD}) {
+		// greedy, restarted every WAVE_SEGMENT chunks (so the device can pack segments in parallel)
 		int2 cur = make_int2(0, 0);
 		bool open = false;
-		for (const int2& c : chunks[k]) {
-			if (open && c.y - cur.x > 64) {
+		for (size_t q = 0; q < chunks[k].size(); q++) {
+			const int2& c = chunks[k][q];
+			if (open && (q % WAVE_SEGMENT == 0 || c.y - cur.x > 64)) {
 				waves[k].push_back(cur);
 				open = false;
 			}
@@ -313,12 +354,10 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			items[kind].resize(size_t(lit[q]));
 		}
 	}
-	hvec<int4> leaf_items(static_cast<size_t>(nl));
 #pragma omp parallel for schedule(dynamic, 64)
 	for (int64_t i = 0; i < nl; i++) {
 		const int32_t lf = lvs[size_t(i)];
 		const Node& nd = T.nodes[lf];
-		leaf_items[size_t(i)] = make_int4(lf, nd.pbegin, nd.pend, 0);
 		for (int q = 0; q < 4; q++) {
 			const KindSrc& k = lsrc[q];
 			const size_t o = size_t(4 * i + q);
@@ -365,23 +404,6 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			err = "an interaction list exceeds 2^31 entries (32-bit list offsets)";
 			return TG_EINVAL;
 		}
-	// root-to-leaf paths for the local-expansion pass
-	hvec<int> path, level(static_cast<size_t>(nn));
-	for (int64_t i = 0; i < nn; i++) level[size_t(i)] = T.nodes[size_t(i)].level;
-	{
-		std::vector<int64_t> poff(size_t(nl) + 1, 0);
-		for (int64_t i = 0; i < nl; i++) poff[size_t(i) + 1] = poff[size_t(i)] + T.nodes[lvs[size_t(i)]].level + 1;
-		path.resize(size_t(poff[size_t(nl)]));
-#pragma omp parallel for schedule(static)
-		for (int64_t i = 0; i < nl; i++) {
-			leaf_items[size_t(i)].w = int(poff[size_t(i)]);
-			int64_t p = poff[size_t(i) + 1];
-			for (int a = leaf_items[size_t(i)].x; a >= 0; a = T.nodes[a].parent) path[size_t(--p)] = a;
-		}
-	}
-	W.parts = std::move(parts);
-	W.multis = std::move(multis);
-	W.npos = std::move(npos);
 	for (int k = 0; k < NKINDS; k++) {
 		W.items[k] = std::move(items[k]);
 		W.lists[k] = std::move(lists[k]);
@@ -392,9 +414,6 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 	}
 	W.slot_node = std::move(slot_node);
 	W.slot_ptr = std::move(slot_ptr);
-	W.leaf_items = std::move(leaf_items);
-	W.path = std::move(path);
-	W.level = std::move(level);
 	return TG_OK;
 }
 

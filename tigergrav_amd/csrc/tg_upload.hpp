@@ -38,6 +38,17 @@ struct HostWork {
 	int64_t n_slots = 0;
 };
 
+// node kinds: list entries per partial slot at most (a chunk longer than 64 is one wave whose lanes
+// take its entries 64 apart)
+constexpr int NODE_CHUNK_MAX = 512;
+
+// node kinds' chunks are packed into waves greedily, the packing restarted every WAVE_SEGMENT
+// chunks (host and device pack alike)
+constexpr int WAVE_SEGMENT = 256;
+
+// the tree's part of W: parts, multis, npos, leaf_items, path, level; TG_OK or a negative status
+int prepare_tree_work(const Tree& T, int64_t n, HostWork& W, std::string& err);
+
 // builds W from the tree and its lists (n particles; node_chunk entries per CC/CP partial slot);
 // TG_OK or a negative status with err set
 int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_chunk, HostWork& W, std::string& err);

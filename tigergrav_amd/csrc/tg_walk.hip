@@ -222,10 +222,21 @@ __global__ __launch_bounds__(64) void k_walk_sizes(const WalkNode* __restrict__ 
 	for (int k = 0; k < NCOL; k++) sizes[int64_t(k) * nu + i] = z[k];
 }
 
-// every column's total: exclusive offset plus size of the last unit
+// the sizes of all columns are scanned as one sequence: every column's start (tot[NCOL + k]) and
+// total (tot[k])
 __global__ void k_walk_totals(const int64_t* __restrict__ sizes, const int64_t* __restrict__ offs, int nu, int64_t* __restrict__ tot) {
 	const int k = threadIdx.x;
-	if (k < NCOL) tot[k] = nu > 0 ? offs[int64_t(k) * nu + nu - 1] + sizes[int64_t(k) * nu + nu - 1] : 0;
+	if (k < NCOL) {
+		const int64_t c0 = offs[int64_t(k) * nu];
+		tot[NCOL + k] = c0;
+		tot[k] = offs[int64_t(k) * nu + nu - 1] + sizes[int64_t(k) * nu + nu - 1] - c0;
+	}
+}
+
+// offsets within each column: the one-sequence scan less the column's start
+__global__ __launch_bounds__(256) void k_walk_colfix(const int64_t* __restrict__ tot, int nu, int64_t* __restrict__ offs) {
+	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+	if (i < int64_t(NCOL) * nu) offs[i] -= tot[NCOL + i / nu];
 }
 
 struct WalkPools {
@@ -308,21 +319,25 @@ __global__ __launch_bounds__(64) void k_walk_node_len(const int32_t* __restrict_
 	}
 }
 
-// a step's segments into the node-order lists: dst = ptr[node] + fill[node]
+// a step's segments into the node-order lists: dst = ptr[node] + fill[node].  One wave per unit,
+// its lanes copying consecutive entries.
 __global__ __launch_bounds__(64) void k_walk_place(const int32_t* __restrict__ nodes_of, int nu, const int64_t* __restrict__ sizes,
 		const int64_t* __restrict__ offs, const int64_t* __restrict__ base, int64_t nn, const int64_t* __restrict__ ptr,
 		int64_t* __restrict__ fill, WalkPools src, int32_t* const* __restrict__ dst) {
-	const int i = blockIdx.x * 64 + threadIdx.x;
-	if (i >= nu) return;
+	const int i = blockIdx.x;
+	const int lane = threadIdx.x;
 	const int64_t nd = nodes_of[i];
 	for (int k = 0; k < K_N; k++) {
 		const int64_t len = sizes[int64_t(k) * nu + i];
 		if (len == 0) continue;
 		const int64_t from = base[k] + offs[int64_t(k) * nu + i];
-		int64_t& f = fill[int64_t(k) * (nn + 1) + nd];
-		const int64_t to = ptr[int64_t(k) * (nn + 1) + nd] + f;
-		for (int64_t l = 0; l < len; l++) dst[k][to + l] = src.out[k][from + l];
-		f += len;
+		int64_t* f = fill + int64_t(k) * (nn + 1) + nd;
+		const int64_t to = ptr[int64_t(k) * (nn + 1) + nd] + *f;
+		const int32_t* a = src.out[k] + from;
+		int32_t* b = dst[k] + to;
+		for (int64_t l = lane; l < len; l += 64) b[l] = a[l];
+		__syncthreads();  // every lane has read the fill before lane 0 moves it
+		if (lane == 0) *f += len;
 	}
 }
 
@@ -353,6 +368,36 @@ struct Pool {
 	}
 };
 
+// device memory handed out in 256-byte-aligned pieces from blocks of >= 64 MiB, freed together
+struct Arena {
+	std::vector<void*> blocks;
+	char* cur = nullptr;
+	size_t left = 0;
+	template <class T>
+	hipError_t get(size_t count, T** p) {
+		const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+		if (bytes > left) {
+			const size_t b = std::max<size_t>(bytes, size_t(64) << 20);
+			void* q = nullptr;
+			const hipError_t e = hipMalloc(&q, b);
+			if (e != hipSuccess) return e;
+			blocks.push_back(q);
+			cur = static_cast<char*>(q);
+			left = b;
+		}
+		*p = reinterpret_cast<T*>(cur);
+		cur += bytes;
+		left -= bytes;
+		return hipSuccess;
+	}
+	void release() {
+		for (void* b : blocks) (void) hipFree(b);
+		blocks.clear();
+		cur = nullptr;
+		left = 0;
+	}
+};
+
 // one step's record for the final placement: its units' nodes, sizes and offsets (device) and
 // the output bases (host)
 struct StepLog {
@@ -367,7 +412,15 @@ struct StepLog {
 
 static double wall() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
-hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t_out, unsigned keep_mask, int32_t** kept) {
+void DeviceLists::release() {
+	if (nodes) (void) hipFree(nodes);
+	if (ptr_all) (void) hipFree(ptr_all);
+	for (auto& v : val)
+		if (v) (void) hipFree(v);
+	*this = DeviceLists();
+}
+
+hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t_out, DeviceLists* keep) {
 	double tw[5] = {wall(), 0, 0, 0, 0};
 	const int64_t nn = int64_t(T.nodes.size());
 	hipError_t e = hipSuccess;
@@ -384,7 +437,8 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	void* scan_tmp = nullptr;
 	size_t scan_tmp_bytes = 0;
 	int32_t overflow = 0;
-	int64_t units_cap = 0, next_units_cap = 0;
+	int64_t units_cap = 0, next_units_cap = 0, counts_cap = 0;
+	Arena arena;
 	Pool lists_in, lists_out, out[K_N], fin[K_N];
 	std::vector<StepLog> log;
 	int64_t tot[K_N] = {};
@@ -407,7 +461,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	}
 	WCHK(hipMalloc(&dev_overflow, sizeof(int32_t)));
 	WCHK(hipMemsetAsync(dev_overflow, 0, sizeof(int32_t), s));
-	WCHK(hipMalloc(&dev_tot, NCOL * sizeof(int64_t)));
+	WCHK(hipMalloc(&dev_tot, 2 * NCOL * sizeof(int64_t)));
 	P.nodes = static_cast<const WalkNode*>(dev_nodes);
 	P.theta_inv = 1.0 / T.P.theta;
 	P.h = T.P.soft_len;
@@ -431,11 +485,17 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		StepLog L;
 		L.nu = nu;
 		for (int k = 0; k < K_N; k++) L.base[k] = tot[k];
-		WCHK(hipMalloc(&L.nodes_of, size_t(nu) * sizeof(int32_t)));
-		WCHK(hipMalloc(&L.sizes, size_t(nu) * NCOL * sizeof(int64_t)));
-		WCHK(hipMalloc(&L.offs, size_t(nu) * NCOL * sizeof(int64_t)));
+		WCHK(arena.get(size_t(nu), &L.nodes_of));
+		WCHK(arena.get(size_t(nu) * NCOL, &L.sizes));
+		WCHK(arena.get(size_t(nu) * NCOL, &L.offs));
 		log.push_back(L);
-		WCHK(hipMalloc(&dev_counts, size_t(nu) * 8 * sizeof(int32_t)));
+		if (int64_t(nu) * 8 > counts_cap) {
+			WCHK(hipStreamSynchronize(s));  // the last step's kernels read the old counts
+			if (dev_counts) (void) hipFree(dev_counts);
+			dev_counts = nullptr;
+			counts_cap = std::max<int64_t>(int64_t(nu) * 8, 2 * counts_cap);
+			WCHK(hipMalloc(&dev_counts, size_t(counts_cap) * sizeof(int32_t)));
+		}
 		P.lists = lists_in.p;
 		hipLaunchKernelGGL(k_walk_count, dim3(g), dim3(64), 0, s, P, dev_units, nu, dev_counts);
 		WCHK(hipGetLastError());
@@ -444,21 +504,23 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		hipLaunchKernelGGL(k_walk_nodes_of, dim3(g), dim3(64), 0, s, dev_units, nu, L.nodes_of);
 		WCHK(hipGetLastError());
 		{
+			// all columns in one scan, then each column's start taken off
+			const int64_t m = int64_t(nu) * NCOL;
 			size_t need = 0;
-			WCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, L.sizes, L.offs, nu, s));
+			WCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, L.sizes, L.offs, m, s));
 			if (need > scan_tmp_bytes) {
 				if (scan_tmp) (void) hipFree(scan_tmp);
 				scan_tmp = nullptr;
 				scan_tmp_bytes = std::max(need, 2 * scan_tmp_bytes);
 				WCHK(hipMalloc(&scan_tmp, scan_tmp_bytes));
 			}
-			for (int k = 0; k < NCOL; k++) {
-				size_t b = scan_tmp_bytes;
-				WCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, L.sizes + int64_t(k) * nu, L.offs + int64_t(k) * nu, nu, s));
-			}
+			size_t b = scan_tmp_bytes;
+			WCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, L.sizes, L.offs, m, s));
+			hipLaunchKernelGGL(k_walk_totals, dim3(1), dim3(64), 0, s, L.sizes, L.offs, nu, dev_tot);
+			WCHK(hipGetLastError());
+			hipLaunchKernelGGL(k_walk_colfix, dim3(unsigned((m + 255) / 256)), dim3(256), 0, s, dev_tot, nu, L.offs);
+			WCHK(hipGetLastError());
 		}
-		hipLaunchKernelGGL(k_walk_totals, dim3(1), dim3(64), 0, s, L.sizes, L.offs, nu, dev_tot);
-		WCHK(hipGetLastError());
 		int64_t t[NCOL];
 		WCHK(hipMemcpyAsync(t, dev_tot, sizeof(t), hipMemcpyDeviceToHost, s));
 		WCHK(hipStreamSynchronize(s));
@@ -494,9 +556,6 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 					int(next_units_cap));
 			WCHK(hipGetLastError());
 		}
-		WCHK(hipStreamSynchronize(s));
-		(void) hipFree(dev_counts);
-		dev_counts = nullptr;
 		std::swap(lists_in, lists_out);
 		std::swap(dev_units, dev_next_units);
 		std::swap(units_cap, next_units_cap);
@@ -553,26 +612,44 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			const StepLog& L = log[st];
 			e = hipMemcpyAsync(dev_base + K_N * st, L.base, K_N * sizeof(int64_t), hipMemcpyHostToDevice, s);
 			if (e != hipSuccess) break;
-			hipLaunchKernelGGL(k_walk_place, dim3(unsigned((L.nu + 63) / 64)), dim3(64), 0, s, L.nodes_of, L.nu, L.sizes, L.offs,
+			hipLaunchKernelGGL(k_walk_place, dim3(unsigned(L.nu)), dim3(64), 0, s, L.nodes_of, L.nu, L.sizes, L.offs,
 					dev_base + K_N * st, nn, dev_ptr, dev_fill, src, dev_dst);
 			e = hipGetLastError();
 		}
 		Lists& Ls = T.lists;
 		Ls = Lists();
+		if (keep) {
+			// everything stays on the device: the caller's DeviceLists takes the arrays
+			if (e == hipSuccess) e = hipStreamSynchronize(s);
+			if (e != hipSuccess) {
+				(void) hipFree(dev_ptr);
+				err = std::string("list assembly: ") + hipGetErrorString(e);
+				goto done;
+			}
+			keep->release();
+			keep->nn = nn;
+			keep->ptr_all = dev_ptr;
+			keep->nodes = static_cast<WalkNode*>(dev_nodes);
+			dev_nodes = nullptr;
+			for (int k = 0; k < K_N; k++) {
+				keep->val[k] = fin[k].p;
+				keep->len[k] = tot[k];
+				fin[k].p = nullptr;
+				fin[k].cap = 0;
+			}
+			T.census = Census();
+			tw[3] = tw[4] = wall();
+			if (t_out)
+				for (int k = 0; k < 4; k++) t_out[k] = tw[k + 1] - tw[k];
+			goto done;
+		}
 		hvec<int64_t>* ptrs[K_N] = {&Ls.cc_d_ptr, &Ls.cp_d_ptr, &Ls.cc_e_ptr, &Ls.cp_e_ptr,
 				&Ls.pc_d_ptr, &Ls.pp_d_ptr, &Ls.pc_e_ptr, &Ls.pp_e_ptr};
 		hvec<int32_t>* vals[K_N] = {&Ls.cc_d, &Ls.cp_d, &Ls.cc_e, &Ls.cp_e, &Ls.pc_d, &Ls.pp_d, &Ls.pc_e, &Ls.pp_e};
 		for (int k = 0; k < K_N && e == hipSuccess; k++) {
 			ptrs[k]->resize(size_t(np));
-			if (!(kept && (keep_mask >> k) & 1u)) vals[k]->resize(size_t(tot[k]));
+			vals[k]->resize(size_t(tot[k]));
 			e = hipMemcpyAsync(ptrs[k]->data(), dev_ptr + k * np, size_t(np) * sizeof(int64_t), hipMemcpyDeviceToHost, s);
-			if (kept && (keep_mask >> k) & 1u) {
-				vals[k]->clear();
-				kept[k] = fin[k].p;  // the placement kernels above write it; the caller owns it now
-				fin[k].p = nullptr;
-				fin[k].cap = 0;
-				continue;
-			}
 			if (e == hipSuccess && tot[k] > 0)
 				e = hipMemcpyAsync(vals[k]->data(), fin[k].p, size_t(tot[k]) * sizeof(int32_t), hipMemcpyDeviceToHost, s);
 		}
@@ -602,11 +679,7 @@ done:
 	if (dev_dst) (void) hipFree(dev_dst);
 	if (dev_overflow) (void) hipFree(dev_overflow);
 	if (scan_tmp) (void) hipFree(scan_tmp);
-	for (auto& L : log) {
-		if (L.nodes_of) (void) hipFree(L.nodes_of);
-		if (L.sizes) (void) hipFree(L.sizes);
-		if (L.offs) (void) hipFree(L.offs);
-	}
+	arena.release();
 	lists_in.release();
 	lists_out.release();
 	for (auto& p : out) p.release();

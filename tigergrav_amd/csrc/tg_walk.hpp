@@ -19,14 +19,24 @@ struct WalkNode {
 // fills out[i] for every node of T (host, OpenMP; tg_upload.cpp)
 void walk_nodes(const Tree& T, WalkNode* out);
 
+// The walk's lists left in device memory: node-order CSR per kind, kinds in the order cc_d, cp_d,
+// cc_e, cp_e, pc_d, pp_d, pc_e, pp_e (node kinds hold node ids or [begin, end) particle ranges as
+// the host Lists do).  The table of nodes the walk read stays too.  Every array is owned here.
+struct DeviceLists {
+	WalkNode* nodes = nullptr;   // nn nodes
+	int64_t* ptr_all = nullptr;  // 8 x (nn + 1) pointers, kind-major
+	int32_t* val[8] = {};        // values per kind
+	int64_t len[8] = {};         // values per kind (ints)
+	int64_t nn = 0;
+	const int64_t* ptr(int k) const { return ptr_all + int64_t(k) * (nn + 1); }
+	void release();
+};
+
 // Tree::walk on the device: fills T.lists and T.census with exactly the lists the host walk
 // builds (same entries, same order).  Returns hipSuccess or the first HIP error; err names it.
 // t, if given, receives the seconds spent in set-up, the walk steps, list assembly with the copy
-// to the host, and the census.  Lists k (in the order cc_d, cp_d, cc_e, cp_e, pc_d, pp_d, pc_e,
-// pp_e) with bit k of keep_mask set stay on the device: kept[k] receives the device array of
-// their values (the caller frees it with hipFree, whatever the return status) and the host
-// vector of values is left empty; their pointer arrays are copied to the host as for the others.
-hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t = nullptr, unsigned keep_mask = 0,
-		int32_t** kept = nullptr);
+// to the host, and the census.  With keep, nothing is copied to the host: the lists stay on the
+// device in *keep (released first), T.lists is left empty and T.census zero.
+hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t = nullptr, DeviceLists* keep = nullptr);
 
 }  // namespace tg
