@@ -31,20 +31,6 @@ namespace tg {
 
 namespace {
 
-struct DNode {
-	int32_t pbegin, pend, c0, c1, parent, level;
-	uint64_t boxid;
-};
-
-struct DMulti {
-	float m[17];
-	float r;
-	int32_t x[3];
-	int32_t pad;
-	uint64_t N;
-	double rmin[3], rmax[3];
-};
-
 struct SplitInfo {
 	double mid;
 	int32_t dim, first_child, lo_end, pad;  // lo_end: the split point (first particle of the right child)
@@ -480,8 +466,17 @@ struct Buf {
 
 }  // namespace
 
+void DeviceTree::release() {
+	if (pos) (void) hipFree(pos);
+	if (mass) (void) hipFree(mass);
+	if (order) (void) hipFree(order);
+	if (nodes) (void) hipFree(nodes);
+	if (multi) (void) hipFree(multi);
+	*this = DeviceTree();
+}
+
 hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int64_t n, const TreeParams& p,
-		double particle_mass, int device, hipStream_t s, std::string& err, double* t_out, bool on_device) {
+		double particle_mass, int device, hipStream_t s, std::string& err, double* t_out, bool on_device, DeviceTree* keep) {
 	const double t0 = wall();
 	hipError_t e = hipSuccess;
 #define DTCHK(call)                                                    \
@@ -492,6 +487,15 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 			return e;                                                  \
 		}                                                              \
 	} while (0)
+#define DTLAUNCH(name)                                                                 \
+	do {                                                                               \
+		e = hipGetLastError();                                                         \
+		if (e != hipSuccess) {                                                         \
+			err = std::string("device tree build: launch ") + name + ": " + hipGetErrorString(e); \
+			return e;                                                                  \
+		}                                                                              \
+	} while (0)
+	(void) hipGetLastError();  // an error left by an earlier call is not the build's
 	if (n > int64_t(INT32_MAX) - 1) {
 		err = "device tree build: more than 2^31 - 2 particles";
 		return hipErrorInvalidValue;
@@ -524,7 +528,7 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 		}
 		hipLaunchKernelGGL(k_iota64, dim3(blocks(n, 256)), dim3(256), 0, s, dorder.as<int64_t>(), n);
 		hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, s, dpnode.as<int32_t>(), n, 0);
-		DTCHK(hipGetLastError());
+		DTLAUNCH("k_iota64/k_fill_i32");
 	}
 	{
 		const DNode root{0, int32_t(n), -1, -1, -1, 0, 1};
@@ -544,7 +548,7 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 		DTCHK(dinfo.alloc(size_t(nl) * sizeof(SplitInfo)));
 		hipLaunchKernelGGL(k_split_flags, dim3(blocks(nl + 1, 256)), dim3(256), 0, s, dnodes.as<const DNode>(), lb, nl, p.parts_per_node,
 				MAX_SPLIT_LEVEL, dflags.as<int32_t>());
-		DTCHK(hipGetLastError());
+		DTLAUNCH("k_split_flags");
 		size_t need = 0;
 		DTCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, dflags.as<int32_t>(), drank.as<int32_t>(), nl + 1, s));
 		if (need > scan_tmp.bytes) {
@@ -580,7 +584,7 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 				dflags.as<const int32_t>(), drank.as<const int32_t>(), dinfo.as<SplitInfo>());
 		hipLaunchKernelGGL(k_low_flags, dim3(blocks(n + 1, 256)), dim3(256), 0, s, dpos.as<const int32_t>(), dpnode.as<const int32_t>(), n,
 				lb, nl, dinfo.as<const SplitInfo>(), dlow.as<int32_t>());
-		DTCHK(hipGetLastError());
+		DTLAUNCH("k_split_setup/k_low_flags");
 		have = scan_tmp.bytes;
 		DTCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, have, dlow.as<int32_t>(), dS.as<int32_t>(), int(n + 1), s));
 		hipLaunchKernelGGL(k_split_nodes, dim3(blocks(nl, 256)), dim3(256), 0, s, dnodes.as<DNode>(), lb, nl, dinfo.as<SplitInfo>(),
@@ -593,7 +597,7 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 					dpnode.as<int32_t>(), n, lb, nl, dinfo.as<const SplitInfo>(), dnodes.as<const DNode>(), dlow.as<const int32_t>(),
 					dS.as<const int32_t>(), dmr.as<const int32_t>());
 		}
-		DTCHK(hipGetLastError());
+		DTLAUNCH("k_split_nodes/k_high_side_lows/k_exchange");
 		lb = le;
 		le = le + 2 * nsplit;
 	}
@@ -607,9 +611,68 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 		hipLaunchKernelGGL(k_leaf_moments, dim3(blocks(nl, 4)), dim3(64), 0, s, dnodes.as<const DNode>(), b, nl, dpos.as<const int32_t>(),
 				dmass.as<const float>(), dmulti.as<DMulti>());
 		hipLaunchKernelGGL(k_interior_moments, dim3(blocks(nl, 64)), dim3(64), 0, s, dnodes.as<const DNode>(), b, nl, dmulti.as<DMulti>());
-		DTCHK(hipGetLastError());
+		DTLAUNCH("k_leaf_moments/k_interior_moments");
 	}
-	// back to the host tree, through pinned staging
+	if (keep) {
+		// everything stays on the device: the caller's DeviceTree takes the arrays
+		DTCHK(hipStreamSynchronize(s));
+		keep->release();
+		keep->n = n;
+		keep->nn = nn;
+		keep->level_begin = level_begin;
+		keep->pos = static_cast<int32_t*>(dpos.p);
+		keep->mass = static_cast<float*>(dmass.p);
+		keep->order = static_cast<int64_t*>(dorder.p);
+		keep->nodes = static_cast<DNode*>(dnodes.p);
+		keep->multi = static_cast<DMulti*>(dmulti.p);
+		dpos.p = dmass.p = dorder.p = dnodes.p = dmulti.p = nullptr;
+		T.pos.clear();
+		T.mass.clear();
+		T.order.clear();
+		T.nodes.clear();
+		T.leaves.clear();
+		T.max_level = int(level_begin.size()) - 2;
+		if (t_out) {
+			t_out[0] = t1 - t0;
+			t_out[1] = wall() - t1;
+		}
+		return hipSuccess;
+	}
+#undef DTLAUNCH
+	{
+		DeviceTree view;  // the build's arrays, still owned by the buffers above
+		view.n = n;
+		view.nn = nn;
+		view.pos = static_cast<int32_t*>(dpos.p);
+		view.mass = static_cast<float*>(dmass.p);
+		view.order = static_cast<int64_t*>(dorder.p);
+		view.nodes = static_cast<DNode*>(dnodes.p);
+		view.multi = static_cast<DMulti*>(dmulti.p);
+		e = device_tree_to_host(view, T, s, err);
+		view = DeviceTree();  // not released: the buffers free them
+		if (e != hipSuccess) return e;
+	}
+	if (t_out) {
+		t_out[0] = t1 - t0;
+		t_out[1] = wall() - t1;
+	}
+	return hipSuccess;
+#undef DTCHK
+}
+
+hipError_t device_tree_to_host(const DeviceTree& D, Tree& T, hipStream_t s, std::string& err) {
+	hipError_t e = hipSuccess;
+#define DTCHK(call)                                                    \
+	do {                                                               \
+		e = (call);                                                    \
+		if (e != hipSuccess) {                                         \
+			err = std::string(#call) + ": " + hipGetErrorString(e);    \
+			return e;                                                  \
+		}                                                              \
+	} while (0)
+	const int64_t n = D.n;
+	const int nn = int(D.nn);
+	// through pinned staging
 	std::lock_guard<std::mutex> lock(pinned_mtx);
 	const size_t b_pos = size_t(3 * n) * 4, b_mass = size_t(n) * 4, b_order = size_t(n) * 8;
 	const size_t b_nodes = size_t(nn) * sizeof(DNode), b_multi = size_t(nn) * sizeof(DMulti);
@@ -619,12 +682,12 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 	DTCHK(pinned_staging(o_multi + b_multi, &stage));
 	char* st = static_cast<char*>(stage);
 	if (n > 0) {
-		DTCHK(hipMemcpyAsync(st, dpos.p, b_pos, hipMemcpyDeviceToHost, s));
-		DTCHK(hipMemcpyAsync(st + o_mass, dmass.p, b_mass, hipMemcpyDeviceToHost, s));
-		DTCHK(hipMemcpyAsync(st + o_order, dorder.p, b_order, hipMemcpyDeviceToHost, s));
+		DTCHK(hipMemcpyAsync(st, D.pos, b_pos, hipMemcpyDeviceToHost, s));
+		DTCHK(hipMemcpyAsync(st + o_mass, D.mass, b_mass, hipMemcpyDeviceToHost, s));
+		DTCHK(hipMemcpyAsync(st + o_order, D.order, b_order, hipMemcpyDeviceToHost, s));
 	}
-	DTCHK(hipMemcpyAsync(st + o_nodes, dnodes.p, b_nodes, hipMemcpyDeviceToHost, s));
-	DTCHK(hipMemcpyAsync(st + o_multi, dmulti.p, b_multi, hipMemcpyDeviceToHost, s));
+	DTCHK(hipMemcpyAsync(st + o_nodes, D.nodes, b_nodes, hipMemcpyDeviceToHost, s));
+	DTCHK(hipMemcpyAsync(st + o_multi, D.multi, b_multi, hipMemcpyDeviceToHost, s));
 	DTCHK(hipStreamSynchronize(s));
 	T.pos.resize(size_t(3 * n));
 	T.mass.resize(size_t(n));
@@ -637,10 +700,10 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 		const int64_t nch = (n + chunk - 1) / chunk;
 #pragma omp parallel for schedule(static)
 		for (int64_t c = 0; c < nch; c++) {
-			const int64_t b = c * chunk, e = std::min(n, b + chunk);
-			std::memcpy(T.pos.data() + 3 * b, st + 12 * b, size_t(e - b) * 12);
-			std::memcpy(T.mass.data() + b, st + o_mass + 4 * b, size_t(e - b) * 4);
-			std::memcpy(T.order.data() + b, st + o_order + 8 * b, size_t(e - b) * 8);
+			const int64_t b = c * chunk, e2 = std::min(n, b + chunk);
+			std::memcpy(T.pos.data() + 3 * b, st + 12 * b, size_t(e2 - b) * 12);
+			std::memcpy(T.mass.data() + b, st + o_mass + 4 * b, size_t(e2 - b) * 4);
+			std::memcpy(T.order.data() + b, st + o_order + 8 * b, size_t(e2 - b) * 8);
 		}
 	}
 	T.nodes.resize(size_t(nn));
@@ -670,10 +733,6 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 	for (int i = 0; i < nn; i++) {
 		if (T.nodes[size_t(i)].leaf()) T.leaves.push_back(i);
 		T.max_level = std::max(T.max_level, T.nodes[size_t(i)].level);
-	}
-	if (t_out) {
-		t_out[0] = t1 - t0;
-		t_out[1] = wall() - t1;
 	}
 	return hipSuccess;
 #undef DTCHK

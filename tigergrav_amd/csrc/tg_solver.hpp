@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "tg_devtree.hpp"
 #include "tg_kernels.hpp"
 #include "tg_tree.hpp"
 #include "tg_upload.hpp"
@@ -85,6 +86,14 @@ struct Solver {
 	int n_leaf_items = 0;
 	int64_t n_slots = 0;
 	int64_t list_bytes = 0;
+	// the device build's tree (gpu_tree with gpu_walk): the walk's node table and the solve's tree
+	// arrays are made from it on the device (upload_tree_device), and the host copy in `tree` only
+	// on demand (host_tree)
+	DeviceTree dtree;
+	bool tree_on_device = false;  // tree holds no copy of dtree yet
+	int64_t n_nodes = 0;          // nodes of the current tree
+	int64_t n_leaves = 0;         // its leaves, empty ones included
+	int max_level = 0;            // its deepest level
 	// the device walk's lists (gpu_walk): the work of every kind is built from them on the device
 	// (upload_lists_device), the CC and PC work lists are views of them, and the host copy in
 	// tree.lists is made only on demand (host_lists)
@@ -106,6 +115,8 @@ struct Solver {
 	int set_particles(const int32_t* x, const float* mass, int64_t count, bool on_device = false);
 	int upload();
 	int upload_lists_device();  // every kind's work from dlists, on the device (tg_prep.hip)
+	int upload_tree_device();   // the tree's arrays from dtree, on the device (tg_prep.hip)
+	int host_tree();            // copies the device build's tree into tree
 	int host_lists();  // copies the device walk's lists into tree.lists
 	int upload_sources();
 	int update_particles(const int32_t* x, const float* mass);
@@ -123,6 +134,7 @@ struct Solver {
 	int forces(double* out4);  // N x (phi, gx, gy, gz), input order
 	int forces_device(double* out4_dev);  // the same into device memory, on the solver's stream (waits for it)
 	DeviceArray d_order;  // input index of each tree-order particle, uploaded by forces_device
+	DeviceArray d_fin;    // forces in input order, for the copy to the host
 	bool order_on_device = false;
 	~Solver();
 };

@@ -12,9 +12,11 @@
 //   DRAIN  (leaf n, one checklist, direct or Ewald): one round of the leaf's drain loop
 //          (src/tree.cpp:530-602): opened leaves become PP ranges, far nodes PC, the rest is
 //          opened or split for the next round.
-// Per step, on the device: k_walk_count counts every unit's outputs, k_walk_sizes turns the counts
+// Per step, on the device: k_walk_count (one wave per unit, 64 checklist entries at a time)
+// classifies every entry and counts every unit's outputs, k_walk_sizes turns the counts
 // into per-unit sizes (entries of each output, of the next checklists, next units), exclusive
-// scans give the offsets, k_walk_write writes and k_walk_units emits the next step's units; the
+// scans give the offsets, k_walk_write (one wave per unit, from the kept categories) writes and
+// k_walk_units emits the next step's units; the
 // host reads back only the step's totals (to size the pools).  Each node's outputs are segments
 // in step order, which is the host walk's order; k_walk_place puts them into node-order CSR at
 // the end.  The MAC arithmetic is the host walk's, in FP64 without
@@ -37,6 +39,7 @@ enum { MODE_FIRST = 0, MODE_DRAIN_D = 1, MODE_DRAIN_E = 2 };
 struct WalkUnit {
 	int32_t node, mode, len_d, len_e;
 	int64_t off_d, off_e;
+	int32_t child, pad;  // a FIRST unit's sibling (0, 1): the two children of a node share their checklists
 };
 
 struct WalkParams {
@@ -86,54 +89,56 @@ __device__ __forceinline__ bool far_test(const WalkParams& P, const WalkNode& n,
 	return n.rmax < P.r_ewald;
 }
 
-// counts of one checklist pass: FIRST {cc, cp ranges, opened, deferred}; DRAIN {pp ranges, pc,
-// opened, deferred}
-__device__ void count_pass(const WalkParams& P, const WalkNode& n, const double* X, const int32_t* list, int len, bool ewald_kind,
-		bool drain, int* c) {
-	for (int i = 0; i < len; i++) {
-		const int32_t e = list[i];
-		const bool opened = e < 0;
-		const WalkNode cn = P.nodes[opened ? ~e : e];
-		if (cn.pend == cn.pbegin) continue;
-		const bool far = far_test(P, n, X, cn, ewald_kind);
-		if (drain) {
-			if (opened)
-				c[0]++;
-			else if (far)
-				c[1]++;
-			else if (cn.c0 < 0)
-				c[2]++;
-			else
-				c[3]++;
-		} else {
-			if (far) {
-				if (opened)
-					c[1]++;
-				else
-					c[0]++;
-			} else if (cn.c0 < 0) {
-				c[2]++;
-			} else {
-				c[3]++;
-			}
+// Checklist entries are classified 64 at a time by one wave, each lane one entry, in checklist
+// order.  Categories: FIRST {0 cc, 1 cp range, 2 opened, 3 deferred}; DRAIN {0 pp range, 1 pc,
+// 2 opened, 3 deferred}; 4: an empty node (nothing).  The count pass keeps every entry's category
+// (one byte, beside the entry) for the write pass, which then places each lane's outputs by its
+// rank among the lanes of the same category: the serial pass's order.  The two children of a node
+// read the same checklists, so entry i keeps a category byte per child (2 i + child).
+enum { CAT_SKIP = 4 };
+
+__device__ __forceinline__ unsigned long long lanes_below(int lane) { return lane ? ~0ull >> (64 - lane) : 0ull; }
+
+__device__ __forceinline__ int classify(const WalkParams& P, const WalkNode& n, const double* X, int32_t e, bool ewald_kind, bool drain) {
+	const bool opened = e < 0;
+	const WalkNode cn = P.nodes[opened ? ~e : e];
+	if (cn.pend == cn.pbegin) return CAT_SKIP;
+	const bool far = far_test(P, n, X, cn, ewald_kind);
+	if (drain) return opened ? 0 : far ? 1 : cn.c0 < 0 ? 2 : 3;
+	return far ? (opened ? 1 : 0) : cn.c0 < 0 ? 2 : 3;
+}
+
+// one checklist pass classified and counted by the wave; c (4, the same in every lane) += counts
+__device__ void count_pass(const WalkParams& P, const WalkNode& n, const double* X, const int32_t* list, uint8_t* codes, int len,
+		bool ewald_kind, bool drain, int* c) {
+	const int lane = threadIdx.x & 63;
+	for (int b = 0; b < len; b += 64) {
+		const int i = b + lane;
+		int cat = CAT_SKIP;
+		if (i < len) {
+			cat = classify(P, n, X, list[i], ewald_kind, drain);
+			codes[2 * i] = uint8_t(cat);
 		}
+		for (int k = 0; k < 4; k++) c[k] += __popcll(__ballot(cat == k));
 	}
 }
 
-__global__ __launch_bounds__(64) void k_walk_count(WalkParams P, const WalkUnit* __restrict__ units, int nunits, int32_t* __restrict__ counts) {
-	const int i = blockIdx.x * 64 + threadIdx.x;
+// one wave (work group) per unit
+__global__ __launch_bounds__(64) void k_walk_count(WalkParams P, uint8_t* __restrict__ codes, const WalkUnit* __restrict__ units, int nunits,
+		int32_t* __restrict__ counts) {
+	const int i = blockIdx.x;
 	if (i >= nunits) return;
 	const WalkUnit u = units[i];
 	const WalkNode n = P.nodes[u.node];
 	const double X[3] = {pos_d(n.x[0]), pos_d(n.x[1]), pos_d(n.x[2])};
 	int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	if (u.mode == MODE_FIRST) {
-		count_pass(P, n, X, P.lists + u.off_d, u.len_d, false, false, c);
-		if (P.ewald) count_pass(P, n, X, P.lists + u.off_e, u.len_e, true, false, c + 4);
+		count_pass(P, n, X, P.lists + u.off_d, codes + 2 * u.off_d + u.child, u.len_d, false, false, c);
+		if (P.ewald) count_pass(P, n, X, P.lists + u.off_e, codes + 2 * u.off_e + u.child, u.len_e, true, false, c + 4);
 	} else {
-		count_pass(P, n, X, P.lists + u.off_d, u.len_d, u.mode == MODE_DRAIN_E, true, c);
+		count_pass(P, n, X, P.lists + u.off_d, codes + 2 * u.off_d + u.child, u.len_d, u.mode == MODE_DRAIN_E, true, c);
 	}
-	for (int k = 0; k < 8; k++) counts[8 * int64_t(i) + k] = c[k];
+	if ((threadIdx.x & 63) < 8) counts[8 * int64_t(i) + (threadIdx.x & 63)] = c[threadIdx.x & 63];
 }
 
 __device__ __forceinline__ void put_at(int32_t* p, int64_t i, int64_t cap, int32_t v, int32_t* overflow) {
@@ -143,45 +148,48 @@ __device__ __forceinline__ void put_at(int32_t* p, int64_t i, int64_t cap, int32
 		*overflow = 1;
 }
 
-// one checklist pass, written: FIRST into (a: cc, b: cp) and next; DRAIN into (a: pp, b: pc) and next
-__device__ void write_pass(const WalkParams& P, const WalkNode& n, const double* X, const int32_t* list, int len, bool ewald_kind,
-		bool drain, int32_t* a, int64_t oa, int64_t capa, int32_t* b, int64_t ob, int64_t capb, int32_t* next, int64_t on, int64_t capn,
-		int n_opened, int32_t* overflow) {
+// one checklist pass, written by the wave from the count pass's categories: FIRST into (a: cc, b: cp)
+// and next; DRAIN into (a: pp, b: pc) and next.  Next holds the opened leaves first (n_opened of
+// them), then the deferred nodes' children.
+__device__ void write_pass(const WalkParams& P, const int32_t* list, const uint8_t* codes, int len, bool drain, int32_t* a, int64_t oa,
+		int64_t capa, int32_t* b, int64_t ob, int64_t capb, int32_t* next, int64_t on, int64_t capn, int n_opened, int32_t* overflow) {
+	const int lane = threadIdx.x & 63;
+	const unsigned long long below = lanes_below(lane);
 	int64_t op = on, od = on + n_opened;
-	for (int i = 0; i < len; i++) {
-		const int32_t e = list[i];
-		const bool opened = e < 0;
-		const int32_t id = opened ? ~e : e;
-		const WalkNode cn = P.nodes[id];
-		if (cn.pend == cn.pbegin) continue;
-		const bool far = far_test(P, n, X, cn, ewald_kind);
-		if (drain) {
-			if (opened) {
-				put_at(a, oa++, capa, cn.pbegin, overflow);
-				put_at(a, oa++, capa, cn.pend, overflow);
-			} else if (far) {
-				put_at(b, ob++, capb, id, overflow);
-			} else if (cn.c0 < 0) {
-				put_at(next, op++, capn, ~id, overflow);
-			} else {
-				put_at(next, od++, capn, cn.c0, overflow);
-				put_at(next, od++, capn, cn.c1, overflow);
-			}
-		} else {
-			if (far) {
-				if (opened) {
-					put_at(b, ob++, capb, cn.pbegin, overflow);
-					put_at(b, ob++, capb, cn.pend, overflow);
-				} else {
-					put_at(a, oa++, capa, id, overflow);
-				}
-			} else if (cn.c0 < 0) {
-				put_at(next, op++, capn, ~id, overflow);
-			} else {
-				put_at(next, od++, capn, cn.c0, overflow);
-				put_at(next, od++, capn, cn.c1, overflow);
+	for (int base = 0; base < len; base += 64) {
+		const int i = base + lane;
+		const int cat = i < len ? int(codes[2 * i]) : CAT_SKIP;
+		const int32_t e = i < len ? list[i] : 0;
+		unsigned long long m[4];
+		for (int k = 0; k < 4; k++) m[k] = __ballot(cat == k);
+		if (cat != CAT_SKIP) {
+			const int32_t id = e < 0 ? ~e : e;
+			const int r = __popcll(m[cat] & below);
+			if (cat == 2) {
+				put_at(next, op + r, capn, ~id, overflow);
+			} else if (cat == 3) {
+				const WalkNode cn = P.nodes[id];
+				put_at(next, od + 2 * r, capn, cn.c0, overflow);
+				put_at(next, od + 2 * r + 1, capn, cn.c1, overflow);
+			} else if ((cat == 0) == drain) {  // a range: PP (drain) or CP (first pass)
+				const WalkNode cn = P.nodes[id];
+				int32_t* dst = drain ? a : b;
+				const int64_t o = (drain ? oa : ob) + 2 * r;
+				const int64_t cap = drain ? capa : capb;
+				put_at(dst, o, cap, cn.pbegin, overflow);
+				put_at(dst, o + 1, cap, cn.pend, overflow);
+			} else {  // a node: CC (first pass) or PC (drain)
+				if (drain)
+					put_at(b, ob + r, capb, id, overflow);
+				else
+					put_at(a, oa + r, capa, id, overflow);
 			}
 		}
+		const int n0 = __popcll(m[0]), n1 = __popcll(m[1]);
+		oa += drain ? 2 * n0 : n0;
+		ob += drain ? n1 : 2 * n1;
+		op += __popcll(m[2]);
+		od += 2 * __popcll(m[3]);
 	}
 }
 
@@ -247,28 +255,27 @@ struct WalkPools {
 	int32_t* overflow;
 };
 
-__global__ __launch_bounds__(64) void k_walk_write(WalkParams P, const WalkUnit* __restrict__ units, int nu, const int32_t* __restrict__ counts,
-		const int64_t* __restrict__ offs, WalkPools W) {
-	const int i = blockIdx.x * 64 + threadIdx.x;
+// one wave (work group) per unit
+__global__ __launch_bounds__(64) void k_walk_write(WalkParams P, const uint8_t* __restrict__ codes, const WalkUnit* __restrict__ units, int nu,
+		const int32_t* __restrict__ counts, const int64_t* __restrict__ offs, WalkPools W) {
+	const int i = blockIdx.x;
 	if (i >= nu) return;
 	const WalkUnit u = units[i];
-	const WalkNode n = P.nodes[u.node];
-	const double X[3] = {pos_d(n.x[0]), pos_d(n.x[1]), pos_d(n.x[2])};
 	const int32_t* c = counts + 8 * int64_t(i);
 	auto at = [&](int k) { return W.base[k] + offs[int64_t(k) * nu + i]; };
 	const int64_t nb = offs[int64_t(COL_NEXT) * nu + i];
 	if (u.mode == MODE_FIRST) {
-		write_pass(P, n, X, P.lists + u.off_d, u.len_d, false, false, W.out[K_CC_D], at(K_CC_D), W.cap[K_CC_D], W.out[K_CP_D], at(K_CP_D),
-				W.cap[K_CP_D], W.next, nb, W.cap[K_N], c[2], W.overflow);
+		write_pass(P, P.lists + u.off_d, codes + 2 * u.off_d + u.child, u.len_d, false, W.out[K_CC_D], at(K_CC_D), W.cap[K_CC_D], W.out[K_CP_D],
+				at(K_CP_D), W.cap[K_CP_D], W.next, nb, W.cap[K_N], c[2], W.overflow);
 		if (P.ewald)
-			write_pass(P, n, X, P.lists + u.off_e, u.len_e, true, false, W.out[K_CC_E], at(K_CC_E), W.cap[K_CC_E], W.out[K_CP_E], at(K_CP_E),
-					W.cap[K_CP_E], W.next, nb + int64_t(c[2]) + 2 * int64_t(c[3]), W.cap[K_N], c[6], W.overflow);
+			write_pass(P, P.lists + u.off_e, codes + 2 * u.off_e + u.child, u.len_e, false, W.out[K_CC_E], at(K_CC_E), W.cap[K_CC_E], W.out[K_CP_E],
+					at(K_CP_E), W.cap[K_CP_E], W.next, nb + int64_t(c[2]) + 2 * int64_t(c[3]), W.cap[K_N], c[6], W.overflow);
 	} else if (u.mode == MODE_DRAIN_D) {
-		write_pass(P, n, X, P.lists + u.off_d, u.len_d, false, true, W.out[K_PP_D], at(K_PP_D), W.cap[K_PP_D], W.out[K_PC_D], at(K_PC_D),
-				W.cap[K_PC_D], W.next, nb, W.cap[K_N], c[2], W.overflow);
+		write_pass(P, P.lists + u.off_d, codes + 2 * u.off_d + u.child, u.len_d, true, W.out[K_PP_D], at(K_PP_D), W.cap[K_PP_D], W.out[K_PC_D],
+				at(K_PC_D), W.cap[K_PC_D], W.next, nb, W.cap[K_N], c[2], W.overflow);
 	} else {
-		write_pass(P, n, X, P.lists + u.off_d, u.len_d, true, true, W.out[K_PP_E], at(K_PP_E), W.cap[K_PP_E], W.out[K_PC_E], at(K_PC_E),
-				W.cap[K_PC_E], W.next, nb, W.cap[K_N], c[2], W.overflow);
+		write_pass(P, P.lists + u.off_d, codes + 2 * u.off_d + u.child, u.len_d, true, W.out[K_PP_E], at(K_PP_E), W.cap[K_PP_E], W.out[K_PC_E],
+				at(K_PC_E), W.cap[K_PC_E], W.next, nb, W.cap[K_N], c[2], W.overflow);
 	}
 }
 
@@ -293,14 +300,14 @@ __global__ __launch_bounds__(64) void k_walk_units(const WalkNode* __restrict__ 
 		if (n.c0 >= 0) {
 			const int ch[2] = {n.c0, n.c1};
 			for (int k = 0; k < 2; k++)
-				if (nodes[ch[k]].pend > nodes[ch[k]].pbegin) emit(WalkUnit{ch[k], MODE_FIRST, ld, le, nb, nb + ld});
+				if (nodes[ch[k]].pend > nodes[ch[k]].pbegin) emit(WalkUnit{ch[k], MODE_FIRST, ld, le, nb, nb + ld, k, 0});
 		} else {
-			if (ld > 0) emit(WalkUnit{u.node, MODE_DRAIN_D, ld, 0, nb, 0});
-			if (le > 0) emit(WalkUnit{u.node, MODE_DRAIN_E, le, 0, nb + ld, 0});
+			if (ld > 0) emit(WalkUnit{u.node, MODE_DRAIN_D, ld, 0, nb, 0, 0, 0});
+			if (le > 0) emit(WalkUnit{u.node, MODE_DRAIN_E, le, 0, nb + ld, 0, 0, 0});
 		}
 	} else {
 		const int32_t l = c[2] + 2 * c[3];
-		if (l > 0) emit(WalkUnit{u.node, u.mode, l, 0, nb, 0});
+		if (l > 0) emit(WalkUnit{u.node, u.mode, l, 0, nb, 0, 0, 0});
 	}
 }
 
@@ -420,9 +427,10 @@ void DeviceLists::release() {
 	*this = DeviceLists();
 }
 
-hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t_out, DeviceLists* keep) {
+hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t_out, DeviceLists* keep, WalkNode* given,
+		int64_t given_nn) {
 	double tw[5] = {wall(), 0, 0, 0, 0};
-	const int64_t nn = int64_t(T.nodes.size());
+	const int64_t nn = given ? given_nn : int64_t(T.nodes.size());
 	hipError_t e = hipSuccess;
 	void* dev_nodes = nullptr;
 	WalkUnit* dev_units = nullptr;
@@ -437,13 +445,16 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	void* scan_tmp = nullptr;
 	size_t scan_tmp_bytes = 0;
 	int32_t overflow = 0;
-	int64_t units_cap = 0, next_units_cap = 0, counts_cap = 0;
+	int64_t units_cap = 0, next_units_cap = 0, counts_cap = 0, codes_cap = 0, in_len = 2;
+	uint8_t* dev_codes = nullptr;  // the category of every checklist entry of the step: byte 2 i + child for entry i
 	Arena arena;
 	Pool lists_in, lists_out, out[K_N], fin[K_N];
 	std::vector<StepLog> log;
 	int64_t tot[K_N] = {};
 	int nu = 0;
 	WalkParams P{};
+	WalkNode root_node{};
+	hipError_t pending = hipSuccess;
 #define WCHK(call)                                     \
 	do {                                               \
 		e = (call);                                    \
@@ -452,8 +463,19 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			goto done;                                 \
 		}                                              \
 	} while (0)
+#define WLAUNCH(name)                                                                       \
+	do {                                                                                    \
+		e = hipGetLastError();                                                              \
+		if (e != hipSuccess) {                                                              \
+			err = std::string("launch ") + name + ": " + hipGetErrorString(e);              \
+			if (pending != hipSuccess) err += std::string(" (pending before the walk: ") + hipGetErrorString(pending) + ")"; \
+			goto done;                                                                      \
+		}                                                                                   \
+	} while (0)
+	pending = hipGetLastError();  // an error left by an earlier call is not the walk's
+	dev_nodes = given;  // owned from here on
 	WCHK(hipSetDevice(device));
-	{
+	if (!given) {
 		hvec<WalkNode> wn(size_t(std::max<int64_t>(nn, 1)));
 		walk_nodes(T, wn.data());
 		WCHK(hipMalloc(&dev_nodes, wn.size() * sizeof(WalkNode)));
@@ -467,10 +489,19 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	P.h = T.P.soft_len;
 	P.r_ewald = float(2.0 * T.P.theta * std::pow(1.0e-3 / 8.0, 1.0 / 3.0));  // src/tree.cpp:403-404
 	P.ewald = T.P.ewald ? 1 : 0;
-	if (nn > 0 && T.nodes[0].pend > T.nodes[0].pbegin) {
+	if (nn > 0) {
+		if (given) {
+			WCHK(hipMemcpyAsync(&root_node, dev_nodes, sizeof(WalkNode), hipMemcpyDeviceToHost, s));
+			WCHK(hipStreamSynchronize(s));
+		} else {
+			root_node.pbegin = T.nodes[0].pbegin;
+			root_node.pend = T.nodes[0].pend;
+		}
+	}
+	if (nn > 0 && root_node.pend > root_node.pbegin) {
 		// the root's checklists: {root} for both kinds
 		const int32_t root[2] = {0, 0};
-		const WalkUnit r0{0, MODE_FIRST, 1, P.ewald ? 1 : 0, 0, 1};
+		const WalkUnit r0{0, MODE_FIRST, 1, P.ewald ? 1 : 0, 0, 1, 0, 0};
 		WCHK(lists_in.reserve(2, s));
 		WCHK(hipMemcpyAsync(lists_in.p, root, sizeof(root), hipMemcpyHostToDevice, s));
 		units_cap = 1024;
@@ -496,13 +527,21 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			counts_cap = std::max<int64_t>(int64_t(nu) * 8, 2 * counts_cap);
 			WCHK(hipMalloc(&dev_counts, size_t(counts_cap) * sizeof(int32_t)));
 		}
+		if (2 * in_len > codes_cap) {
+			WCHK(hipStreamSynchronize(s));  // the last step's kernels read the old categories
+			if (dev_codes) (void) hipFree(dev_codes);
+			dev_codes = nullptr;
+			codes_cap = std::max<int64_t>(2 * in_len, 2 * codes_cap);
+			WCHK(hipMalloc(&dev_codes, size_t(codes_cap)));
+		}
 		P.lists = lists_in.p;
-		hipLaunchKernelGGL(k_walk_count, dim3(g), dim3(64), 0, s, P, dev_units, nu, dev_counts);
-		WCHK(hipGetLastError());
+		WLAUNCH("walk step set-up");
+		hipLaunchKernelGGL(k_walk_count, dim3(unsigned(nu)), dim3(64), 0, s, P, dev_codes, dev_units, nu, dev_counts);
+		WLAUNCH("k_walk_count");
 		hipLaunchKernelGGL(k_walk_sizes, dim3(g), dim3(64), 0, s, P.nodes, dev_units, nu, dev_counts, P.ewald, L.sizes);
-		WCHK(hipGetLastError());
+		WLAUNCH("k_walk_sizes");
 		hipLaunchKernelGGL(k_walk_nodes_of, dim3(g), dim3(64), 0, s, dev_units, nu, L.nodes_of);
-		WCHK(hipGetLastError());
+		WLAUNCH("k_walk_nodes_of");
 		{
 			// all columns in one scan, then each column's start taken off
 			const int64_t m = int64_t(nu) * NCOL;
@@ -517,9 +556,9 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			size_t b = scan_tmp_bytes;
 			WCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, L.sizes, L.offs, m, s));
 			hipLaunchKernelGGL(k_walk_totals, dim3(1), dim3(64), 0, s, L.sizes, L.offs, nu, dev_tot);
-			WCHK(hipGetLastError());
+			WLAUNCH("k_walk_totals");
 			hipLaunchKernelGGL(k_walk_colfix, dim3(unsigned((m + 255) / 256)), dim3(256), 0, s, dev_tot, nu, L.offs);
-			WCHK(hipGetLastError());
+			WLAUNCH("k_walk_colfix");
 		}
 		int64_t t[NCOL];
 		WCHK(hipMemcpyAsync(t, dev_tot, sizeof(t), hipMemcpyDeviceToHost, s));
@@ -550,16 +589,17 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			W.cap[K_N] = lists_out.cap;
 			W.next = lists_out.p;
 			W.overflow = dev_overflow;
-			hipLaunchKernelGGL(k_walk_write, dim3(g), dim3(64), 0, s, P, dev_units, nu, dev_counts, L.offs, W);
-			WCHK(hipGetLastError());
+			hipLaunchKernelGGL(k_walk_write, dim3(unsigned(nu)), dim3(64), 0, s, P, dev_codes, dev_units, nu, dev_counts, L.offs, W);
+			WLAUNCH("k_walk_write");
 			hipLaunchKernelGGL(k_walk_units, dim3(g), dim3(64), 0, s, P.nodes, dev_units, nu, dev_counts, L.offs, P.ewald, dev_next_units,
 					int(next_units_cap));
-			WCHK(hipGetLastError());
+			WLAUNCH("k_walk_units");
 		}
 		std::swap(lists_in, lists_out);
 		std::swap(dev_units, dev_next_units);
 		std::swap(units_cap, next_units_cap);
 		nu = int(t[COL_UNITS]);
+		in_len = t[COL_NEXT];
 	}
 	WCHK(hipMemcpyAsync(&overflow, dev_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
 	WCHK(hipStreamSynchronize(s));
@@ -579,7 +619,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		WCHK(hipMemsetAsync(dev_fill, 0, size_t(K_N * np) * sizeof(int64_t), s));
 		for (const StepLog& L : log) {
 			hipLaunchKernelGGL(k_walk_node_len, dim3(unsigned((L.nu + 63) / 64)), dim3(64), 0, s, L.nodes_of, L.nu, L.sizes, nn, dev_node_len);
-			WCHK(hipGetLastError());
+			WLAUNCH("k_walk_node_len");
 		}
 		int64_t* dev_ptr = nullptr;
 		WCHK(hipMalloc(&dev_ptr, size_t(K_N * np) * sizeof(int64_t)));
@@ -667,11 +707,13 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	}
 done:
 #undef WCHK
+#undef WLAUNCH
 	(void) hipStreamSynchronize(s);
 	if (dev_nodes) (void) hipFree(dev_nodes);
 	if (dev_units) (void) hipFree(dev_units);
 	if (dev_next_units) (void) hipFree(dev_next_units);
 	if (dev_counts) (void) hipFree(dev_counts);
+	if (dev_codes) (void) hipFree(dev_codes);
 	if (dev_tot) (void) hipFree(dev_tot);
 	if (dev_node_len) (void) hipFree(dev_node_len);
 	if (dev_fill) (void) hipFree(dev_fill);

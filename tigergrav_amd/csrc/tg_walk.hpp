@@ -37,6 +37,15 @@ struct DeviceLists {
 // t, if given, receives the seconds spent in set-up, the walk steps, list assembly with the copy
 // to the host, and the census.  With keep, nothing is copied to the host: the lists stay on the
 // device in *keep (released first), T.lists is left empty and T.census zero.
-hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t = nullptr, DeviceLists* keep = nullptr);
+// With nodes (a device table of nn nodes, which device_walk then owns and frees or hands to keep),
+// T supplies only its parameters: the tree itself may be on the device only.
+hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t = nullptr, DeviceLists* keep = nullptr,
+		WalkNode* nodes = nullptr, int64_t nn = 0);
+
+struct DeviceTree;
+// the walk's node table of a device-built tree, made on the device (*out: nn nodes, device memory
+// the caller frees): the centres, radii and children, and each node's sink span from a bottom-up
+// union of its non-empty leaves' particle boxes, as Tree::sink_spans forms it
+hipError_t walk_nodes_device(const DeviceTree& D, WalkNode** out, hipStream_t s, std::string& err);
 
 }  // namespace tg
