@@ -100,8 +100,11 @@ int tg_solver_info(tg_solver* s, tg_solve_info* info);
  * the PP-direct work items (first sink, count | near-prefix length << 7, list begin, list end),
  * and "pp_d_list", their flattened source indices; for a device solver the device work of kind K
  * (tg_solve_info order) "work_items_K" (leaf kinds: int4 items as above; node kinds: int2 waves of
- * list entries), "work_list_K", "work_owner_K" (node kinds: the partial slot of each entry), and
- * "slot_node", "slot_ptr" (each slot's node; each node's first slot)).  Returns the size in bytes;
+ * list entries), "work_list_K", "work_owner_K" (node kinds: the partial slot of each entry),
+ * "slot_node", "slot_ptr" (each slot's node; each node's first slot), and the tree arrays the
+ * solve reads: "solve_parts" (int4 x, y, z, mass bits), "solve_multis", "solve_nodepos",
+ * "solve_level", "solve_parent", "solve_leaf_items" (node, first particle, end, path offset),
+ * "solve_path", "solve_pleaf" (each particle's leaf), "solve_order").  Returns the size in bytes;
  * dst may be NULL to query it. */
 int64_t tg_solver_export(tg_solver* s, const char* name, void* dst, int64_t capacity);
 const char* tg_solver_last_error(void);

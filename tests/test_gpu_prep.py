@@ -1,8 +1,10 @@
-"""The list work built on the device from the device walk's lists (tigergrav_amd/csrc/tg_prep.hip)
-against the work the host builds from the host walk's lists (prepare_work,
-tigergrav_amd/csrc/tg_upload.cpp): on the same particles every work array -- leaf items in their
-sorted order, expanded PP and CP lists with PP direct's near prefix first, the CC/CP waves, the
-partial slot of every entry, each slot's node, each node's first slot -- is identical element for
+"""The solve's work built on the device from the device tree and the device walk's lists
+(tigergrav_amd/csrc/tg_prep.hip) against the work the host builds from the host walk's lists and
+a host copy of the tree (prepare_work, tigergrav_amd/csrc/tg_upload.cpp): on the same particles
+every work array -- leaf items in their sorted order, expanded PP and CP lists with PP direct's
+near prefix first, the CC/CP waves, the partial slot of every entry, each slot's node, each node's
+first slot -- and every tree array the solve reads (particle records, multipoles, centres, levels,
+parents, leaves, root paths, each particle's leaf, the input order) is identical element for
 element, the interaction census agrees, and the solves give the same forces bit for bit.  The two
 walks give identical lists (tests/test_gpu_walk.py), so any difference is the preparation's."""
 import numpy as np
@@ -44,6 +46,10 @@ def _work(s):
             d[f"work_owner_{k}"] = s.export(f"work_owner_{k}")
     d["slot_node"] = s.export("slot_node")
     d["slot_ptr"] = s.export("slot_ptr")
+    # the tree arrays the solve reads (made on the device from a device-kept tree, on the host
+    # from a host tree)
+    for name in ("parts", "multis", "nodepos", "level", "parent", "leaf_items", "path", "pleaf", "order"):
+        d["solve_" + name] = s.export("solve_" + name)
     return d
 
 
@@ -60,6 +66,8 @@ def test_device_work_matches_host_work(case):
     ia, ib = host.info(), dev.info()
     assert ia["count"] == ib["count"]
     assert ia["items"] == ib["items"] and ia["list_len"] == ib["list_len"]
+    for key in ("n_nodes", "n_leaves", "max_level"):
+        assert ia[key] == ib[key], key
     # PP direct carries a near prefix somewhere, and the waves pack more than one chunk somewhere
     pp = b["work_items_0"].reshape(-1, 4)
     assert pp.shape[0] > 0 and np.any((pp[:, 1] >> 7) > 0)

@@ -30,6 +30,22 @@ __host__ __device__ constexpr int pp_item_y(int count, int64_t near_len) {
 	return int(unsigned(count) | (unsigned(near_len >= PP_NEAR_ALL ? PP_NEAR_ALL : near_len) << PP_NEAR_SHIFT));
 }
 
+// The dispatch order of leaf items: by cost class, heaviest class first, items of one class in leaf
+// (tree) order, so that items running together share most of their sources in the L2.  The cost
+// is the list length times the sink count rounded up to a power of two; its class is its bit
+// length.
+__host__ __device__ constexpr int item_class(int64_t list_len, int cnt) {
+	int sh = 0;
+	while (cnt > 1 && (1 << sh) < cnt) sh++;
+	uint64_t c = uint64_t(list_len) << sh;
+	int b = 0;
+	while (c) {
+		b++;
+		c >>= 1;
+	}
+	return b;
+}
+
 struct MultiSrc {
 	float m[17];
 	int32_t x[3];

@@ -15,7 +15,7 @@
 // length of the sink leaf's box, the test made in FP64 with the host's operations.  Scans over
 // the leaves give the offsets; the PP ranges are expanded (PP direct: near ranges first, both
 // groups in list order), the items of <= 64 sinks written, and every kind's items put in the
-// order of a stable sort by descending cost (a radix sort of (cost, index) pairs).  The PC lists
+// order of a stable sort by descending cost class (a radix sort of (class, index) pairs).  The PC lists
 // are the walk's own.
 #include <hipcub/hipcub.hpp>
 
@@ -310,15 +310,17 @@ __device__ bool range_near(const LeafIn& A, int4 slo, int4 shi, int b, int e) {
 }
 
 // per leaf and kind: entries (list ints for PC, expanded sources for PP), near-prefix length (PP
-// direct) and items; census sums (sinks x entries) and the raw list length of the leaves (to
-// check that the leaves hold every entry)
+// direct) and items; per leaf the census terms (sinks x entries, sums[q]) and the raw list
+// lengths (sums[4 + q], to check that the leaves hold every entry), summed afterwards by
+// reductions (one atomic per leaf on a shared total would serialise at one cache line)
 __global__ __launch_bounds__(64) void k_leaf_count(LeafIn A, int64_t* __restrict__ lent, int64_t* __restrict__ lnear, int64_t* __restrict__ litem,
-		unsigned long long* __restrict__ sums) {
+		int64_t* __restrict__ sums) {
 	const int64_t i = blockIdx.x;
 	const int lane = threadIdx.x;
 	const int64_t np = A.nl + 1;
 	if (i == A.nl) {
 		if (lane < 4) lent[lane * np + i] = litem[lane * np + i] = 0;
+		if (lane < 8) sums[lane * np + i] = 0;
 		if (lane == 0) lnear[i] = 0;
 		return;
 	}
@@ -345,8 +347,8 @@ __global__ __launch_bounds__(64) void k_leaf_count(LeafIn A, int64_t* __restrict
 			lent[q * np + i] = ent;
 			litem[q * np + i] = ent > 0 ? (ns + 63) / 64 : 0;
 			if (q == 1) lnear[i] = nearl;
-			atomicAdd(sums + q, (unsigned long long) (ns * ent));
-			atomicAdd(sums + 4 + q, (unsigned long long) (ke - kb));
+			sums[q * np + i] = ns * ent;
+			sums[(4 + q) * np + i] = ke - kb;
 		}
 	}
 }
@@ -358,9 +360,6 @@ struct LeafOut {
 	int* idx[4];
 };
 
-__device__ __forceinline__ unsigned long long item_cost(int64_t len, int cnt) {
-	return (unsigned long long) (len * (int64_t(1) << (cnt <= 1 ? 0 : 32 - __clz(cnt - 1))));
-}
 
 // the expanded PP lists and every kind's items, in leaf order: one wave per leaf
 __global__ __launch_bounds__(64) void k_leaf_write(LeafIn A, const int64_t* __restrict__ lent, const int64_t* __restrict__ olent,
@@ -400,7 +399,7 @@ __global__ __launch_bounds__(64) void k_leaf_write(LeafIn A, const int64_t* __re
 			const int y = (q & 1) ? pp_item_y(cnt, nearl) : cnt;
 			const int64_t it = it0 + k;
 			O.items[q][it] = make_int4(b, y, int(lb), int(le));
-			O.key[q][it] = item_cost(le - lb, cnt);
+			O.key[q][it] = (unsigned long long) item_class(le - lb, cnt);
 			O.idx[q][it] = int(it);
 		}
 	}
@@ -423,7 +422,139 @@ __global__ void k_last(const int64_t* const* __restrict__ arrs, const int64_t* _
 
 unsigned blocks(int64_t n, int per) { return unsigned((n + per - 1) / per); }
 
+// ---- the tree's arrays from a device-built tree ---------------------------------------------------
+
+// the walk's node table: one level of nodes, deepest levels first.  sbox: per node the box of its
+// particles (leaves) or the union of its non-empty children's (Tree::sink_spans); span its
+// diagonal, in FP64 with the host's operations
+__global__ __launch_bounds__(256) void k_tw_walk_nodes(const DNode* __restrict__ nodes, const DMulti* __restrict__ multi, int lb, int nl,
+		double* __restrict__ sbox, WalkNode* __restrict__ wn) {
+#pragma clang fp contract(off)
+	const int j = blockIdx.x * 256 + threadIdx.x;
+	if (j >= nl) return;
+	const int k = lb + j;
+	const DNode nd = nodes[k];
+	const DMulti& m = multi[k];
+	double mn[3] = {1.0, 1.0, 1.0}, mx[3] = {0.0, 0.0, 0.0};
+	const bool empty = nd.pend == nd.pbegin;
+	if (!empty) {
+		if (nd.c0 < 0) {
+			for (int d = 0; d < 3; d++) {
+				mn[d] = m.rmin[d];
+				mx[d] = m.rmax[d];
+			}
+		} else {
+			const int ch[2] = {nd.c0, nd.c1};
+			for (int c = 0; c < 2; c++) {
+				if (nodes[ch[c]].pend == nodes[ch[c]].pbegin) continue;
+				const double* b = sbox + 6 * int64_t(ch[c]);
+				for (int d = 0; d < 3; d++) {
+					mx[d] = mx[d] < b[3 + d] ? b[3 + d] : mx[d];
+					mn[d] = b[d] < mn[d] ? b[d] : mn[d];
+				}
+			}
+		}
+	}
+	double* o = sbox + 6 * int64_t(k);
+	for (int d = 0; d < 3; d++) {
+		o[d] = mn[d];
+		o[3 + d] = mx[d];
+	}
+	float span = 0.0f;
+	if (!empty) {
+		double s2 = 0.0;
+		for (int d = 0; d < 3; d++) s2 += (mx[d] - mn[d]) * (mx[d] - mn[d]);
+		span = float(sqrt(s2));
+	}
+	WalkNode w;
+	w.pbegin = nd.pbegin;
+	w.pend = nd.pend;
+	w.c0 = nd.c0;
+	w.c1 = nd.c1;
+	for (int d = 0; d < 3; d++) w.x[d] = m.x[d];
+	w.r = m.r;
+	w.rmax = span;
+	wn[k] = w;
+}
+
+// particle records (x, y, z, mass bits)
+__global__ __launch_bounds__(256) void k_tw_parts(const int32_t* __restrict__ pos, const float* __restrict__ mass, int64_t n, int4* __restrict__ parts) {
+	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+	if (i < n) parts[i] = make_int4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], __float_as_int(mass[i]));
+}
+
+// per node: source multipole, centre, level, parent; flags of the non-empty leaves and their
+// root-path lengths (arrays of nn + 1, the last zero); the leaf count (empty leaves included)
+__global__ __launch_bounds__(256) void k_tw_nodes(const DNode* __restrict__ nodes, const DMulti* __restrict__ multi, int64_t nn,
+		MultiSrc* __restrict__ multis, int4* __restrict__ npos, int* __restrict__ level, int* __restrict__ parent, int64_t* __restrict__ lflag,
+		int64_t* __restrict__ plen, unsigned long long* __restrict__ n_leaves) {
+	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+	// the leaf count: one atomic per wave
+	const bool is_leaf = i < nn && nodes[i].c0 < 0;
+	const unsigned long long lv = __ballot(is_leaf);
+	if ((threadIdx.x & 63) == 0 && lv) atomicAdd(n_leaves, (unsigned long long) __popcll(lv));
+	if (i > nn) return;
+	if (i == nn) {
+		lflag[i] = plen[i] = 0;
+		return;
+	}
+	const DNode nd = nodes[i];
+	const DMulti& m = multi[i];
+	MultiSrc ms;
+	for (int q = 0; q < 17; q++) ms.m[q] = m.m[q];
+	for (int d = 0; d < 3; d++) ms.x[d] = m.x[d];
+	multis[i] = ms;
+	npos[i] = make_int4(m.x[0], m.x[1], m.x[2], 0);
+	level[i] = nd.level;
+	parent[i] = nd.parent;
+	const bool leaf = nd.c0 < 0;
+	const bool use = leaf && nd.pend > nd.pbegin;
+	lflag[i] = use ? 1 : 0;
+	plen[i] = use ? nd.level + 1 : 0;
+}
+
+// the non-empty leaves in node order: (node, first particle, end, root-path offset) and the paths
+__global__ __launch_bounds__(256) void k_tw_leaves(const DNode* __restrict__ nodes, int64_t nn, const int64_t* __restrict__ lflag,
+		const int64_t* __restrict__ loff, const int64_t* __restrict__ poff, int4* __restrict__ leaf_items, int* __restrict__ path) {
+	const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+	if (i >= nn || !lflag[i]) return;
+	const DNode nd = nodes[i];
+	leaf_items[loff[i]] = make_int4(int(i), nd.pbegin, nd.pend, int(poff[i]));
+	int64_t p = poff[i] + nd.level + 1;
+	for (int a = int(i); a >= 0; a = nodes[a].parent) path[--p] = a;
+}
+
+// each particle's leaf: one wave per leaf
+__global__ __launch_bounds__(64) void k_tw_pleaf(const int4* __restrict__ leaf_items, int* __restrict__ pleaf) {
+	const int4 li = leaf_items[blockIdx.x];
+	for (int p = li.y + int(threadIdx.x); p < li.z; p += 64) pleaf[p] = li.x;
+}
+
 }  // namespace
+
+hipError_t walk_nodes_device(const DeviceTree& D, WalkNode** out, hipStream_t s, std::string& err) {
+	*out = nullptr;
+	const int64_t nn = D.nn;
+	WalkNode* wn = nullptr;
+	double* sbox = nullptr;
+	hipError_t e = hipMalloc(&wn, size_t(std::max<int64_t>(nn, 1)) * sizeof(WalkNode));
+	if (e == hipSuccess) e = hipMalloc(&sbox, size_t(std::max<int64_t>(nn, 1)) * 6 * sizeof(double));
+	for (size_t l = D.level_begin.size() - 1; e == hipSuccess && l-- > 0;) {
+		const int b = D.level_begin[l], nl = D.level_begin[l + 1] - b;
+		if (nl <= 0) continue;
+		hipLaunchKernelGGL(k_tw_walk_nodes, dim3(blocks(nl, 256)), dim3(256), 0, s, D.nodes, D.multi, b, nl, sbox, wn);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
+	if (sbox) (void) hipFree(sbox);
+	if (e != hipSuccess) {
+		if (wn) (void) hipFree(wn);
+		err = std::string("walk node table: ") + hipGetErrorString(e);
+		return e;
+	}
+	*out = wn;
+	return hipSuccess;
+}
 
 #define PCHK(call)                                                   \
 	do {                                                             \
@@ -434,10 +565,92 @@ unsigned blocks(int64_t n, int per) { return unsigned((n + per - 1) / per); }
 		}                                                            \
 	} while (0)
 
+int Solver::upload_tree_device() {
+	hipStream_t s = stream;
+	const DeviceTree& D = dtree;
+	const int64_t nn = D.nn;
+	if (D.level_begin.size() < 2 || D.level_begin.back() != nn) {
+		err = "device tree work: the level ranges do not cover the nodes";
+		return TG_EINVAL;
+	}
+	if (max_level + 1 > 64) {  // the local-expansion pass's limit
+		err = "tree deeper than the local-expansion pass supports (64 levels)";
+		return TG_EINVAL;
+	}
+	PCHK(d_parts.alloc(std::max<size_t>(size_t(n) * sizeof(int4), 16)));
+	PCHK(d_multis.alloc(std::max<size_t>(size_t(nn) * sizeof(MultiSrc), 16)));
+	PCHK(d_nodepos.alloc(std::max<size_t>(size_t(nn) * sizeof(int4), 16)));
+	PCHK(d_level.alloc(std::max<size_t>(size_t(nn) * 4, 16)));
+	PCHK(d_parent.alloc(std::max<size_t>(size_t(nn) * 4, 16)));
+	PCHK(d_pleaf.alloc(std::max<size_t>(size_t(n) * 4, 16)));
+	PCHK(d_order.alloc(std::max<size_t>(size_t(n) * 8, 16)));
+	PCHK(d_Lnode.alloc(std::max<size_t>(size_t(nn) * 35 * sizeof(double), 16)));
+	DeviceArray flag, loff, plen, poff, cnt, scan_tmp;
+	struct Free {
+		hipStream_t s;
+		DeviceArray* a[6];
+		~Free() {
+			(void) hipStreamSynchronize(s);
+			for (auto* x : a) x->release();
+		}
+	} free_on_return{s, {&flag, &loff, &plen, &poff, &cnt, &scan_tmp}};
+	const int64_t np = nn + 1;
+	PCHK(flag.alloc(np * 8));
+	PCHK(loff.alloc(np * 8));
+	PCHK(plen.alloc(np * 8));
+	PCHK(poff.alloc(np * 8));
+	PCHK(cnt.alloc(64));
+	if (n > 0) {
+		hipLaunchKernelGGL(k_tw_parts, dim3(blocks(n, 256)), dim3(256), 0, s, D.pos, D.mass, n, d_parts.as<int4>());
+		PCHK(hipGetLastError());
+		PCHK(hipMemcpyAsync(d_order.p, D.order, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+	}
+	order_on_device = true;
+	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnt.as<unsigned long long>(), 1);
+	PCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_tw_nodes, dim3(blocks(np, 256)), dim3(256), 0, s, D.nodes, D.multi, nn, d_multis.as<MultiSrc>(), d_nodepos.as<int4>(),
+			d_level.as<int>(), d_parent.as<int>(), flag.as<int64_t>(), plen.as<int64_t>(), cnt.as<unsigned long long>());
+	PCHK(hipGetLastError());
+	for (int k = 0; k < 2; k++) {
+		const int64_t* in = (k ? plen : flag).as<const int64_t>();
+		int64_t* out = (k ? poff : loff).as<int64_t>();
+		size_t need = 0;
+		PCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, np, s));
+		PCHK(scan_tmp.alloc(std::max<size_t>(need, 16)));
+		PCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, need, in, out, np, s));
+	}
+	int64_t tot[3];
+	PCHK(hipMemcpyAsync(tot, loff.as<int64_t>() + nn, 8, hipMemcpyDeviceToHost, s));
+	PCHK(hipMemcpyAsync(tot + 1, poff.as<int64_t>() + nn, 8, hipMemcpyDeviceToHost, s));
+	PCHK(hipMemcpyAsync(tot + 2, cnt.p, 8, hipMemcpyDeviceToHost, s));
+	PCHK(hipStreamSynchronize(s));
+	if (tot[1] > int64_t(INT32_MAX)) {
+		err = "root-to-leaf paths exceed 2^31 entries";
+		return TG_EINVAL;
+	}
+	n_leaf_items = int(tot[0]);
+	n_leaves = tot[2];
+	path_len = tot[1];
+	PCHK(d_leaf_items.alloc(std::max<size_t>(size_t(tot[0]) * sizeof(int4), 16)));
+	PCHK(d_path.alloc(std::max<size_t>(size_t(tot[1]) * 4, 16)));
+	if (nn > 0) {
+		hipLaunchKernelGGL(k_tw_leaves, dim3(blocks(nn, 256)), dim3(256), 0, s, D.nodes, nn, flag.as<const int64_t>(), loff.as<const int64_t>(),
+				poff.as<const int64_t>(), d_leaf_items.as<int4>(), d_path.as<int>());
+		PCHK(hipGetLastError());
+	}
+	if (tot[0] > 0) {
+		hipLaunchKernelGGL(k_tw_pleaf, dim3(unsigned(tot[0])), dim3(64), 0, s, d_leaf_items.as<const int4>(), d_pleaf.as<int>());
+		PCHK(hipGetLastError());
+	}
+	level_begin.assign(D.level_begin.begin(), D.level_begin.end());
+	PCHK(hipStreamSynchronize(s));
+	return TG_OK;
+}
+
 int Solver::upload_lists_device() {
 	hipStream_t s = stream;
 	const DeviceLists& L = dlists;
-	const int64_t nn = int64_t(tree.nodes.size());
+	const int64_t nn = n_nodes;
 	const int64_t nl = n_leaf_items;
 	const int chunk = std::min(NODE_CHUNK_MAX, std::max(1, node_chunk));
 	if (L.nn != nn || !L.ptr_all) {
@@ -636,11 +849,10 @@ int Solver::upload_lists_device() {
 	int64_t* litem = static_cast<int64_t*>(scratch(4 * lp * 8));
 	int64_t* olitem = static_cast<int64_t*>(scratch(4 * lp * 8));
 	int64_t* lnear = static_cast<int64_t*>(scratch(lp * 8));
-	unsigned long long* sums = static_cast<unsigned long long*>(scratch(8 * 8));
-	if (!box || !lent || !olent || !litem || !olitem || !lnear || !sums) PCHK(hipErrorOutOfMemory);
+	int64_t* sums = static_cast<int64_t*>(scratch(8 * lp * 8));   // per leaf, 8 columns
+	int64_t* sum_tot = static_cast<int64_t*>(scratch(8 * 8));     // their totals
+	if (!box || !lent || !olent || !litem || !olitem || !lnear || !sums || !sum_tot) PCHK(hipErrorOutOfMemory);
 	li.box = box;
-	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, sums, 8);
-	PCHK(hipGetLastError());
 	if (nl > 0) {
 		hipLaunchKernelGGL(k_leaf_boxes, dim3(unsigned(nl)), dim3(64), 0, s, li.leaf_items, li.parts, box);
 		PCHK(hipGetLastError());
@@ -650,6 +862,13 @@ int Solver::upload_lists_device() {
 	for (int q = 0; q < 4; q++) {
 		PCHK(exclusive_scan(lent + q * lp, olent + q * lp, lp));
 		PCHK(exclusive_scan(litem + q * lp, olitem + q * lp, lp));
+	}
+	for (int k = 0; k < 8; k++) {
+		size_t need = 0;
+		PCHK(hipcub::DeviceReduce::Sum(nullptr, need, sums + k * lp, sum_tot + k, lp, s));
+		void* tmp_r = scratch(need);
+		if (!tmp_r) PCHK(hipErrorOutOfMemory);
+		PCHK(hipcub::DeviceReduce::Sum(tmp_r, need, sums + k * lp, sum_tot + k, lp, s));
 	}
 	int64_t llen[4], nitems[4];
 	uint64_t census[8];
@@ -663,7 +882,7 @@ int Solver::upload_lists_device() {
 			at.push_back(nl);
 		}
 		for (int k = 0; k < 8; k++) {
-			arrs.push_back(reinterpret_cast<const int64_t*>(sums));
+			arrs.push_back(sum_tot);
 			at.push_back(k);
 		}
 		int64_t v[16];
@@ -707,7 +926,8 @@ int Solver::upload_lists_device() {
 		hipLaunchKernelGGL(k_leaf_write, dim3(unsigned(nl)), dim3(64), 0, s, li, lent, olent, lnear, olitem, lo);
 		PCHK(hipGetLastError());
 	}
-	// heaviest items first, ties in leaf order: a stable radix sort of (cost, index), descending
+	// heaviest cost class first, leaf order within a class: a stable radix sort of (class, index),
+	// descending
 	for (int q = 0; q < 4; q++) {
 		KindWork& w = work[LEAF_KIND[q]];
 		const int m = int(nitems[q]);
@@ -719,10 +939,10 @@ int Solver::upload_lists_device() {
 		int* idx_out = static_cast<int*>(scratch(size_t(m) * 4));
 		if (!key_out || !idx_out) PCHK(hipErrorOutOfMemory);
 		size_t need = 0;
-		PCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, lo.key[q], key_out, lo.idx[q], idx_out, m, 0, 40, s));
+		PCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, lo.key[q], key_out, lo.idx[q], idx_out, m, 0, 8, s));
 		void* sort_tmp = scratch(need);
 		if (!sort_tmp) PCHK(hipErrorOutOfMemory);
-		PCHK(hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, need, lo.key[q], key_out, lo.idx[q], idx_out, m, 0, 40, s));
+		PCHK(hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, need, lo.key[q], key_out, lo.idx[q], idx_out, m, 0, 8, s));
 		hipLaunchKernelGGL(k_gather_items, dim3(blocks(m, 256)), dim3(256), 0, s, lo.items[q], idx_out, int64_t(m), w.items.as<int4>());
 		PCHK(hipGetLastError());
 	}

@@ -36,7 +36,7 @@ hipError_t DeviceArray::alloc(size_t n) {
 }
 
 void DeviceArray::release() {
-	if (p && owned) hipFree(p);
+	if (p && owned) (void) hipFree(p);
 	p = nullptr;
 	bytes = 0;
 	owned = true;
@@ -319,6 +319,7 @@ Solver::~Solver() {
 			w.owner.release();
 		}
 		dlists.release();
+		dtree.release();
 		if (stream2) {
 			hipStreamSynchronize(stream2);
 			hipStreamDestroy(stream2);
@@ -346,10 +347,15 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 	double t0 = now();
 	uniform_mass = (mass == nullptr);
 	double t1, t2;
+	// with the walk on the device too, the tree stays there: the walk and the solve's arrays are
+	// made from it on the device, and the host copy only on demand (host_tree)
+	const bool keep_tree = gpu_tree && gpu_walk && device >= 0;
+	tree_on_device = false;
 	if (gpu_tree && device >= 0) {
 		double tt[2] = {0.0, 0.0};
-		const hipError_t e = device_build_tree(tree, x, mass, count, P, m, device, stream, err, tt, on_device);
+		const hipError_t e = device_build_tree(tree, x, mass, count, P, m, device, stream, err, tt, on_device, keep_tree ? &dtree : nullptr);
 		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
+		tree_on_device = keep_tree;
 		t1 = t0 + tt[0];
 		t2 = now();
 	} else {
@@ -358,13 +364,22 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 		tree.compute_multipoles();
 		t2 = now();
 	}
+	if (!tree_on_device) dtree.release();
+	n_nodes = tree_on_device ? dtree.nn : int64_t(tree.nodes.size());
+	n_leaves = tree_on_device ? 0 : int64_t(tree.leaves.size());  // counted by upload_tree_device
+	max_level = tree_on_device ? int(dtree.level_begin.size()) - 2 : tree.max_level;
 	// the work lists of the last build may be views of the walk's lists, released below
 	for (auto& w : work)
 		if (!w.list.owned) w.list.release();
 	lists_on_device = false;
 	if (gpu_walk) {
 		// the lists stay on the device; the work is built from them there (upload_lists_device)
-		const hipError_t e = device_walk(tree, device, stream, err, t_detail, &dlists);
+		WalkNode* wn = nullptr;
+		if (tree_on_device) {
+			const hipError_t e = walk_nodes_device(dtree, &wn, stream, err);
+			if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
+		}
+		const hipError_t e = device_walk(tree, device, stream, err, t_detail, &dlists, wn, n_nodes);
 		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
 		lists_on_device = true;
 	} else {
@@ -384,6 +399,10 @@ template <class T, class A>
 static hipError_t put(DeviceArray& d, const std::vector<T, A>& h, hipStream_t s);
 
 int Solver::update_particles(const int32_t* x, const float* mass) {
+	{
+		const int rc = host_tree();  // the moments are recomputed on the host tree
+		if (rc != TG_OK) return rc;
+	}
 	if (tree.nodes.empty() || n == 0) {
 		err = "update_particles before set_particles";
 		return TG_EINVAL;
@@ -485,6 +504,14 @@ static hipError_t put(DeviceArray& d, const std::vector<T, A>& h, hipStream_t s)
 	return hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
 }
 
+int Solver::host_tree() {
+	if (!tree_on_device) return TG_OK;
+	SCHK(hipSetDevice(device));
+	SCHK(device_tree_to_host(dtree, tree, stream, err));
+	tree_on_device = false;
+	return TG_OK;
+}
+
 int Solver::host_lists() {
 	if (!lists_on_device) return TG_OK;
 	SCHK(hipSetDevice(device));
@@ -510,6 +537,23 @@ int Solver::upload() {
 	const int64_t nn = int64_t(T.nodes.size());
 	HostWork W;
 	const double tp0 = now();
+	if (tree_on_device) {
+		// the tree's arrays and the list work, all made on the device
+		adj_checked = false;
+		adj_lists_ready = false;
+		int rc = upload_tree_device();
+		const double tp1 = now();
+		if (rc == TG_OK) rc = upload_lists_device();
+		if (rc != TG_OK) return rc;
+		t_detail[4] = now() - tp0;  // preparation: the tree's arrays and the list work
+		t_detail[5] = 0.0;          // no transfers
+		(void) tp1;
+		SCHK(d_partial.alloc(std::max<size_t>(n_slots * 35 * sizeof(double), 16)));
+		SCHK(d_dL.alloc(std::max<size_t>(size_t(n_nodes) * 35 * sizeof(double), 16)));
+		SCHK(d_out.alloc(std::max<size_t>(n * 4 * sizeof(double), 16)));
+		SCHK(hipStreamSynchronize(stream));
+		return TG_OK;
+	}
 	const bool device_lists = lists_on_device && device >= 0;
 	{
 		// with the lists on the device only the tree's arrays are made here
@@ -555,6 +599,7 @@ int Solver::upload() {
 	adj_checked = false;
 	adj_lists_ready = false;
 	SCHK(put(d_path, path, s));
+	path_len = int64_t(path.size());
 	SCHK(put(d_level, level, s));
 	{
 		// the level-by-level local pass: node parents, level ranges, each particle's leaf
@@ -655,7 +700,7 @@ int Solver::upload() {
 int Solver::enqueue_solve(bool timed) {
 	SCHK(hipSetDevice(device));
 	hipStream_t s = stream;
-	const int64_t nn = int64_t(tree.nodes.size());
+	const int64_t nn = n_nodes;
 	int e = 0;
 	if (timed && solves_recorded - solves_folded >= TIMER_RING) {
 		// the set about to be reused still holds an unfolded solve: add its times first
@@ -748,7 +793,7 @@ int Solver::enqueue_solve(bool timed) {
 	SCHK(mark(e + 2));
 	if (local_pass) {
 		const double phi_self = (-315.0 / 128.0) * m / soft_len;
-		const int maxdepth = tree.max_level + 1;
+		const int maxdepth = max_level + 1;
 		if (local_sweep)
 			hipLaunchKernelGGL(k_l2p_parts, dim3(unsigned((n + 63) / 64)), dim3(64), 0, s, n, d_pleaf.as<int>(), d_nodepos.as<int4>(),
 					d_parts.as<int4>(), d_Lnode.as<double>(), d_out.as<double>(), phi_self, ewald);
@@ -808,13 +853,24 @@ __global__ void k_forces_input_order(const double* __restrict__ out, const int64
 	for (int c = 0; c < 4; c++) f[4 * o + c] = out[4 * i + c];
 }
 
+int Solver::forces_device_order() {
+	// a tree kept on the device left its order in d_order already
+	if (!order_on_device && n > 0) {
+		SCHK(hipSetDevice(device));
+		SCHK(d_order.alloc(size_t(n) * sizeof(int64_t)));
+		SCHK(hipMemcpyAsync(d_order.p, tree.order.data(), size_t(n) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+		SCHK(hipStreamSynchronize(stream));
+		order_on_device = true;
+	}
+	return TG_OK;
+}
+
 int Solver::forces_device(double* out4_dev) {
 	SCHK(hipSetDevice(device));
 	if (n == 0) return TG_OK;
-	if (!order_on_device) {
-		SCHK(d_order.alloc(size_t(n) * sizeof(int64_t)));
-		SCHK(hipMemcpyAsync(d_order.p, tree.order.data(), size_t(n) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
-		order_on_device = true;
+	{
+		const int rc = forces_device_order();
+		if (rc != TG_OK) return rc;
 	}
 	hipLaunchKernelGGL(k_forces_input_order, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, d_out.as<const double>(),
 			d_order.as<const int64_t>(), n, out4_dev);
@@ -825,13 +881,13 @@ int Solver::forces_device(double* out4_dev) {
 
 int Solver::forces(double* out4) {
 	SCHK(hipSetDevice(device));
-	std::vector<double> h(n * 4);
-	if (n > 0) SCHK(hipMemcpyAsync(h.data(), d_out.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+	if (n == 0) return TG_OK;
+	// put into input order on the device, then one copy
+	SCHK(d_fin.alloc(size_t(n) * 4 * sizeof(double)));
+	const int rc = forces_device(d_fin.as<double>());
+	if (rc != TG_OK) return rc;
+	SCHK(hipMemcpyAsync(out4, d_fin.p, size_t(n) * 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
 	SCHK(hipStreamSynchronize(stream));
-	for (int64_t i = 0; i < n; i++) {
-		const int64_t o = tree.order[i];
-		for (int c = 0; c < 4; c++) out4[4 * o + c] = h[4 * i + c];
-	}
 	return TG_OK;
 }
 
@@ -895,6 +951,10 @@ int tg_solver_find_groups(tg_solver* h, double link_len, uint64_t* group, int64_
 	if (!h || h->s.device < 0 || (h->s.n > 0 && !group) || !(link_len > 0.0)) return TG_EINVAL;
 	tg::Solver& S = h->s;
 	const int64_t n = S.n;
+	{
+		const int rc = S.host_tree();  // the candidate search walks the host tree
+		if (rc != TG_OK) return fail(h, rc);
+	}
 	S.fof_group.assign(size_t(n), TG_FOF_NONE);
 	int64_t cand = 0;
 	const hipError_t e = tg::fof_device(S.tree, S.d_parts.as<const int4>(), link_len, S.device, S.stream, S.fof_group.data(), &cand, S.err);
@@ -956,7 +1016,8 @@ int tg_solver_adjoint(tg_solver* h, const float* w, double* grad, double* ms) {
 	if (!h || h->s.device < 0 || (h->s.n > 0 && (!w || !grad))) return TG_EINVAL;
 	tg::Solver& S = h->s;
 	if (!S.adj_checked) {
-		const int rl = S.host_lists();  // the symmetry check and the adjoint's transposes read the lists
+		int rl = S.host_tree();  // the adjoint's set-up reads the tree and the lists on the host
+		if (rl == TG_OK) rl = S.host_lists();
 		if (rl != TG_OK) return fail(h, rl);
 		S.adj_pp_symmetric = S.pp_lists_symmetric();
 		S.adj_checked = true;
@@ -1011,9 +1072,9 @@ int tg_solver_info(tg_solver* h, tg_solve_info* info) {
 	const tg::Tree& T = h->s.tree;
 	std::memset(info, 0, sizeof(*info));
 	info->n = h->s.n;
-	info->n_nodes = int64_t(T.nodes.size());
-	info->n_leaves = int64_t(T.leaves.size());
-	info->max_level = T.max_level;
+	info->n_nodes = h->s.n_nodes;
+	info->n_leaves = h->s.n_leaves;
+	info->max_level = h->s.max_level;
 	const tg::Census& C = T.census;
 	const uint64_t cnt[8] = {C.PP_direct, C.PC_direct, C.CC_direct, C.CP_direct, C.PP_ewald, C.PC_ewald, C.CC_ewald, C.CP_ewald};
 	for (int k = 0; k < 8; k++) info->count[k] = cnt[k];
@@ -1035,7 +1096,8 @@ int tg_solver_info(tg_solver* h, tg_solve_info* info) {
 int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capacity) {
 	if (!h || !name) return TG_EINVAL;
 	if (h->s.device >= 0) {
-		const int rc = h->s.host_lists();
+		int rc = h->s.host_tree();
+		if (rc == TG_OK) rc = h->s.host_lists();
 		if (rc != TG_OK) return fail(h, rc);
 	}
 	const tg::Tree& T = h->s.tree;
@@ -1090,6 +1152,36 @@ int64_t tg_solver_export(tg_solver* h, const char* name, void* dst, int64_t capa
 	TG_EXPORT_LIST(pp_d)
 	TG_EXPORT_LIST(pc_e)
 	TG_EXPORT_LIST(pp_e)
+	else if (nm.rfind("solve_", 0) == 0) {
+		// the tree arrays of the last build as the solve reads them on the device
+		tg::Solver& S = h->s;
+		if (S.device < 0) return TG_EINVAL;
+		const size_t n = size_t(S.n), nnd = size_t(S.n_nodes);
+		const tg::DeviceArray* a = nullptr;
+		size_t bytes = 0;
+		if (nm == "solve_parts") { a = &S.d_parts; bytes = n * 16; }
+		else if (nm == "solve_multis") { a = &S.d_multis; bytes = nnd * sizeof(tg::MultiSrc); }
+		else if (nm == "solve_nodepos") { a = &S.d_nodepos; bytes = nnd * 16; }
+		else if (nm == "solve_level") { a = &S.d_level; bytes = nnd * 4; }
+		else if (nm == "solve_parent") { a = &S.d_parent; bytes = nnd * 4; }
+		else if (nm == "solve_leaf_items") { a = &S.d_leaf_items; bytes = size_t(S.n_leaf_items) * 16; }
+		else if (nm == "solve_path") { a = &S.d_path; bytes = size_t(S.path_len) * 4; }
+		else if (nm == "solve_pleaf") { a = &S.d_pleaf; bytes = n * 4; }
+		else if (nm == "solve_order") {
+			const int rc = S.forces_device_order();
+			if (rc != TG_OK) return fail(h, rc);
+			a = &S.d_order;
+			bytes = n * 8;
+		}
+		else return TG_EINVAL;
+		buf.resize(bytes);
+		if (bytes) {
+			if (hipSetDevice(S.device) != hipSuccess || !a->p || hipMemcpy(buf.data(), a->p, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+				g_solver_err = "tg_solver_export: device copy failed";
+				return TG_EHIP;
+			}
+		}
+	}
 	else if (nm.rfind("work_", 0) == 0 || nm == "slot_node" || nm == "slot_ptr") {
 		// the device work of the last build: work_items_K, work_list_K, work_owner_K (K a kind index)
 		tg::Solver& S = h->s;

@@ -94,6 +94,7 @@ struct Solver {
 	int64_t n_nodes = 0;          // nodes of the current tree
 	int64_t n_leaves = 0;         // its leaves, empty ones included
 	int max_level = 0;            // its deepest level
+	int64_t path_len = 0;         // entries of the root-to-leaf paths (d_path)
 	// the device walk's lists (gpu_walk): the work of every kind is built from them on the device
 	// (upload_lists_device), the CC and PC work lists are views of them, and the host copy in
 	// tree.lists is made only on demand (host_lists)
@@ -133,6 +134,7 @@ struct Solver {
 	int fold_set(int64_t solve);
 	int forces(double* out4);  // N x (phi, gx, gy, gz), input order
 	int forces_device(double* out4_dev);  // the same into device memory, on the solver's stream (waits for it)
+	int forces_device_order();            // d_order: the input index of each tree-order particle
 	DeviceArray d_order;  // input index of each tree-order particle, uploaded by forces_device
 	DeviceArray d_fin;    // forces in input order, for the copy to the host
 	bool order_on_device = false;

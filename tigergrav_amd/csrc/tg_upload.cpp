@@ -386,16 +386,15 @@ int prepare_work(const Tree& T, int64_t n, int ewald, double soft_len, int node_
 			}
 		}
 	}
-	// heaviest leaf items first: the dispatcher hands out work groups in order
-	// (a stable sort: the result is the unique stable order, however the threads merge)
+	// heaviest cost class first, leaf order within a class (item_class): the dispatcher hands out
+	// work groups in order (a stable sort: the result is the unique stable order, however the
+	// threads merge)
 	const int leaf_kinds[4] = {PC_DIRECT, PP_DIRECT, PC_EWALD, PP_EWALD};
 	for (int q = 0; q < 4; q++) {
 		const int k = leaf_kinds[q];
 		__gnu_parallel::stable_sort(items[k].begin(), items[k].end(), [](const int4& a, const int4& b) {
 			const int na = a.y & PP_COUNT_MASK, nb = b.y & PP_COUNT_MASK;
-			const int64_t ca = int64_t(a.w - a.z) * (1 << (na <= 1 ? 0 : 32 - __builtin_clz(na - 1)));
-			const int64_t cb = int64_t(b.w - b.z) * (1 << (nb <= 1 ? 0 : 32 - __builtin_clz(nb - 1)));
-			return ca > cb;
+			return item_class(a.w - a.z, na) > item_class(b.w - b.z, nb);
 		});
 	}
 	
