@@ -6,10 +6,12 @@ Workload (default): 128^3 particles displaced by a Zel'dovich field of a seeded 
 field with P(k) ~ k^-2 (cut at the grid Nyquist), rms displacement 0.3 cell, periodic (Ewald),
 softening 0.02 N^(-1/3), equal masses 1/N.  The timed solve runs at theta = 0.3: the largest
 opening angle whose force error meets RMS |da|/|a| <= 1e-3 against the FP64 direct sum on this
-workload (the `accuracy` field shows it).  The leaf size (16/32/64/128), the CC/CP chunk, the PP
-kernel form and the form of the local pass are autotuned on this workload's own graph-replay time
-(--leaf N with --node-chunk/--pp-persistent/--local-sweep fixes them instead), and the timed steps
-replay the solve from a captured HIP graph.  A second leg, `theta07`, times the same particles at
+workload (the `accuracy` field shows it).  The leaf size (8/16/32/64/128), the CC/CP chunk, the PP
+kernel form, the form of the local pass and the stream overlap are autotuned on this workload's
+own graph-replay time, among the leaf sizes whose error on the same FP64 sample meets the bound
+(the leaf size moves the error too: smaller leaves turn near pairs into multipole interactions);
+--leaf N with --node-chunk/--pp-persistent/--local-sweep/--overlap fixes them instead.  The timed
+steps replay the solve from a captured HIP graph.  A second leg, `theta07`, times the same particles at
 theta = 0.7 (the workload's specified angle; about 10x faster, with RMS error ~0.15, which does not
 meet 1e-3).
 
@@ -195,16 +197,17 @@ def adjoint_leg(s, x, ewald, h, m, forward_ms, steps):
             "loss": "sum_i w_i . g_i, w ~ N(0, I), seed 5"}
 
 
-def build_solver(x, ewald, h, m, theta, leaf, leaves, knobs, host_walk=False):
+def build_solver(x, ewald, h, m, theta, leaf, leaves, knobs, host_walk=False, accept=None):
     """A solver for the particle set at this opening angle, with the leaf size and knobs fixed
-    (leaf > 0) or autotuned over `leaves` on graph-replay time (leaf == 0).  Returns the solver
-    (particles set), the tuned configuration, the tuning table and the host build time (s)."""
+    (leaf > 0) or autotuned over `leaves` on graph-replay time (leaf == 0; with accept, among the
+    leaf sizes it accepts).  Returns the solver (particles set), the tuned configuration, the
+    tuning table and the host build time (s)."""
     from tigergrav_amd.autotune import apply, autotune
     from tigergrav_amd.solver import Solver
 
     best, table = None, None
     if leaf == 0:
-        best, table = autotune(x, ewald, h, m, theta, leaves=leaves)
+        best, table = autotune(x, ewald, h, m, theta, leaves=leaves, accept=accept)
         leaf = best["leaf"]
     else:
         best = {"leaf": leaf}
@@ -259,7 +262,7 @@ def profile_command(workload, theta, best, steps=3):
                     ("overlap", "--overlap")):
         if best.get(k) is not None:
             cmd += f" {flag} {best[k]}"
-    return cmd + f" --steps {steps} --warmup 1 --no-cpu --no-theta07 --accuracy-sample 1000"
+    return cmd + f" --steps {steps} --warmup 1 --no-cpu --no-theta07 --no-step --accuracy-sample 1000"
 
 
 def roofline_of(workload, theta, best, count, kms):
@@ -299,6 +302,66 @@ def roofline_of(workload, theta, best, count, kms):
     return roofline
 
 
+def step_leg(s, x, h, m, ewald, reps=10):
+    """Kick and drift (part_vect_kick / part_vect_drift, src/part_vect.cpp:341-436,493-540) of the
+    workload's particles, device-resident, with the solve's forces: ms per call (median of `reps`
+    after a warm call; each call returns its statistics to the host) and the rate at the
+    algorithmic bytes per particle -- kick: forces 32 + velocity 12 + rung 1 read, velocity 12 +
+    rung 1 written; drift: position 12 + velocity 12 read, position 12 written."""
+    import torch
+
+    from tigergrav_amd import step
+
+    n = x.shape[0]
+    dev = torch.device("cuda:0")
+    f = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    L = step._bind()
+    step._check(L.tg_solver_get_forces_device(s._h, f.data_ptr()), "tg_solver_get_forces_device")
+    v = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    rung = torch.zeros(n, dtype=torch.int8, device=dev)
+    xd = torch.as_tensor(np.ascontiguousarray(x, np.int32)).to(dev)
+    P = step.params(n, soft_len=h, ewald=ewald, m=m)
+    out = {}
+    for name, call, nbytes in (("kick", lambda: step.kick(P, f, v, rung, 0, True), 58),
+                               ("drift", lambda: step.drift(P, xd, v, 6, drift_dt=step.rung_to_dt(1.0, 6)), 36)):
+        call()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            call()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.median(ts)) * 1e3
+        gbs = n * nbytes / (ms / 1e3) / 1e9
+        out[name] = {"ms": ms, "bytes_per_particle": nbytes, "GB_per_s": gbs, "frac_of_hbm": gbs / PEAK_HBM_GBS}
+    out["note"] = "wall time per call including the statistics' return to the host"
+    return out
+
+
+def fof_leg(b=0.2):
+    """Friends-of-friends groups (tree::find_groups) at linking length b on a clustered set of the
+    same size (the near-lattice workload has almost no pairs within 0.2 of the mean spacing): 2M
+    particles in 8 Plummer spheres and a uniform background, periodic, leaf 32.  The first call
+    (with the host copy of the device tree it needs) and a repeat."""
+    from tigergrav_amd import ics
+    from tigergrav_amd.solver import Solver
+
+    x = ics.plummer_spheres(2 ** 21, seed=2)
+    n = x.shape[0]
+    s = Solver(ewald=True, soft_len=ics.soft_len(n), m=1.0 / n, theta=0.7, parts_per_node=32, device=0)
+    s.set_particles(x)
+    t0 = time.perf_counter()
+    g, ng = s.find_groups(b)
+    t1 = time.perf_counter()
+    g2, ng2 = s.find_groups(b)
+    t2 = time.perf_counter()
+    assert ng == ng2 and np.array_equal(g, g2)
+    s.close()
+    return {"workload": "8 Plummer spheres + background, 2,097,152 particles", "b": b, "groups": int(ng),
+            "linked_particles": int(np.sum(g != 0xFFFFFFFFFFFFFF)), "s_first": t1 - t0, "s_repeat": t2 - t1}
+
+
 def accuracy_of(f, sinks, fref, ns):
     from tigergrav_amd.accuracy import error_stats
 
@@ -318,8 +381,12 @@ def main():
     ap.add_argument("--leaf", type=int, default=0,
                     help="particles per leaf; 0 autotunes leaf (16, 32, 64, 128), CC/CP chunk, PP kernel form and "
                          "local pass")
-    ap.add_argument("--wide-autotune", action="store_true", help="autotune over leaves of 8 as well")
+    ap.add_argument("--wide-autotune", action="store_true", help="autotune over leaves of 4, 12, 24 and 48 as well")
+    ap.add_argument("--max-rms", type=float, default=1e-3,
+                    help="the autotuner takes only leaf sizes whose RMS |da|/|a| on the accuracy sample is at most this "
+                         "(the metric's bound; 0 takes every leaf size)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-step", action="store_true", help="skip the kick/drift and friends-of-friends legs")
     ap.add_argument("--pp-persistent", type=int, default=None, choices=(0, 1),
                     help="with a fixed --leaf: the persistent (1) or one-group-per-item (0) PP kernel")
     ap.add_argument("--node-chunk", type=int, default=None, help="with a fixed --leaf: CC/CP list entries per partial slot")
@@ -365,11 +432,24 @@ def main():
     N = x.shape[0]
     h, m = ics.soft_len(N), 1.0 / N
     # the adjoint handles leaves of at most 64 particles
-    leaves = tuple(l for l in ((8,) if args.wide_autotune else ()) + (16, 32, 64, 128) if l <= 64 or not args.adjoint)
+    wide = (4, 12, 24, 48) if args.wide_autotune else ()
+    leaves = tuple(sorted(l for l in (8, 16, 32, 64, 128) + wide if l <= 64 or not args.adjoint))
     knobs = {"node_chunk": args.node_chunk, "pp_persistent": args.pp_persistent, "local_sweep": args.local_sweep,
              "overlap": args.overlap}
+    # the accuracy reference: FP64 direct sum (converged Ewald when periodic) on the GPU for a seeded
+    # sample of sinks; the autotuner keeps the leaf sizes that meet the bound on it
+    ns = min(N, args.accuracy_sample)
+    sinks = np.sort(np.random.default_rng(7).choice(N, ns, replace=False)) if ns < N else np.arange(N)
+    t0 = time.perf_counter()
+    fref = direct_forces(x, sinks, ewald, h, m)
+    t_ref = time.perf_counter() - t0
+    accept = None
+    if args.max_rms > 0:
+        def accept(solver):
+            rms = accuracy_of(solver.forces(), sinks, fref, ns)["rms"]
+            return rms <= args.max_rms, {"rms": rms, "bound": args.max_rms}
     free0 = device_free_bytes()
-    s, best, tuning, t_host = build_solver(x, ewald, h, m, args.theta, args.leaf, leaves, knobs, args.host_walk)
+    s, best, tuning, t_host = build_solver(x, ewald, h, m, args.theta, args.leaf, leaves, knobs, args.host_walk, accept)
     info = s.info()
     count = info["count"]
     total = sum(count.values())
@@ -396,8 +476,7 @@ def main():
         "roofline": roofline_of(args.workload, args.theta, best, count, kms),
         "profile_cmd": profile_command(args.workload, args.theta, best),
     }
-    # force accuracy of the solve: FP64 direct sum (converged Ewald when periodic) on the GPU for a
-    # seeded sample of sinks, after the timed region
+    # force accuracy of the timed solve against the reference above
     f = s.forces()
     # end to end: host particles to forces.  The first build on a fresh solver allocates its device
     # buffers; a time step rebuilds on the same solver, so the steady state is the median of three
@@ -413,13 +492,9 @@ def main():
     reb.sort(key=lambda r: r["total"])
     out["build_s"]["rebuild"] = reb[1]
     out["s_per_force_solve_end_to_end"] = reb[1]["total"] + ms / 1e3
-    ns = min(N, args.accuracy_sample)
-    sinks = np.sort(np.random.default_rng(7).choice(N, ns, replace=False)) if ns < N else np.arange(N)
-    t0 = time.perf_counter()
-    fref = direct_forces(x, sinks, ewald, h, m)
     out["accuracy"] = accuracy_of(f, sinks, fref, ns)
     out["accuracy"].update({"reference": "FP64 direct sum over all N sources" + (", converged Ewald sum" if ewald else ""),
-                            "s": time.perf_counter() - t0})
+                            "s": t_ref})
     # device memory held by the solver (all of it stays allocated between solves): resident data
     # after the forward, and after the adjoint's buffers are added
     fp = {"forward_bytes": free0 - device_free_bytes()}
@@ -429,6 +504,9 @@ def main():
         fp["forward_plus_adjoint_bytes"] = free0 - device_free_bytes()
     fp["method"] = "hipMemGetInfo difference from before the solver was created (the accuracy checks free their buffers)"
     out["hbm_footprint"] = fp
+    if not args.no_step:
+        out["step"] = step_leg(s, x, h, m, ewald)
+        out["fof"] = fof_leg()
     if not args.no_cpu:
         out["cpu_baseline"], f_cpu = cpu_baseline(s, ewald, h, m, args.theta, best["leaf"])
         if f_cpu is not None:

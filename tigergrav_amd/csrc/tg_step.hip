@@ -5,14 +5,15 @@
 // One thread per particle.  Every per-particle expression keeps the reference's types and operand
 // order -- float velocities, FP64 forces, the float conversions its vect<float> operators make --
 // with no contraction, so velocities, rungs and positions come out as the reference's.  The
-// statistics are summed per work group (tree reduction in LDS) and then over the groups in order
-// by one more work group: deterministic, though not the reference's particle order.
+// statistics are summed per work group (tree reduction in LDS) and then over the groups in a fixed
+// order by one more work group: deterministic, though not the reference's particle order.
 #include "tg_step.h"
 
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -173,20 +174,33 @@ __global__ __launch_bounds__(BLOCK) void k_drift(int64_t n, int32_t* __restrict_
 	group_sum(s, partial, nullptr, 0, nullptr);
 }
 
-// the groups' partials added in group order
-__global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ partial, const int* __restrict__ rung_part, int ng,
+// the groups' partials: 256 threads each add a contiguous run of groups in group order, then
+// thread 0 adds the 256 run sums in order (a fixed order, so the totals are deterministic)
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partial, const int* __restrict__ rung_part, int ng,
 		double* __restrict__ out) {
-	const int k = threadIdx.x;
-	if (k < NSTAT) {
-		double a = 0.0;
-		for (int g = 0; g < ng; g++) a += partial[NSTAT * g + k];
-		out[k] = a;
+	__shared__ double run[NSTAT][256];
+	__shared__ int rrun[256];
+	const int t = threadIdx.x;
+	const int per = (ng + 255) / 256;
+	const int g0 = t * per, g1 = g0 + per < ng ? g0 + per : ng;
+	double a[NSTAT] = {0, 0, 0, 0, 0, 0, 0, 0};
+	int r = 0;
+	for (int g = g0; g < g1; g++) {
+		for (int k = 0; k < NSTAT; k++) a[k] += partial[NSTAT * g + k];
+		if (rung_part) r = rung_part[g] > r ? rung_part[g] : r;
 	}
-	if (k == NSTAT) {
-		int r = 0;
-		if (rung_part)
-			for (int g = 0; g < ng; g++) r = rung_part[g] > r ? rung_part[g] : r;
-		out[NSTAT] = double(r);
+	for (int k = 0; k < NSTAT; k++) run[k][t] = a[k];
+	rrun[t] = r;
+	__syncthreads();
+	if (t < NSTAT) {
+		double s = 0.0;
+		for (int q = 0; q < 256; q++) s += run[t][q];
+		out[t] = s;
+	}
+	if (t == NSTAT) {
+		int m = 0;
+		for (int q = 0; q < 256; q++) m = rrun[q] > m ? rrun[q] : m;
+		out[NSTAT] = double(m);
 	}
 }
 
@@ -199,6 +213,44 @@ __global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ 
 			goto done;                                        \
 		}                                                     \
 	} while (0)
+
+// Per-device scratch of the calls (group partials, the rung maxima, the error flag, the totals),
+// kept between calls and grown when a larger particle count needs it: a call allocates nothing in
+// the steady state of a time step.  Calls are serialised by the mutex.
+struct Scratch {
+	double* partial = nullptr;
+	int* rung_part = nullptr;
+	int* bad = nullptr;
+	double* sum = nullptr;
+	int cap = 0;  // groups
+};
+std::mutex g_scratch_mtx;
+std::vector<Scratch> g_scratch;
+
+hipError_t scratch_for(int device, int ng, Scratch** out) {
+	if (size_t(device) >= g_scratch.size()) g_scratch.resize(size_t(device) + 1);
+	Scratch& S = g_scratch[size_t(device)];
+	if (ng > S.cap || !S.bad) {
+		if (S.partial) (void) hipFree(S.partial);
+		if (S.rung_part) (void) hipFree(S.rung_part);
+		if (S.bad) (void) hipFree(S.bad);
+		if (S.sum) (void) hipFree(S.sum);
+		S = Scratch();
+		const int cap = ng > 1024 ? ng : 1024;
+		hipError_t e = hipMalloc(&S.partial, size_t(cap) * NSTAT * sizeof(double));
+		if (e == hipSuccess) e = hipMalloc(&S.rung_part, size_t(cap) * sizeof(int));
+		if (e == hipSuccess) e = hipMalloc(&S.bad, sizeof(int));
+		if (e == hipSuccess) e = hipMalloc(&S.sum, (NSTAT + 1) * sizeof(double));
+		if (e != hipSuccess) return e;
+		S.cap = cap;
+	}
+	*out = &S;
+	return hipSuccess;
+}
+
+__global__ void k_clear_flag(int* f) {
+	if (threadIdx.x == 0) *f = 0;
+}
 
 bool bad_params(const tg_step_params* p) {
 	if (!p) return true;
@@ -223,6 +275,8 @@ int tg_kick(const tg_step_params* p, int64_t n, const tg_force* f, float* v, int
 	int* rung_part = nullptr;
 	int* bad = nullptr;
 	double* sum = nullptr;
+	Scratch* S = nullptr;
+	std::lock_guard<std::mutex> lock(g_scratch_mtx);
 	double hs[NSTAT + 1] = {};
 	int hbad = 0;
 	KickArgs A{};
@@ -245,14 +299,15 @@ int tg_kick(const tg_step_params* p, int64_t n, const tg_force* f, float* v, int
 		A.dt2[r] = p->kick_dt2[r];
 	}
 	STCHK(hipSetDevice(p->device));
-	STCHK(hipMalloc(&partial, size_t(ng) * NSTAT * sizeof(double)));
-	STCHK(hipMalloc(&rung_part, size_t(ng) * sizeof(int)));
-	STCHK(hipMalloc(&bad, sizeof(int)));
-	STCHK(hipMalloc(&sum, (NSTAT + 1) * sizeof(double)));
-	STCHK(hipMemset(bad, 0, sizeof(int)));
+	STCHK(scratch_for(p->device, ng, &S));
+	partial = S->partial;
+	rung_part = S->rung_part;
+	bad = S->bad;
+	sum = S->sum;
+	hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, 0, bad);
 	hipLaunchKernelGGL(k_kick, dim3(ng), dim3(BLOCK), 0, 0, A, n, f, v, rung, partial, rung_part, bad);
 	STCHK(hipGetLastError());
-	hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, 0, partial, rung_part, ng, sum);
+	hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, 0, partial, rung_part, ng, sum);
 	STCHK(hipGetLastError());
 	STCHK(hipMemcpy(hs, sum, sizeof(hs), hipMemcpyDeviceToHost));
 	STCHK(hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
@@ -271,10 +326,6 @@ int tg_kick(const tg_step_params* p, int64_t n, const tg_force* f, float* v, int
 		rc = TG_EINVAL;
 	}
 done:
-	if (partial) (void) hipFree(partial);
-	if (rung_part) (void) hipFree(rung_part);
-	if (bad) (void) hipFree(bad);
-	if (sum) (void) hipFree(sum);
 	return rc;
 }
 
@@ -288,21 +339,22 @@ int tg_drift(const tg_step_params* p, int64_t n, int32_t* x, const float* v, int
 	const int ng = int((n + BLOCK - 1) / BLOCK) > 0 ? int((n + BLOCK - 1) / BLOCK) : 1;
 	double* partial = nullptr;
 	double* sum = nullptr;
+	Scratch* S = nullptr;
+	std::lock_guard<std::mutex> lock(g_scratch_mtx);
 	double hs[NSTAT + 1] = {};
 	// the drift step: the finest active rung's, or drift_dt with cosmology (src/part_vect.cpp:505-506)
 	const double dt = p->cosmic ? drift_dt : p->t_max / double(1 << max_rung);
 	STCHK(hipSetDevice(p->device));
-	STCHK(hipMalloc(&partial, size_t(ng) * NSTAT * sizeof(double)));
-	STCHK(hipMalloc(&sum, (NSTAT + 1) * sizeof(double)));
+	STCHK(scratch_for(p->device, ng, &S));
+	partial = S->partial;
+	sum = S->sum;
 	hipLaunchKernelGGL(k_drift, dim3(ng), dim3(BLOCK), 0, 0, n, x, v, dt, p->m, 1.0 / (a1 * a1), partial);
 	STCHK(hipGetLastError());
-	hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, 0, partial, nullptr, ng, sum);
+	hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, 0, partial, nullptr, ng, sum);
 	STCHK(hipGetLastError());
 	STCHK(hipMemcpy(hs, sum, sizeof(hs), hipMemcpyDeviceToHost));
 	if (ekin) *ekin = hs[7];
 done:
-	if (partial) (void) hipFree(partial);
-	if (sum) (void) hipFree(sum);
 	return rc;
 }
 

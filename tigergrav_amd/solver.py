@@ -116,6 +116,17 @@ class Solver:
         _check(lib().tg_solver_adjoint(self._h, ptr(w), ptr(g), ptr(ms)), "tg_solver_adjoint")
         return (g, float(ms[0])) if with_time else g
 
+    def find_groups(self, link_len):
+        """Friends-of-friends groups of the current particles (tree::find_groups, src/tree.cpp:637-760):
+        (group id per particle in input order, number of groups); link_len in units of the mean
+        particle spacing (the reference's b).  Ids are tree-order indices, as the reference's."""
+        L = lib()
+        L.tg_solver_find_groups.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.POINTER(C.c_int64)]
+        g = np.zeros(self.n, np.uint64)
+        ng = C.c_int64()
+        _check(L.tg_solver_find_groups(self._h, float(link_len), g.ctypes.data, C.byref(ng)), "tg_solver_find_groups")
+        return g, ng.value
+
     def solve(self):
         _check(lib().tg_solver_solve(self._h), "tg_solver_solve")
 
