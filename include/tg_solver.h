@@ -64,7 +64,7 @@ int tg_solver_adjoint(tg_solver* s, const float* w, double* grad, double* ms);
 /* enqueue one force solve on the solver's stream (replaces tree::kick_fmm's kernel calls) */
 int tg_solver_solve(tg_solver* s);
 /* tuning knobs (no reference counterpart): "pp_persistent" 0/1 runs PP direct as a persistent
- * queue-driven kernel; "node_chunk" 1..512 sources per CC/CP partial slot (applies from the next
+ * queue-driven kernel; "node_chunk" 1..4096 (4096, the default: a whole list) sources per CC/CP partial slot (applies from the next
  * tg_solver_set_particles); "resident" work groups of the persistent form; "local_sweep" 1 runs the
  * local-expansion pass level by level (the reference's order, src/tree.cpp:400-401) instead of
  * per-leaf sums over the root path (0, the default); "gpu_walk" 1 (the default

@@ -97,6 +97,7 @@ def test_rebuild_after_device_work_keeps_lists_and_forces():
     s.synchronize()
     f = s.forces()
     t = Solver(ewald=True, soft_len=ics.soft_len(x1.shape[0]), m=1.0 / x1.shape[0], theta=0.5, parts_per_node=32, device=0)
+    t.tune("node_chunk", 64)  # as s was built
     t.set_particles(x2)
     t.solve()
     t.synchronize()

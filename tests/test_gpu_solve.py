@@ -74,7 +74,7 @@ def test_solve_parity(case):
     assert all(v >= 0 for v in ms.values())
 
 
-@pytest.mark.parametrize("chunk", [64, 256, 500])
+@pytest.mark.parametrize("chunk", [64, 256, 500, 4096])
 def test_solve_parity_long_chunks(chunk):
     """CC/CP partial slots of more than 64 list entries: one wave per slot whose lanes each add
     several entries in order before the store.  Periodic Zel'dovich set at theta 0.3 (node lists of
@@ -94,7 +94,7 @@ def test_solve_parity_long_chunks(chunk):
     f = s.forces()
     T = s.tree_arrays()
     cc = np.diff(T["cc_d_ptr"])
-    assert cc.max() > chunk or chunk == 500  # slots longer than a wave occur
+    assert cc.max() > min(chunk, 400)  # slots longer than a wave occur
     ref_tree, _ = ob.oracle_solve({"ewald": 1, "soft_len": h, "m": m}, T)
     ref = np.empty_like(ref_tree)
     ref[T["order"]] = ref_tree

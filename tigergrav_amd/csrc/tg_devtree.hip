@@ -17,6 +17,7 @@
 // of 16, the lanes summed in order at the end), interior nodes by one thread each.  Every
 // floating-point expression keeps the host's operand order and roundings (no contraction).
 #include "tg_devtree.hpp"
+#include "tg_devmem.hpp"
 #include "tg_math.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -447,14 +448,14 @@ struct Buf {
 	void* p = nullptr;
 	size_t bytes = 0;
 	~Buf() {
-		if (p) (void) hipFree(p);
+		if (p) dev_free(p);
 	}
 	hipError_t alloc(size_t b) {
 		if (b <= bytes) return hipSuccess;
-		if (p) (void) hipFree(p);
+		if (p) dev_free(p);
 		p = nullptr;
 		bytes = 0;
-		const hipError_t e = hipMalloc(&p, b < 256 ? 256 : b);
+		const hipError_t e = dev_malloc(&p, b < 256 ? 256 : b);
 		if (e == hipSuccess) bytes = b < 256 ? 256 : b;
 		return e;
 	}
@@ -467,11 +468,11 @@ struct Buf {
 }  // namespace
 
 void DeviceTree::release() {
-	if (pos) (void) hipFree(pos);
-	if (mass) (void) hipFree(mass);
-	if (order) (void) hipFree(order);
-	if (nodes) (void) hipFree(nodes);
-	if (multi) (void) hipFree(multi);
+	if (pos) dev_free(pos);
+	if (mass) dev_free(mass);
+	if (order) dev_free(order);
+	if (nodes) dev_free(nodes);
+	if (multi) dev_free(multi);
 	*this = DeviceTree();
 }
 

@@ -24,6 +24,7 @@
 
 #include "tg_gravity.h"
 #include "tg_solver.hpp"
+#include "tg_devmem.hpp"
 
 namespace tg {
 namespace {
@@ -537,8 +538,8 @@ hipError_t walk_nodes_device(const DeviceTree& D, WalkNode** out, hipStream_t s,
 	const int64_t nn = D.nn;
 	WalkNode* wn = nullptr;
 	double* sbox = nullptr;
-	hipError_t e = hipMalloc(&wn, size_t(std::max<int64_t>(nn, 1)) * sizeof(WalkNode));
-	if (e == hipSuccess) e = hipMalloc(&sbox, size_t(std::max<int64_t>(nn, 1)) * 6 * sizeof(double));
+	hipError_t e = dev_malloc(&wn, size_t(std::max<int64_t>(nn, 1)) * sizeof(WalkNode));
+	if (e == hipSuccess) e = dev_malloc(&sbox, size_t(std::max<int64_t>(nn, 1)) * 6 * sizeof(double));
 	for (size_t l = D.level_begin.size() - 1; e == hipSuccess && l-- > 0;) {
 		const int b = D.level_begin[l], nl = D.level_begin[l + 1] - b;
 		if (nl <= 0) continue;
@@ -546,9 +547,9 @@ hipError_t walk_nodes_device(const DeviceTree& D, WalkNode** out, hipStream_t s,
 		e = hipGetLastError();
 	}
 	if (e == hipSuccess) e = hipStreamSynchronize(s);
-	if (sbox) (void) hipFree(sbox);
+	if (sbox) dev_free(sbox);
 	if (e != hipSuccess) {
-		if (wn) (void) hipFree(wn);
+		if (wn) dev_free(wn);
 		err = std::string("walk node table: ") + hipGetErrorString(e);
 		return e;
 	}

@@ -18,6 +18,7 @@
 #include "tg_gravity.h"
 #include "tg_local.hpp"
 #include "tg_solver.hpp"
+#include "tg_devmem.hpp"
 #include "tg_devtree.hpp"
 #include "tg_fof.hpp"
 #include "tg_upload.hpp"
@@ -30,13 +31,13 @@ hipError_t DeviceArray::alloc(size_t n) {
 	if (n <= bytes && p && owned) return hipSuccess;
 	release();
 	if (n == 0) return hipSuccess;
-	hipError_t e = hipMalloc(&p, n);
+	hipError_t e = dev_malloc(&p, n);
 	if (e == hipSuccess) bytes = n;
 	return e;
 }
 
 void DeviceArray::release() {
-	if (p && owned) (void) hipFree(p);
+	if (p && owned) dev_free(p);
 	p = nullptr;
 	bytes = 0;
 	owned = true;
@@ -277,6 +278,7 @@ Solver::~Solver() {
 	if (stream) {
 		hipSetDevice(device);
 		hipStreamSynchronize(stream);
+		if (stream2) hipStreamSynchronize(stream2);
 		drop_graph();
 		for (auto& e : ev)
 			if (e) hipEventDestroy(e);
@@ -327,6 +329,7 @@ Solver::~Solver() {
 		if (ev_fork) hipEventDestroy(ev_fork);
 		if (ev_join) hipEventDestroy(ev_join);
 		hipStreamDestroy(stream);
+		dev_trim(device);  // the cached blocks this solver's buffers went back to
 	}
 }
 
@@ -336,6 +339,13 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 		return TG_EINVAL;
 	}
 	drop_graph();  // the captured launches point at buffers that may be reallocated
+	if (device >= 0) {
+		// the last solves may still read the buffers this build hands back to the device cache
+		// (tg_devmem.hpp), where the build's own stream would take them again
+		SCHK(hipSetDevice(device));
+		SCHK(hipStreamSynchronize(stream));
+		if (stream2) SCHK(hipStreamSynchronize(stream2));
+	}
 	order_on_device = false;
 	fof_group.clear();
 	n = count;

@@ -42,7 +42,7 @@ struct Solver {
 	double soft_len = 0.0, m = 0.0, theta = 0.7;
 	int parts_per_node = 64;
 	bool uniform_mass = true;
-	int node_chunk = 64;  // sources per CC/CP partial slot (<= NODE_CHUNK_MAX); applies from the next set_particles
+	int node_chunk = 4096;  // sources per CC/CP partial slot (<= NODE_CHUNK_MAX); applies from the next set_particles
 	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
 	bool overlap = true;         // graph replays run the node kinds on a second stream beside the leaf kinds
 	hipStream_t stream2 = nullptr;

@@ -40,7 +40,7 @@ struct HostWork {
 
 // node kinds: list entries per partial slot at most (a chunk longer than 64 is one wave whose lanes
 // take its entries 64 apart)
-constexpr int NODE_CHUNK_MAX = 512;
+constexpr int NODE_CHUNK_MAX = 4096;
 
 // node kinds' chunks are packed into waves greedily, the packing restarted every WAVE_SEGMENT
 // chunks (host and device pack alike)

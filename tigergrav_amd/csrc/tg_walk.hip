@@ -22,6 +22,7 @@
 // the end.  The MAC arithmetic is the host walk's, in FP64 without
 // contraction, so every decision is bit-identical.
 #include "tg_walk.hpp"
+#include "tg_devmem.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -357,19 +358,19 @@ struct Pool {
 		int64_t nc = std::max<int64_t>(n, cap * 2);
 		nc = std::max<int64_t>(nc, 1024);
 		int32_t* q = nullptr;
-		hipError_t e = hipMalloc(&q, size_t(nc) * sizeof(int32_t));
+		hipError_t e = dev_malloc(&q, size_t(nc) * sizeof(int32_t));
 		if (e != hipSuccess) return e;
 		if (p && cap > 0 && keep) {
 			if ((e = hipMemcpyAsync(q, p, size_t(cap) * sizeof(int32_t), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
 			if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
 		}
-		if (p) (void) hipFree(p);
+		if (p) dev_free(p);
 		p = q;
 		cap = nc;
 		return hipSuccess;
 	}
 	void release() {
-		if (p) (void) hipFree(p);
+		if (p) dev_free(p);
 		p = nullptr;
 		cap = 0;
 	}
@@ -386,7 +387,7 @@ struct Arena {
 		if (bytes > left) {
 			const size_t b = std::max<size_t>(bytes, size_t(64) << 20);
 			void* q = nullptr;
-			const hipError_t e = hipMalloc(&q, b);
+			const hipError_t e = dev_malloc(&q, b);
 			if (e != hipSuccess) return e;
 			blocks.push_back(q);
 			cur = static_cast<char*>(q);
@@ -398,7 +399,7 @@ struct Arena {
 		return hipSuccess;
 	}
 	void release() {
-		for (void* b : blocks) (void) hipFree(b);
+		for (void* b : blocks) dev_free(b);
 		blocks.clear();
 		cur = nullptr;
 		left = 0;
@@ -420,10 +421,10 @@ struct StepLog {
 static double wall() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
 void DeviceLists::release() {
-	if (nodes) (void) hipFree(nodes);
-	if (ptr_all) (void) hipFree(ptr_all);
+	if (nodes) dev_free(nodes);
+	if (ptr_all) dev_free(ptr_all);
 	for (auto& v : val)
-		if (v) (void) hipFree(v);
+		if (v) dev_free(v);
 	*this = DeviceLists();
 }
 
@@ -478,12 +479,12 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	if (!given) {
 		hvec<WalkNode> wn(size_t(std::max<int64_t>(nn, 1)));
 		walk_nodes(T, wn.data());
-		WCHK(hipMalloc(&dev_nodes, wn.size() * sizeof(WalkNode)));
+		WCHK(dev_malloc(&dev_nodes, wn.size() * sizeof(WalkNode)));
 		WCHK(hipMemcpyAsync(dev_nodes, wn.data(), wn.size() * sizeof(WalkNode), hipMemcpyHostToDevice, s));
 	}
-	WCHK(hipMalloc(&dev_overflow, sizeof(int32_t)));
+	WCHK(dev_malloc(&dev_overflow, sizeof(int32_t)));
 	WCHK(hipMemsetAsync(dev_overflow, 0, sizeof(int32_t), s));
-	WCHK(hipMalloc(&dev_tot, 2 * NCOL * sizeof(int64_t)));
+	WCHK(dev_malloc(&dev_tot, 2 * NCOL * sizeof(int64_t)));
 	P.nodes = static_cast<const WalkNode*>(dev_nodes);
 	P.theta_inv = 1.0 / T.P.theta;
 	P.h = T.P.soft_len;
@@ -505,7 +506,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		WCHK(lists_in.reserve(2, s));
 		WCHK(hipMemcpyAsync(lists_in.p, root, sizeof(root), hipMemcpyHostToDevice, s));
 		units_cap = 1024;
-		WCHK(hipMalloc(&dev_units, size_t(units_cap) * sizeof(WalkUnit)));
+		WCHK(dev_malloc(&dev_units, size_t(units_cap) * sizeof(WalkUnit)));
 		WCHK(hipMemcpyAsync(dev_units, &r0, sizeof(r0), hipMemcpyHostToDevice, s));
 		nu = 1;
 	}
@@ -522,17 +523,17 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		log.push_back(L);
 		if (int64_t(nu) * 8 > counts_cap) {
 			WCHK(hipStreamSynchronize(s));  // the last step's kernels read the old counts
-			if (dev_counts) (void) hipFree(dev_counts);
+			if (dev_counts) dev_free(dev_counts);
 			dev_counts = nullptr;
 			counts_cap = std::max<int64_t>(int64_t(nu) * 8, 2 * counts_cap);
-			WCHK(hipMalloc(&dev_counts, size_t(counts_cap) * sizeof(int32_t)));
+			WCHK(dev_malloc(&dev_counts, size_t(counts_cap) * sizeof(int32_t)));
 		}
 		if (2 * in_len > codes_cap) {
 			WCHK(hipStreamSynchronize(s));  // the last step's kernels read the old categories
-			if (dev_codes) (void) hipFree(dev_codes);
+			if (dev_codes) dev_free(dev_codes);
 			dev_codes = nullptr;
 			codes_cap = std::max<int64_t>(2 * in_len, 2 * codes_cap);
-			WCHK(hipMalloc(&dev_codes, size_t(codes_cap)));
+			WCHK(dev_malloc(&dev_codes, size_t(codes_cap)));
 		}
 		P.lists = lists_in.p;
 		WLAUNCH("walk step set-up");
@@ -548,10 +549,10 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			size_t need = 0;
 			WCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, L.sizes, L.offs, m, s));
 			if (need > scan_tmp_bytes) {
-				if (scan_tmp) (void) hipFree(scan_tmp);
+				if (scan_tmp) dev_free(scan_tmp);
 				scan_tmp = nullptr;
 				scan_tmp_bytes = std::max(need, 2 * scan_tmp_bytes);
-				WCHK(hipMalloc(&scan_tmp, scan_tmp_bytes));
+				WCHK(dev_malloc(&scan_tmp, scan_tmp_bytes));
 			}
 			size_t b = scan_tmp_bytes;
 			WCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, L.sizes, L.offs, m, s));
@@ -574,10 +575,10 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		}
 		WCHK(lists_out.reserve(std::max<int64_t>(t[COL_NEXT], 1), s, false));
 		if (t[COL_UNITS] > next_units_cap) {
-			if (dev_next_units) (void) hipFree(dev_next_units);
+			if (dev_next_units) dev_free(dev_next_units);
 			dev_next_units = nullptr;
 			next_units_cap = std::max<int64_t>(t[COL_UNITS], 2 * next_units_cap);
-			WCHK(hipMalloc(&dev_next_units, size_t(next_units_cap) * sizeof(WalkUnit)));
+			WCHK(dev_malloc(&dev_next_units, size_t(next_units_cap) * sizeof(WalkUnit)));
 		}
 		{
 			WalkPools W{};
@@ -613,8 +614,8 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		// node-order CSR per kind: per-node lengths over all steps, their scan, then every step's
 		// segments placed in step order
 		const int64_t np = nn + 1;
-		WCHK(hipMalloc(&dev_node_len, size_t(K_N * np) * sizeof(int64_t)));
-		WCHK(hipMalloc(&dev_fill, size_t(K_N * np) * sizeof(int64_t)));
+		WCHK(dev_malloc(&dev_node_len, size_t(K_N * np) * sizeof(int64_t)));
+		WCHK(dev_malloc(&dev_fill, size_t(K_N * np) * sizeof(int64_t)));
 		WCHK(hipMemsetAsync(dev_node_len, 0, size_t(K_N * np) * sizeof(int64_t), s));
 		WCHK(hipMemsetAsync(dev_fill, 0, size_t(K_N * np) * sizeof(int64_t), s));
 		for (const StepLog& L : log) {
@@ -622,30 +623,30 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			WLAUNCH("k_walk_node_len");
 		}
 		int64_t* dev_ptr = nullptr;
-		WCHK(hipMalloc(&dev_ptr, size_t(K_N * np) * sizeof(int64_t)));
+		WCHK(dev_malloc(&dev_ptr, size_t(K_N * np) * sizeof(int64_t)));
 		for (int k = 0; k < K_N; k++) {
 			size_t need = 0;
 			e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, dev_node_len + k * np, dev_ptr + k * np, np, s);
 			if (e == hipSuccess && need > scan_tmp_bytes) {
-				if (scan_tmp) (void) hipFree(scan_tmp);
+				if (scan_tmp) dev_free(scan_tmp);
 				scan_tmp = nullptr;
 				scan_tmp_bytes = need;
-				e = hipMalloc(&scan_tmp, scan_tmp_bytes);
+				e = dev_malloc(&scan_tmp, scan_tmp_bytes);
 			}
 			size_t b = scan_tmp_bytes;
 			if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, dev_node_len + k * np, dev_ptr + k * np, np, s);
 			if (e == hipSuccess) e = fin[k].reserve(std::max<int64_t>(tot[k], 1), s, false);
 			if (e != hipSuccess) {
-				(void) hipFree(dev_ptr);
+				dev_free(dev_ptr);
 				err = std::string("list assembly: ") + hipGetErrorString(e);
 				goto done;
 			}
 		}
 		int32_t* dst_h[K_N];
 		for (int k = 0; k < K_N; k++) dst_h[k] = fin[k].p;
-		e = hipMalloc(&dev_dst, sizeof(dst_h));
+		e = dev_malloc(&dev_dst, sizeof(dst_h));
 		if (e == hipSuccess) e = hipMemcpyAsync(dev_dst, dst_h, sizeof(dst_h), hipMemcpyHostToDevice, s);
-		if (e == hipSuccess) e = hipMalloc(&dev_base, K_N * sizeof(int64_t) * std::max<size_t>(log.size(), 1));
+		if (e == hipSuccess) e = dev_malloc(&dev_base, K_N * sizeof(int64_t) * std::max<size_t>(log.size(), 1));
 		WalkPools src{};
 		for (int k = 0; k < K_N; k++) src.out[k] = out[k].p;
 		for (size_t st = 0; st < log.size() && e == hipSuccess; st++) {
@@ -662,7 +663,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			// everything stays on the device: the caller's DeviceLists takes the arrays
 			if (e == hipSuccess) e = hipStreamSynchronize(s);
 			if (e != hipSuccess) {
-				(void) hipFree(dev_ptr);
+				dev_free(dev_ptr);
 				err = std::string("list assembly: ") + hipGetErrorString(e);
 				goto done;
 			}
@@ -694,7 +695,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 				e = hipMemcpyAsync(vals[k]->data(), fin[k].p, size_t(tot[k]) * sizeof(int32_t), hipMemcpyDeviceToHost, s);
 		}
 		if (e == hipSuccess) e = hipStreamSynchronize(s);
-		(void) hipFree(dev_ptr);
+		dev_free(dev_ptr);
 		if (e != hipSuccess) {
 			err = std::string("list assembly: ") + hipGetErrorString(e);
 			goto done;
@@ -709,18 +710,18 @@ done:
 #undef WCHK
 #undef WLAUNCH
 	(void) hipStreamSynchronize(s);
-	if (dev_nodes) (void) hipFree(dev_nodes);
-	if (dev_units) (void) hipFree(dev_units);
-	if (dev_next_units) (void) hipFree(dev_next_units);
-	if (dev_counts) (void) hipFree(dev_counts);
-	if (dev_codes) (void) hipFree(dev_codes);
-	if (dev_tot) (void) hipFree(dev_tot);
-	if (dev_node_len) (void) hipFree(dev_node_len);
-	if (dev_fill) (void) hipFree(dev_fill);
-	if (dev_base) (void) hipFree(dev_base);
-	if (dev_dst) (void) hipFree(dev_dst);
-	if (dev_overflow) (void) hipFree(dev_overflow);
-	if (scan_tmp) (void) hipFree(scan_tmp);
+	if (dev_nodes) dev_free(dev_nodes);
+	if (dev_units) dev_free(dev_units);
+	if (dev_next_units) dev_free(dev_next_units);
+	if (dev_counts) dev_free(dev_counts);
+	if (dev_codes) dev_free(dev_codes);
+	if (dev_tot) dev_free(dev_tot);
+	if (dev_node_len) dev_free(dev_node_len);
+	if (dev_fill) dev_free(dev_fill);
+	if (dev_base) dev_free(dev_base);
+	if (dev_dst) dev_free(dev_dst);
+	if (dev_overflow) dev_free(dev_overflow);
+	if (scan_tmp) dev_free(scan_tmp);
 	arena.release();
 	lists_in.release();
 	lists_out.release();

@@ -131,7 +131,7 @@ class Solver:
         _check(lib().tg_solver_solve(self._h), "tg_solver_solve")
 
     def tune(self, knob, value):
-        """Set a tuning knob: "pp_persistent" (0/1), "node_chunk" (1..512, from the next
+        """Set a tuning knob: "pp_persistent" (0/1), "node_chunk" (1..4096, from the next
         set_particles), "resident" (work groups of the persistent PP kernel)."""
         _check(lib().tg_solver_tune(self._h, knob.encode(), int(value)), "tg_solver_tune")
 
