@@ -379,7 +379,7 @@ def main():
     ap.add_argument("--theta", type=float, default=0.3,
                     help="opening angle of the timed solve (0.3: the largest that meets RMS 1e-3 on the 2M workload)")
     ap.add_argument("--leaf", type=int, default=0,
-                    help="particles per leaf; 0 autotunes leaf (16, 32, 64, 128), CC/CP chunk, PP kernel form and "
+                    help="particles per leaf; 0 autotunes leaf (8, 16, 32, 64, 128), CC/CP chunk, PP kernel form and "
                          "local pass")
     ap.add_argument("--wide-autotune", action="store_true", help="autotune over leaves of 4, 12, 24 and 48 as well")
     ap.add_argument("--max-rms", type=float, default=1e-3,
@@ -392,8 +392,9 @@ def main():
     ap.add_argument("--node-chunk", type=int, default=None, help="with a fixed --leaf: CC/CP list entries per partial slot")
     ap.add_argument("--local-sweep", type=int, default=None, choices=(0, 1),
                     help="with a fixed --leaf: the local-expansion pass level by level (1) or by per-leaf path sums (0)")
-    ap.add_argument("--overlap", type=int, default=None, choices=(0, 1),
-                    help="with a fixed --leaf: graph replays run the node kinds on a second stream (1) or on one stream (0)")
+    ap.add_argument("--overlap", type=int, default=None, choices=(0, 1, 2),
+                    help="with a fixed --leaf: graph replays run on one stream (0), the node kinds on a second stream (1), "
+                         "and also the Ewald node kinds on a third (2)")
     ap.add_argument("--host-walk", action="store_true",
                     help="build the tree, multipoles and interaction lists on the host instead of the device")
     ap.add_argument("--no-graph", action="store_true",

@@ -75,7 +75,8 @@ int tg_solver_solve(tg_solver* s);
  * "gpu_tree" 1 (the default for device solvers) builds the tree and multipoles on the device, 0 on
  * the host (the same tree bit for bit; from the next tg_solver_set_particles);
  * "overlap" 1 (the default) lets graph replays run the node kinds on a second stream beside the
- * leaf kinds (the same results bit for bit), 0 keeps one stream */
+ * leaf kinds, 2 also runs the Ewald node kinds on a third stream beside the direct ones (the same
+ * results bit for bit), 0 keeps one stream */
 int tg_solver_tune(tg_solver* s, const char* knob, int value);
 /* the same solve replayed from a HIP graph captured on its first call after set_particles (one
  * submission instead of 11 or more launches; records no per-kind timings; with "overlap" the node and

@@ -249,8 +249,9 @@ def test_graph_replay_matches_eager():
 
 
 def test_overlap_knob_is_bit_identical():
-    """Graph replays with the node kinds on a second stream ("overlap" 1, the default) and on one
-    stream give the eager solve's forces bit for bit, with both local-pass forms."""
+    """Graph replays with the node kinds on a second stream ("overlap" 1, the default), with the
+    Ewald node kinds on a third ("overlap" 2) and on one stream give the eager solve's forces bit
+    for bit, with both local-pass forms."""
     from tigergrav_amd import ics
     from tigergrav_amd.solver import Solver
 
@@ -264,7 +265,7 @@ def test_overlap_knob_is_bit_identical():
         s.solve()
         s.synchronize()
         eager = s.forces()
-        for ov in (1, 0, 1):
+        for ov in (1, 0, 2, 1, 2):
             s.tune("overlap", ov)
             for _ in range(2):
                 s.solve_graph()
@@ -356,7 +357,7 @@ def test_autotune_picks_the_fastest_tried_configuration():
     h, m = ics.soft_len(n), 1.0 / n
     print(f"autotune: N={n}", flush=True)
     best, table = autotune(x, True, h, m, 0.7, leaves=(32, 64), chunks=(64,), solves=3)
-    assert len(table) == 2 * 1 * 2 * 2 * 2  # leaves x chunks x PP forms x local passes x overlap
+    assert len(table) == 2 * 1 * 2 * 2 * 3  # leaves x chunks x PP forms x local passes x overlap
     assert all(r["ms"] > 0 for r in table) and best["ms"] == min(r["ms"] for r in table)
     s = Solver(ewald=True, soft_len=h, m=m, theta=0.7, parts_per_node=best["leaf"], device=0)
     apply(s, best)

@@ -13,7 +13,7 @@ from .solver import Solver
 
 
 def autotune(x, ewald, soft_len, m, theta, leaves=(8, 16, 32, 64, 128), chunks=(256, 4096), persistent=(0, 1), sweep=(0, 1),
-             overlap=(1, 0), solves=10, device=0, accept=None):
+             overlap=(1, 0, 2), solves=10, device=0, accept=None):
     """Returns (best, table): best = {"leaf", "node_chunk", "pp_persistent", "local_sweep", "overlap", "ms"};
     table lists every configuration tried with its ms per solve over `solves` graph replays
     (host clock, the device synchronised before and after).

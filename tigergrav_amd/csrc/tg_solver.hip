@@ -232,8 +232,10 @@ int Solver::init(int dev) {
 	SCHK(init_device_tables(dev));
 	SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 	SCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+	SCHK(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
 	SCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
 	SCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+	SCHK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
 	SCHK(d_queue.alloc(256));
 	int cus = 0;
 	SCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -279,6 +281,7 @@ Solver::~Solver() {
 		hipSetDevice(device);
 		hipStreamSynchronize(stream);
 		if (stream2) hipStreamSynchronize(stream2);
+		if (stream3) hipStreamSynchronize(stream3);
 		drop_graph();
 		for (auto& e : ev)
 			if (e) hipEventDestroy(e);
@@ -326,8 +329,13 @@ Solver::~Solver() {
 			hipStreamSynchronize(stream2);
 			hipStreamDestroy(stream2);
 		}
+		if (stream3) {
+			hipStreamSynchronize(stream3);
+			hipStreamDestroy(stream3);
+		}
 		if (ev_fork) hipEventDestroy(ev_fork);
 		if (ev_join) hipEventDestroy(ev_join);
+		if (ev_join3) hipEventDestroy(ev_join3);
 		hipStreamDestroy(stream);
 		dev_trim(device);  // the cached blocks this solver's buffers went back to
 	}
@@ -345,6 +353,7 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 		SCHK(hipSetDevice(device));
 		SCHK(hipStreamSynchronize(stream));
 		if (stream2) SCHK(hipStreamSynchronize(stream2));
+		if (stream3) SCHK(hipStreamSynchronize(stream3));
 	}
 	order_on_device = false;
 	fof_group.clear();
@@ -704,9 +713,11 @@ int Solver::upload() {
 
 // One solve.  The leaf kinds (PC, PP) accumulate into the zeroed outputs; the node kinds (CC, CP),
 // their sums per node and the level-by-level L2L touch none of them, so in an untimed solve (a
-// graph capture) they run on a second stream beside the leaf kinds.  L2P adds last, after both.
-// A timed solve runs everything on one stream in the same order of additions, so both give the
-// same bits.
+// graph capture) they run on a second stream beside the leaf kinds.  Each node kind writes partial
+// slots of its own, so with overlap 2 the Ewald node kinds run on a third stream beside the direct
+// ones (the table-bound CC_ewald leaves issue slots that the VALU-bound kernels fill); the node
+// sums wait for both.  L2P adds last, after everything.  A timed solve runs everything on one
+// stream in the same order of additions, so all forms give the same bits.
 int Solver::enqueue_solve(bool timed) {
 	SCHK(hipSetDevice(device));
 	hipStream_t s = stream;
@@ -721,11 +732,14 @@ int Solver::enqueue_solve(bool timed) {
 	hipEvent_t* evs = ev.data() + size_t(2) * TIMER_EVSPANS * size_t(solves_recorded % TIMER_RING);
 	if (timed) solves_recorded++;
 	auto mark = [&](int k) -> hipError_t { return timed ? hipEventRecord(evs[k], s) : hipSuccess; };
-	const bool fork = overlap && !timed;
-	hipStream_t sn = fork ? stream2 : s;  // the node side
+	const bool fork = overlap > 0 && !timed;
+	const bool fork3 = overlap > 1 && !timed;
+	hipStream_t sn = fork ? stream2 : s;     // the node side
+	hipStream_t se = fork3 ? stream3 : sn;   // its Ewald kinds
 	if (fork) {
 		SCHK(hipEventRecord(ev_fork, s));
 		SCHK(hipStreamWaitEvent(stream2, ev_fork, 0));
+		if (fork3) SCHK(hipStreamWaitEvent(stream3, ev_fork, 0));
 	}
 	NodeArgs na{};
 	na.sinkpos = d_nodepos.as<int4>();
@@ -741,11 +755,16 @@ int Solver::enqueue_solve(bool timed) {
 		na.owner = work[k].owner.as<int>();
 		na.nwaves = work[k].nitems;
 		SCHK(mark(2 * k));
+		const bool ew = k == CC_EWALD || k == CP_EWALD;
 		if (k == CC_DIRECT || k == CC_EWALD)
-			SCHK(launch_cc(na, k == CC_EWALD, sn));
+			SCHK(launch_cc(na, ew, ew ? se : sn));
 		else
-			SCHK(launch_cp(na, k == CP_EWALD, sn));
+			SCHK(launch_cp(na, ew, ew ? se : sn));
 		SCHK(mark(2 * k + 1));
+	}
+	if (fork3) {
+		SCHK(hipEventRecord(ev_join3, stream3));
+		SCHK(hipStreamWaitEvent(stream2, ev_join3, 0));
 	}
 	e = 2 * NKINDS;
 	SCHK(mark(e));
@@ -1005,7 +1024,8 @@ int tg_solver_tune(tg_solver* h, const char* knob, int value) {
 		if (value < 1) return TG_EINVAL;
 		S.resident_groups = value;
 	} else if (k == "overlap") {
-		S.overlap = value != 0;
+		if (value < 0 || value > 2) return TG_EINVAL;
+		S.overlap = value;
 	} else if (k == "local_sweep") {
 		S.local_sweep = value != 0;
 	} else if (k == "gpu_walk") {

@@ -44,9 +44,11 @@ struct Solver {
 	bool uniform_mass = true;
 	int node_chunk = 4096;  // sources per CC/CP partial slot (<= NODE_CHUNK_MAX); applies from the next set_particles
 	bool pp_persistent = false;  // PP direct as a persistent queue-driven kernel
-	bool overlap = true;         // graph replays run the node kinds on a second stream beside the leaf kinds
-	hipStream_t stream2 = nullptr;
-	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+	// graph replays: 0 one stream; 1 the node kinds on a second stream beside the leaf kinds; 2 as 1,
+	// with the Ewald node kinds on a third stream beside the direct ones
+	int overlap = 1;
+	hipStream_t stream2 = nullptr, stream3 = nullptr;
+	hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
 	bool local_sweep = false;    // local pass level by level (k_l2l_level) instead of per-leaf path sums (k_local)
 	bool gpu_walk = false;       // build the interaction lists on the device (device_walk); on for device solvers
 	bool gpu_tree = false;       // build the tree and multipoles on the device (device_build_tree); on for device solvers
