@@ -530,16 +530,41 @@ __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 	if (j < wv.y) {
 		slot = A.owner ? A.owner[j] : int(blockIdx.x);  // the plain drop-in: one slot per wave, one sink
 		const int4 X = A.sinkpos[A.slot_node ? A.slot_node[slot] : 0];
+		// software pipeline (direct kinds): the next entry's multipole and the list index after it
+		// are loaded while the current entry is evaluated.  The Ewald kinds load in order: their
+		// table row already holds most registers, and the 20 more cost a wave per SIMD (measured:
+		// no gain)
+		auto index_at = [&](int jj) { return jj < wv.y ? (A.list ? A.list[jj] : jj) : -1; };
+		auto load_multi = [&](int idx, float4* m) {
+			if (idx >= 0) {
+				const float4* src = reinterpret_cast<const float4*>(A.multis + idx);
+#pragma unroll
+				for (int q = 0; q < 5; q++) m[q] = src[q];
+			}
+		};
+		constexpr bool PF = !EWALD;
+		float4 nm[5];
+		int next_idx = index_at(j);
+		if (PF) {
+			load_multi(next_idx, nm);
+			next_idx = index_at(j + 64);
+		}
 		for (; j < wv.y; j += 64) {
-			const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[j] : j));
+			if (!PF) {
+				load_multi(next_idx, nm);
+				next_idx = index_at(j + 64);
+			}
 			float v[20];
 #pragma unroll
 			for (int q = 0; q < 5; q++) {
-				const float4 t = src[q];
-				v[4 * q] = t.x;
-				v[4 * q + 1] = t.y;
-				v[4 * q + 2] = t.z;
-				v[4 * q + 3] = t.w;
+				v[4 * q] = nm[q].x;
+				v[4 * q + 1] = nm[q].y;
+				v[4 * q + 2] = nm[q].z;
+				v[4 * q + 3] = nm[q].w;
+			}
+			if (PF) {
+				load_multi(next_idx, nm);
+				next_idx = index_at(j + 128);
 			}
 			const float x[3] = {sepu<EWALD || WRAP>(X.x, __float_as_int(v[17])) * POS_INVF,
 					sepu<EWALD || WRAP>(X.y, __float_as_int(v[18])) * POS_INVF,

@@ -81,6 +81,16 @@ __host__ __device__ constexpr float ef3(int s) { return mult3(T3[s]) * (1.0f / 6
 constexpr double POS_INV = 1.0 / 4294967296.0;
 constexpr float POS_INVF = 1.0f / 4294967296.0f;
 
+// calls f(std::integral_constant<int, i>) for i = 0..N-1, so slot indices are compile-time
+template <int... I, class F>
+__device__ __forceinline__ void static_for_seq(std::integer_sequence<int, I...>, F&& f) {
+	(f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+	static_for_seq(std::make_integer_sequence<int, N>{}, f);
+}
+
 // ---- derivative tensor from radial factors ----------------------------------------------------
 // For a radial function f(r), with d_k = (r^-1 d/dr)^k f, the Cartesian derivatives are
 //   D_a    = x_a d1
@@ -88,6 +98,23 @@ constexpr float POS_INVF = 1.0f / 4294967296.0f;
 //   D_abc  = x_a x_b x_c d3 + (delta_ab x_c + delta_ac x_b + delta_bc x_a) d2
 //   D_abcd = x_a x_b x_c x_d d4 + (6 delta x x terms) d3 + (3 delta delta terms) d2
 // (the reference builds the same tensor by in-place slot updates, tigergrav/green.hpp:62-117).
+// Each index pattern takes its delta terms factored out, with per-axis factors shared between
+// patterns (a, b, c distinct):
+//   D_aaa  = x_a (x_a^2 d3 + 3 d2)              D_aab = x_b w_a,   w_a = x_a^2 d3 + d2
+//   D_abc  = x_a x_b x_c d3
+//   D_aaaa = x_a^2 (x_a^2 d4 + 6 d3) + 3 d2     D_aaab = x_a x_b u_a,   u_a = x_a^2 d4 + 3 d3
+//   D_aabb = x_a^2 v_b + w_b                    D_aabc = x_b x_c v_a,   v_a = x_a^2 d4 + d3
+// about 60 operations for the 35 slots where the expanded sums take about 140.
+__host__ __device__ constexpr int count_of(const Tup& t, int n, int v) {
+	return (n > 0 && t.i[0] == v) + (n > 1 && t.i[1] == v) + (n > 2 && t.i[2] == v) + (n > 3 && t.i[3] == v);
+}
+// the axis that occurs exactly k times in the first n indices of t (-1 if none)
+__host__ __device__ constexpr int axis_with(const Tup& t, int n, int k) {
+	return count_of(t, n, 0) == k ? 0 : count_of(t, n, 1) == k ? 1 : count_of(t, n, 2) == k ? 2 : -1;
+}
+// the axis that does not occur (-1 if all do)
+__host__ __device__ constexpr int axis_missing(const Tup& t, int n) { return axis_with(t, n, 0); }
+
 __device__ __forceinline__ void radial_tensor(float* D, float d0, float d1, float d2, float d3, float d4, const float* x) {
 	D[0] = d0;
 #pragma unroll
@@ -99,40 +126,52 @@ __device__ __forceinline__ void radial_tensor(float* D, float d0, float d1, floa
 		x2[s] = x[a] * x[b];
 		D[4 + s] = (a == b) ? fmaf(x2[s], d2, d1) : x2[s] * d2;
 	}
-	float x3[10];
-#pragma unroll
-	for (int s = 0; s < 10; s++) {
-		const int a = T3[s].i[0], b = T3[s].i[1], c = T3[s].i[2];
-		x3[s] = x2[s2(a, b)] * x[c];
-		float t = x3[s] * d3;
-		// delta terms: count equal pairs
-		if (a == b && b == c)
-			t = fmaf(3.0f * x[a], d2, t);
-		else if (a == b)
-			t = fmaf(x[c], d2, t);
-		else if (b == c)
-			t = fmaf(x[a], d2, t);
-		D[10 + s] = t;
-	}
-	const float x2d3[6] = {x2[0] * d3, x2[1] * d3, x2[2] * d3, x2[3] * d3, x2[4] * d3, x2[5] * d3};
-#pragma unroll
-	for (int s = 0; s < 15; s++) {
-		const int a = T4[s].i[0], b = T4[s].i[1], c = T4[s].i[2], d = T4[s].i[3];
-		float t = x3[s3(a, b, c)] * x[d] * d4;
-		// sum over the 6 index pairs (p,q) with equal values of x_(other two) d3
-		float dd = 0.0f;
-		int npairs = 0;
-		if (a == b) { dd += x2d3[s2(c, d)]; npairs++; }
-		if (a == c) { dd += x2d3[s2(b, d)]; npairs++; }
-		if (a == d) { dd += x2d3[s2(b, c)]; npairs++; }
-		if (b == c) { dd += x2d3[s2(a, d)]; npairs++; }
-		if (b == d) { dd += x2d3[s2(a, c)]; npairs++; }
-		if (c == d) { dd += x2d3[s2(a, b)]; npairs++; }
-		if (npairs) t += dd;
-		const int dd2 = (a == b && c == d) + (a == c && b == d) + (a == d && b == c);
-		if (dd2) t = fmaf(float(dd2), d2, t);
-		D[20 + s] = t;
-	}
+	const float d2x3 = 3.0f * d2, d3x3 = 3.0f * d3, d3x6 = 6.0f * d3;
+	float w[3], u[3], v[3];
+	static_for<3>([&](auto a_) {
+		constexpr int a = decltype(a_)::value;
+		constexpr int aa = s2(a, a);
+		const float xaa = x2[aa];
+		w[a] = fmaf(xaa, d3, d2);
+		u[a] = fmaf(xaa, d4, d3x3);
+		v[a] = fmaf(xaa, d4, d3);
+	});
+	static_for<10>([&](auto s_) {
+		constexpr int s = decltype(s_)::value;
+		constexpr Tup t = T3[s];
+		constexpr int a3 = axis_with(t, 3, 3), a2 = axis_with(t, 3, 2), a1 = axis_with(t, 3, 1);
+		if constexpr (a3 >= 0) {
+			constexpr int aa = s2(a3, a3);
+			D[10 + s] = x[a3] * fmaf(x2[aa], d3, d2x3);
+		} else if constexpr (a2 >= 0) {
+			D[10 + s] = x[a1] * w[a2];
+		} else {
+			D[10 + s] = x2[1] * (x[2] * d3);  // x2[1] = x_0 x_1
+		}
+	});
+	static_for<15>([&](auto s_) {
+		constexpr int s = decltype(s_)::value;
+		constexpr Tup t = T4[s];
+		constexpr int a4 = axis_with(t, 4, 4), a3 = axis_with(t, 4, 3), a2 = axis_with(t, 4, 2), a1 = axis_with(t, 4, 1);
+		if constexpr (a4 >= 0) {
+			constexpr int aa = s2(a4, a4);
+			const float xaa = x2[aa];
+			D[20 + s] = fmaf(xaa, fmaf(xaa, d4, d3x6), d2x3);
+		} else if constexpr (a3 >= 0) {
+			constexpr int ab = s2(a3, a1);
+			D[20 + s] = x2[ab] * u[a3];
+		} else if constexpr (a2 >= 0 && a1 < 0) {
+			// two axes twice each: a2 is the first of them, b the one neither missing nor a2
+			constexpr int b = 3 - a2 - axis_missing(t, 4);
+			constexpr int aa = s2(a2, a2);
+			D[20 + s] = fmaf(x2[aa], v[b], w[b]);
+		} else {
+			// one axis twice (a2), the other two once
+			constexpr int b = a2 == 0 ? 1 : 0, c = a2 == 2 ? 1 : 2;
+			constexpr int bc = s2(b, c);
+			D[20 + s] = x2[bc] * v[a2];
+		}
+	});
 }
 
 // derivatives of -1/r (tigergrav/green.hpp:167-186): zero below r = 1e-9 like the reference
@@ -326,15 +365,23 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 	ewt_transpose_if<NOUT, 0, 1>(D, s3);
 	ewt_transpose_if<NOUT, 1, 2>(D, s2);
 	ewt_transpose_if<NOUT, 0, 1>(D, s1);
-	const bool z = !(r2 > 1e-12f);
+	// the sign of a component is the product of the signs of the coordinates it holds an odd
+	// number of times: one multiply per component from the products of the three signs
+	const float sgp[8] = {1.0f, sg[0], sg[1], sg[0] * sg[1], sg[2], sg[0] * sg[2], sg[1] * sg[2], sg[0] * sg[1] * sg[2]};
 	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
 		constexpr int k = decltype(k_)::value;
-		float v = D[k];
-		if constexpr (ewt::odd_in(k, 0)) v *= sg[0];
-		if constexpr (ewt::odd_in(k, 1)) v *= sg[1];
-		if constexpr (ewt::odd_in(k, 2)) v *= sg[2];
-		D[k] = z ? (k == 0 ? 2.8372975f : 0.0f) : v;
+		constexpr int o = (ewt::odd_in(k, 0) ? 1 : 0) | (ewt::odd_in(k, 1) ? 2 : 0) | (ewt::odd_in(k, 2) ? 4 : 0);
+		if constexpr (o != 0) D[k] *= sgp[o];
 	});
+	// a coincident pair (rare; the interpolation above stays finite there) takes the reference's
+	// constant, behind a branch that the whole wave skips when none of its lanes has one
+	const bool z = !(r2 > 1e-12f);
+	if (__builtin_expect(__ballot(z) != 0ull, 0)) {
+		ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
+			constexpr int k = decltype(k_)::value;
+			D[k] = z ? (k == 0 ? 2.8372975f : 0.0f) : D[k];
+		});
+	}
 }
 
 __device__ __forceinline__ void green_ewald(float* D, const float* x) { green_ewald_tab<35>(D, x, c_ewt); }
