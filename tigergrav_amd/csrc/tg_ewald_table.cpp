@@ -154,7 +154,8 @@ void ewald_green64(const double* x, double* D) {
 }
 
 void ewald_table_build(std::vector<float>& T) {
-	T.assign(size_t(TG_EWT_P) * TG_EWT_S, 0.0f);
+	T.assign(size_t(TG_EWT_ROWS) * TG_EWT_S, 0.0f);
+	T[size_t(TG_EWT_P) * TG_EWT_S] = 2.8372975f;  // the coincident-pair row (tigergrav/green.hpp:272)
 	radial_terms();  // the term tables, before the threads read them
 	fourier_terms();
 	// the points with i >= j >= l; the kernels map every separation onto them by symmetry

@@ -23,6 +23,7 @@
 #define TG_EWT_ORDER 7  // highest derivative order tabulated
 #define TG_EWT_S 120    // slots of orders 0..7
 #define TG_EWT_P 6545   // grid points with i >= j >= l: TG_EWT_N (TG_EWT_N + 1) (TG_EWT_N + 2) / 6
+#define TG_EWT_ROWS (TG_EWT_P + 1)  // and one row for coincident points: the reference's constants
 
 namespace tg {
 namespace ewt {
@@ -87,7 +88,9 @@ static_assert(slot_of(2, 0, 0) == 4 && slot_of(0, 1, 1) == 8 && slot_of(1, 1, 1)
 // FP64 value of the correction and its TG_EWT_S derivatives (orders 0..TG_EWT_ORDER) at x (host);
 // at x = 0 the analytic limit (the kernels apply the reference's r -> 0 convention themselves)
 void ewald_green64(const double* x, double* D);
-// the table: TG_EWT_P rows (the sorted grid points, row wedge_index(i, j, l)), TG_EWT_S floats each
+// the table: TG_EWT_ROWS rows of TG_EWT_S floats: the sorted grid points (row
+// wedge_index(i, j, l)), then row TG_EWT_P = (2.8372975, 0, ..., 0), the reference's values at
+// r -> 0, which a kernel reads with a zero Taylor offset for a coincident pair
 void ewald_table_build(std::vector<float>& T);
 
 }  // namespace tg

@@ -531,9 +531,10 @@ __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 		slot = A.owner ? A.owner[j] : int(blockIdx.x);  // the plain drop-in: one slot per wave, one sink
 		const int4 X = A.sinkpos[A.slot_node ? A.slot_node[slot] : 0];
 		// software pipeline (direct kinds): the next entry's multipole and the list index after it
-		// are loaded while the current entry is evaluated.  The Ewald kinds load in order: their
-		// table row already holds most registers, and the 20 more cost a wave per SIMD (measured:
-		// no gain)
+		// are loaded while the current entry is evaluated, into the other of two buffers (the loop
+		// is unrolled twice, so no copy between them).  The Ewald kinds load in order: their table
+		// row already holds most registers, and the 20 more cost a wave per SIMD (measured: no
+		// gain)
 		auto index_at = [&](int jj) { return jj < wv.y ? (A.list ? A.list[jj] : jj) : -1; };
 		auto load_multi = [&](int idx, float4* m) {
 			if (idx >= 0) {
@@ -542,29 +543,14 @@ __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 				for (int q = 0; q < 5; q++) m[q] = src[q];
 			}
 		};
-		constexpr bool PF = !EWALD;
-		float4 nm[5];
-		int next_idx = index_at(j);
-		if (PF) {
-			load_multi(next_idx, nm);
-			next_idx = index_at(j + 64);
-		}
-		for (; j < wv.y; j += 64) {
-			if (!PF) {
-				load_multi(next_idx, nm);
-				next_idx = index_at(j + 64);
-			}
+		auto entry = [&](const float4* m) {
 			float v[20];
 #pragma unroll
 			for (int q = 0; q < 5; q++) {
-				v[4 * q] = nm[q].x;
-				v[4 * q + 1] = nm[q].y;
-				v[4 * q + 2] = nm[q].z;
-				v[4 * q + 3] = nm[q].w;
-			}
-			if (PF) {
-				load_multi(next_idx, nm);
-				next_idx = index_at(j + 128);
+				v[4 * q] = m[q].x;
+				v[4 * q + 1] = m[q].y;
+				v[4 * q + 2] = m[q].z;
+				v[4 * q + 3] = m[q].w;
 			}
 			const float x[3] = {sepu<EWALD || WRAP>(X.x, __float_as_int(v[17])) * POS_INVF,
 					sepu<EWALD || WRAP>(X.y, __float_as_int(v[18])) * POS_INVF,
@@ -574,7 +560,32 @@ __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 				green_ewald(D, x);
 			else
 				green_direct(D, x);
-			m2l(L, v, D);
+			m2l<!EWALD>(L, v, D);
+		};
+		float4 ma[5];
+		if (EWALD) {
+			for (; j < wv.y; j += 64) {
+				const float4* src = reinterpret_cast<const float4*>(A.multis + (A.list ? A.list[j] : j));
+#pragma unroll
+				for (int q = 0; q < 5; q++) ma[q] = src[q];
+				entry(ma);
+			}
+		} else {
+			float4 mb[5];
+			load_multi(index_at(j), ma);
+			int next_idx = index_at(j + 64);
+			for (;;) {
+				load_multi(next_idx, mb);
+				next_idx = index_at(j + 128);
+				entry(ma);
+				j += 64;
+				if (j >= wv.y) break;
+				load_multi(next_idx, ma);
+				next_idx = index_at(j + 128);
+				entry(mb);
+				j += 64;
+				if (j >= wv.y) break;
+			}
 		}
 	}
 	node_segment_store(L, slot, min(wv.y - wv.x, 64), A.slot_node, A.partial);

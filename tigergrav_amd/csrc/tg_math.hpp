@@ -189,29 +189,32 @@ __device__ __forceinline__ void green_direct(float* D, const float* x) {
 
 // ---- contractions (src/gravity.cpp:26-169) ----------------------------------------------------
 // M2L: L += contraction of the multipole M with D (reference gravity.cpp:26-90)
+// ACC: every term is added straight into L, in the reference's order of terms (as its loop adds
+// each term into its L); else the terms of each of L[0..9] are summed first and the sum added.
+template <bool ACC>
 __device__ __forceinline__ void m2l(float* L, const float* M, const float* D) {
-	float l0 = M[0] * D[0];
+	float l0 = ACC ? fmaf(M[0], D[0], L[0]) : M[0] * D[0];
 #pragma unroll
 	for (int s = 0; s < 6; s++) l0 = fmaf(M[1 + s] * ef2(s), D[4 + s], l0);
 #pragma unroll
 	for (int s = 0; s < 10; s++) l0 = fmaf(-M[7 + s] * ef3(s), D[10 + s], l0);
-	L[0] += l0;
+	L[0] = ACC ? l0 : L[0] + l0;
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
-		float la = M[0] * D[1 + a];
+		float la = ACC ? fmaf(M[0], D[1 + a], L[1 + a]) : M[0] * D[1 + a];
 #pragma unroll
 		for (int s = 0; s < 6; s++) la = fmaf(M[1 + s] * ef2(s), D[E3(a, T2[s].i[0], T2[s].i[1])], la);
 #pragma unroll
 		for (int s = 0; s < 10; s++) la = fmaf(-M[7 + s] * ef3(s), D[E4(a, T3[s].i[0], T3[s].i[1], T3[s].i[2])], la);
-		L[1 + a] += la;
+		L[1 + a] = ACC ? la : L[1 + a] + la;
 	}
 #pragma unroll
 	for (int t = 0; t < 6; t++) {
 		const int a = T2[t].i[0], b = T2[t].i[1];
-		float lab = M[0] * D[4 + t];
+		float lab = ACC ? fmaf(M[0], D[4 + t], L[4 + t]) : M[0] * D[4 + t];
 #pragma unroll
 		for (int s = 0; s < 6; s++) lab = fmaf(M[1 + s] * ef2(s), D[E4(a, b, T2[s].i[0], T2[s].i[1])], lab);
-		L[4 + t] += lab;
+		L[4 + t] = ACC ? lab : L[4 + t] + lab;
 	}
 #pragma unroll
 	for (int s = 10; s < 35; s++) L[s] = fmaf(M[0], D[s], L[s]);
@@ -292,7 +295,8 @@ __device__ __forceinline__ void ewt_transpose_if(float* D, bool cond) {
 // error below 5e-6 of each order's largest value (1.7e-6 RMS over orders 0..4, 1.1e-7 at order 1).
 // Order-5 components (the adjoint's) take a second-order expansion, the table ending at order 7:
 // error about 1e-4 of that order's largest value.  At |x| <= 1e-6 the reference's convention
-// D = (2.8372975, 0, ..., 0) applies.
+// D = (2.8372975, 0, ..., 0) applies (tigergrav/green.hpp:272): the table's last row, read with a
+// zero offset.
 template <int NOUT>
 __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const float* table) {
 	constexpr int NLOAD = NOUT == 56 ? TG_EWT_S : ewt_nload(NOUT);
@@ -317,14 +321,17 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 	cx(u[1], u[2], s2);
 	const bool s3 = u[1] > u[0];
 	cx(u[0], u[1], s3);
+	// a coincident pair reads the table's last row, (2.8372975, 0, ..., 0), with a zero offset
+	const bool z = !(r2 > 1e-12f);
 	float d[3];
 	int gi[3];
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
 		gi[a] = min(int(fmaf(u[a], float(TG_EWT_INV), 0.5f)), TG_EWT_N - 1);
-		d[a] = u[a] - float(gi[a]) * (1.0f / TG_EWT_INV);
+		d[a] = z ? 0.0f : u[a] - float(gi[a]) * (1.0f / TG_EWT_INV);
 	}
-	const float4* tp = reinterpret_cast<const float4*>(table + size_t(ewt::wedge_index(gi[0], gi[1], gi[2])) * TG_EWT_S);
+	const int row = z ? TG_EWT_P : ewt::wedge_index(gi[0], gi[1], gi[2]);
+	const float4* tp = reinterpret_cast<const float4*>(table + size_t(row) * TG_EWT_S);
 	float t[NLOAD];
 #pragma unroll
 	for (int q = 0; q < NLOAD / 4; q++) {
@@ -373,15 +380,6 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 		constexpr int o = (ewt::odd_in(k, 0) ? 1 : 0) | (ewt::odd_in(k, 1) ? 2 : 0) | (ewt::odd_in(k, 2) ? 4 : 0);
 		if constexpr (o != 0) D[k] *= sgp[o];
 	});
-	// a coincident pair (rare; the interpolation above stays finite there) takes the reference's
-	// constant, behind a branch that the whole wave skips when none of its lanes has one
-	const bool z = !(r2 > 1e-12f);
-	if (__builtin_expect(__ballot(z) != 0ull, 0)) {
-		ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
-			constexpr int k = decltype(k_)::value;
-			D[k] = z ? (k == 0 ? 2.8372975f : 0.0f) : D[k];
-		});
-	}
 }
 
 __device__ __forceinline__ void green_ewald(float* D, const float* x) { green_ewald_tab<35>(D, x, c_ewt); }
