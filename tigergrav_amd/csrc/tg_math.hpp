@@ -189,32 +189,31 @@ __device__ __forceinline__ void green_direct(float* D, const float* x) {
 
 // ---- contractions (src/gravity.cpp:26-169) ----------------------------------------------------
 // M2L: L += contraction of the multipole M with D (reference gravity.cpp:26-90)
-// ACC: every term is added straight into L, in the reference's order of terms (as its loop adds
-// each term into its L); else the terms of each of L[0..9] are summed first and the sum added.
-template <bool ACC>
+// Every term is added straight into L, in the reference's order of terms (as its loop adds each
+// term into its L).
 __device__ __forceinline__ void m2l(float* L, const float* M, const float* D) {
-	float l0 = ACC ? fmaf(M[0], D[0], L[0]) : M[0] * D[0];
+	float l0 = fmaf(M[0], D[0], L[0]);
 #pragma unroll
 	for (int s = 0; s < 6; s++) l0 = fmaf(M[1 + s] * ef2(s), D[4 + s], l0);
 #pragma unroll
 	for (int s = 0; s < 10; s++) l0 = fmaf(-M[7 + s] * ef3(s), D[10 + s], l0);
-	L[0] = ACC ? l0 : L[0] + l0;
+	L[0] = l0;
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
-		float la = ACC ? fmaf(M[0], D[1 + a], L[1 + a]) : M[0] * D[1 + a];
+		float la = fmaf(M[0], D[1 + a], L[1 + a]);
 #pragma unroll
 		for (int s = 0; s < 6; s++) la = fmaf(M[1 + s] * ef2(s), D[E3(a, T2[s].i[0], T2[s].i[1])], la);
 #pragma unroll
 		for (int s = 0; s < 10; s++) la = fmaf(-M[7 + s] * ef3(s), D[E4(a, T3[s].i[0], T3[s].i[1], T3[s].i[2])], la);
-		L[1 + a] = ACC ? la : L[1 + a] + la;
+		L[1 + a] = la;
 	}
 #pragma unroll
 	for (int t = 0; t < 6; t++) {
 		const int a = T2[t].i[0], b = T2[t].i[1];
-		float lab = ACC ? fmaf(M[0], D[4 + t], L[4 + t]) : M[0] * D[4 + t];
+		float lab = fmaf(M[0], D[4 + t], L[4 + t]);
 #pragma unroll
 		for (int s = 0; s < 6; s++) lab = fmaf(M[1 + s] * ef2(s), D[E4(a, b, T2[s].i[0], T2[s].i[1])], lab);
-		L[4 + t] = ACC ? lab : L[4 + t] + lab;
+		L[4 + t] = lab;
 	}
 #pragma unroll
 	for (int s = 10; s < 35; s++) L[s] = fmaf(M[0], D[s], L[s]);
