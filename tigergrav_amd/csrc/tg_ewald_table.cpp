@@ -154,8 +154,8 @@ void ewald_green64(const double* x, double* D) {
 }
 
 void ewald_table_build(std::vector<float>& T) {
-	T.assign(size_t(TG_EWT_ROWS) * TG_EWT_S, 0.0f);
-	T[size_t(TG_EWT_P) * TG_EWT_S] = 2.8372975f;  // the coincident-pair row (tigergrav/green.hpp:272)
+	T.assign(size_t(TG_EWT_ROWS) * TG_EWT_STRIDE, 0.0f);
+	T[size_t(TG_EWT_P) * TG_EWT_STRIDE] = 2.8372975f;  // the coincident-pair row (tigergrav/green.hpp:272)
 	radial_terms();  // the term tables, before the threads read them
 	fourier_terms();
 	// the points with i >= j >= l; the kernels map every separation onto them by symmetry
@@ -166,7 +166,7 @@ void ewald_table_build(std::vector<float>& T) {
 				const double x[3] = {double(i) / TG_EWT_INV, double(j) / TG_EWT_INV, double(l) / TG_EWT_INV};
 				double D[TG_EWT_S];
 				ewald_green64(x, D);
-				float* row = T.data() + size_t(ewt::wedge_index(i, j, l)) * TG_EWT_S;
+				float* row = T.data() + size_t(ewt::wedge_index(i, j, l)) * TG_EWT_STRIDE;
 				for (int k = 0; k < TG_EWT_S; k++) row[k] = float(D[k]);
 			}
 }

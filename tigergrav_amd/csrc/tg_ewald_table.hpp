@@ -6,7 +6,7 @@
 // tigergrav/green.hpp:3-59 truncates them) is smooth over the wrapped cell |x_i| <= 1/2, even in
 // each coordinate and symmetric under permutations of the coordinates.  Its derivatives to
 // seventh order are tabulated in FP64, rounded to float, on the points of the grid
-// {0, 1/64, ..., 1/2}^3 with i >= j >= l (6545 points, 3.1 MB: it stays in one XCD's L2).  A
+// {0, 1/64, ..., 1/2}^3 with i >= j >= l (6545 points, 3.2 MB: it stays in one XCD's L2).  A
 // kernel sorts |x| into that wedge, takes the derivatives it needs from a third-order Taylor
 // expansion about the nearest point, permutes the components back and flips the sign of every
 // component with an odd count of a negative coordinate's index.
@@ -24,6 +24,7 @@
 #define TG_EWT_S 120    // slots of orders 0..7
 #define TG_EWT_P 6545   // grid points with i >= j >= l: TG_EWT_N (TG_EWT_N + 1) (TG_EWT_N + 2) / 6
 #define TG_EWT_ROWS (TG_EWT_P + 1)  // and one row for coincident points: the reference's constants
+#define TG_EWT_STRIDE 124  // floats per row: 496 bytes (measured: rows of 480 or 512 bytes gather slower)
 
 namespace tg {
 namespace ewt {
@@ -64,6 +65,7 @@ constexpr int swap_ab(int k, int a, int b) {
 // table row of the sorted grid point i >= j >= l
 constexpr int wedge_index(int i, int j, int l) { return i * (i + 1) * (i + 2) / 6 + j * (j + 1) / 2 + l; }
 static_assert(wedge_index(TG_EWT_N - 1, TG_EWT_N - 1, TG_EWT_N - 1) == TG_EWT_P - 1, "wedge size");
+static_assert(TG_EWT_STRIDE >= TG_EWT_S && TG_EWT_STRIDE % 4 == 0, "row stride");
 // slot k with one more index a, with indices a and b
 constexpr int up1(int k, int a) { return slot_of(ST.c[k].n0 + (a == 0), ST.c[k].n1 + (a == 1), ST.c[k].n2 + (a == 2)); }
 constexpr int up2(int k, int a, int b) { return up1(up1(k, a), b); }
@@ -88,7 +90,7 @@ static_assert(slot_of(2, 0, 0) == 4 && slot_of(0, 1, 1) == 8 && slot_of(1, 1, 1)
 // FP64 value of the correction and its TG_EWT_S derivatives (orders 0..TG_EWT_ORDER) at x (host);
 // at x = 0 the analytic limit (the kernels apply the reference's r -> 0 convention themselves)
 void ewald_green64(const double* x, double* D);
-// the table: TG_EWT_ROWS rows of TG_EWT_S floats: the sorted grid points (row
+// the table: TG_EWT_ROWS rows of TG_EWT_STRIDE floats (TG_EWT_S slots, then zeros): the sorted grid points (row
 // wedge_index(i, j, l)), then row TG_EWT_P = (2.8372975, 0, ..., 0), the reference's values at
 // r -> 0, which a kernel reads with a zero Taylor offset for a coincident pair
 void ewald_table_build(std::vector<float>& T);

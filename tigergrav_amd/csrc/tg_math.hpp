@@ -330,7 +330,7 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 		d[a] = z ? 0.0f : u[a] - float(gi[a]) * (1.0f / TG_EWT_INV);
 	}
 	const int row = z ? TG_EWT_P : ewt::wedge_index(gi[0], gi[1], gi[2]);
-	const float4* tp = reinterpret_cast<const float4*>(table + size_t(row) * TG_EWT_S);
+	const float4* tp = reinterpret_cast<const float4*>(table + size_t(row) * TG_EWT_STRIDE);
 	float t[NLOAD];
 #pragma unroll
 	for (int q = 0; q < NLOAD / 4; q++) {
