@@ -398,7 +398,7 @@ int Solver::set_particles(const int32_t* x, const float* mass, int64_t count, bo
 			const hipError_t e = walk_nodes_device(dtree, &wn, stream, err);
 			if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
 		}
-		const hipError_t e = device_walk(tree, device, stream, err, t_detail, &dlists, wn, n_nodes);
+		const hipError_t e = device_walk(tree, device, stream, err, t_detail, &dlists, wn, n_nodes, &walk_hint);
 		if (e != hipSuccess) return e == hipErrorOutOfMemory ? TG_ENOMEM : TG_EHIP;
 		lists_on_device = true;
 	} else {

@@ -51,6 +51,7 @@ struct Solver {
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
 	bool local_sweep = false;    // local pass level by level (k_l2l_level) instead of per-leaf path sums (k_local)
 	bool gpu_walk = false;       // build the interaction lists on the device (device_walk); on for device solvers
+	WalkHint walk_hint;          // the buffer sizes the last device walk reached: the next one starts there
 	bool gpu_tree = false;       // build the tree and multipoles on the device (device_build_tree); on for device solvers
 	int resident_groups = 1024;  // work groups of the persistent form (set from the CU count)
 	Tree tree;

@@ -327,26 +327,48 @@ __global__ __launch_bounds__(64) void k_walk_node_len(const int32_t* __restrict_
 	}
 }
 
-// a step's segments into the node-order lists: dst = ptr[node] + fill[node].  One wave per unit,
-// its lanes copying consecutive entries.
+// a step's segments into the node-order lists: dst = ptr[node] + fill[node].  One wave per unit:
+// lanes 0..7 read the eight kinds' lengths and places at once, then all lanes copy the unit's
+// segments as one run of entries (64 at a time across the kinds), so a unit costs one round of
+// metadata loads and one of copies instead of one of each per kind.  Only the one unit of (node,
+// kind) with entries moves its fill counter (k_walk_node_len).
 __global__ __launch_bounds__(64) void k_walk_place(const int32_t* __restrict__ nodes_of, int nu, const int64_t* __restrict__ sizes,
 		const int64_t* __restrict__ offs, const int64_t* __restrict__ base, int64_t nn, const int64_t* __restrict__ ptr,
 		int64_t* __restrict__ fill, WalkPools src, int32_t* const* __restrict__ dst) {
+	__shared__ int64_t s_pre[K_N + 1];
+	__shared__ const int32_t* s_from[K_N];
+	__shared__ int32_t* s_to[K_N];
 	const int i = blockIdx.x;
 	const int lane = threadIdx.x;
 	const int64_t nd = nodes_of[i];
-	for (int k = 0; k < K_N; k++) {
-		const int64_t len = sizes[int64_t(k) * nu + i];
-		if (len == 0) continue;
-		const int64_t from = base[k] + offs[int64_t(k) * nu + i];
-		int64_t* f = fill + int64_t(k) * (nn + 1) + nd;
-		const int64_t to = ptr[int64_t(k) * (nn + 1) + nd] + *f;
-		const int32_t* a = src.out[k] + from;
-		int32_t* b = dst[k] + to;
-		for (int64_t l = lane; l < len; l += 64) b[l] = a[l];
-		__syncthreads();  // every lane has read the fill before lane 0 moves it
-		if (lane == 0) *f += len;
+	int64_t len = 0, f = 0;
+	if (lane < K_N) {
+		len = sizes[int64_t(lane) * nu + i];
+		if (len > 0) {
+			f = fill[int64_t(lane) * (nn + 1) + nd];
+			s_from[lane] = src.out[lane] + base[lane] + offs[int64_t(lane) * nu + i];
+			s_to[lane] = dst[lane] + ptr[int64_t(lane) * (nn + 1) + nd] + f;
+		}
 	}
+	// the start of each kind's part of the run: an exclusive scan of the eight lengths
+	int64_t pre = 0;
+#pragma unroll
+	for (int k = 0; k < K_N - 1; k++) {
+		const int64_t lk = __shfl(len, k, 64);
+		if (lane > k) pre += lk;
+	}
+	if (lane < K_N) s_pre[lane] = pre;
+	if (lane == K_N - 1) s_pre[K_N] = pre + len;
+	__syncthreads();
+	const int64_t total = s_pre[K_N];
+	for (int64_t t = lane; t < total; t += 64) {
+		int k = 0;
+#pragma unroll
+		for (int q = 1; q < K_N; q++) k += (t >= s_pre[q]);  // the last kind that starts at or before t
+		const int64_t o = t - s_pre[k];
+		s_to[k][o] = s_from[k][o];
+	}
+	if (lane < K_N && len > 0) fill[int64_t(lane) * (nn + 1) + nd] = f + len;
 }
 
 // a device buffer of int32 that grows, keeping its contents
@@ -429,7 +451,8 @@ void DeviceLists::release() {
 }
 
 hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t_out, DeviceLists* keep, WalkNode* given,
-		int64_t given_nn) {
+		int64_t given_nn, WalkHint* hint) {
+	static_assert(sizeof(WalkHint::out) / sizeof(int64_t) == K_N, "one hint per output kind");
 	double tw[5] = {wall(), 0, 0, 0, 0};
 	const int64_t nn = given ? given_nn : int64_t(T.nodes.size());
 	hipError_t e = hipSuccess;
@@ -451,6 +474,8 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	Arena arena;
 	Pool lists_in, lists_out, out[K_N], fin[K_N];
 	std::vector<StepLog> log;
+	std::vector<int64_t> base_h;  // every step's pool starts, for the placement
+	int64_t max_lists = 2;        // the longest checklist set of a step, for the hint
 	int64_t tot[K_N] = {};
 	int nu = 0;
 	WalkParams P{};
@@ -485,6 +510,22 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 	WCHK(dev_malloc(&dev_overflow, sizeof(int32_t)));
 	WCHK(hipMemsetAsync(dev_overflow, 0, sizeof(int32_t), s));
 	WCHK(dev_malloc(&dev_tot, 2 * NCOL * sizeof(int64_t)));
+	if (hint) {
+		for (int k = 0; k < K_N; k++)
+			if (hint->out[k] > 0) WCHK(out[k].reserve(hint->out[k], s, false));
+		if (hint->lists > 0) {
+			WCHK(lists_in.reserve(hint->lists, s, false));
+			WCHK(lists_out.reserve(hint->lists, s, false));
+		}
+		if (hint->counts > 0) {
+			counts_cap = hint->counts;
+			WCHK(dev_malloc(&dev_counts, size_t(counts_cap) * sizeof(int32_t)));
+		}
+		if (hint->codes > 0) {
+			codes_cap = hint->codes;
+			WCHK(dev_malloc(&dev_codes, size_t(codes_cap)));
+		}
+	}
 	P.nodes = static_cast<const WalkNode*>(dev_nodes);
 	P.theta_inv = 1.0 / T.P.theta;
 	P.h = T.P.soft_len;
@@ -574,6 +615,7 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 			tot[k] += t[k];
 		}
 		WCHK(lists_out.reserve(std::max<int64_t>(t[COL_NEXT], 1), s, false));
+		max_lists = std::max(max_lists, t[COL_NEXT]);
 		if (t[COL_UNITS] > next_units_cap) {
 			if (dev_next_units) dev_free(dev_next_units);
 			dev_next_units = nullptr;
@@ -610,6 +652,12 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		goto done;
 	}
 	tw[2] = wall();
+	if (hint) {
+		for (int k = 0; k < K_N; k++) hint->out[k] = tot[k];
+		hint->lists = max_lists;
+		hint->counts = counts_cap;
+		hint->codes = codes_cap;
+	}
 	{
 		// node-order CSR per kind: per-node lengths over all steps, their scan, then every step's
 		// segments placed in step order
@@ -647,12 +695,14 @@ hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, dou
 		e = dev_malloc(&dev_dst, sizeof(dst_h));
 		if (e == hipSuccess) e = hipMemcpyAsync(dev_dst, dst_h, sizeof(dst_h), hipMemcpyHostToDevice, s);
 		if (e == hipSuccess) e = dev_malloc(&dev_base, K_N * sizeof(int64_t) * std::max<size_t>(log.size(), 1));
+		// every step's pool starts in one copy (the host array outlives the stream's wait below)
+		base_h.resize(K_N * std::max<size_t>(log.size(), 1));
+		for (size_t st = 0; st < log.size(); st++) std::memcpy(base_h.data() + K_N * st, log[st].base, K_N * sizeof(int64_t));
+		if (e == hipSuccess) e = hipMemcpyAsync(dev_base, base_h.data(), base_h.size() * sizeof(int64_t), hipMemcpyHostToDevice, s);
 		WalkPools src{};
 		for (int k = 0; k < K_N; k++) src.out[k] = out[k].p;
 		for (size_t st = 0; st < log.size() && e == hipSuccess; st++) {
 			const StepLog& L = log[st];
-			e = hipMemcpyAsync(dev_base + K_N * st, L.base, K_N * sizeof(int64_t), hipMemcpyHostToDevice, s);
-			if (e != hipSuccess) break;
 			hipLaunchKernelGGL(k_walk_place, dim3(unsigned(L.nu)), dim3(64), 0, s, L.nodes_of, L.nu, L.sizes, L.offs,
 					dev_base + K_N * st, nn, dev_ptr, dev_fill, src, dev_dst);
 			e = hipGetLastError();

@@ -39,8 +39,17 @@ struct DeviceLists {
 // device in *keep (released first), T.lists is left empty and T.census zero.
 // With nodes (a device table of nn nodes, which device_walk then owns and frees or hands to keep),
 // T supplies only its parameters: the tree itself may be on the device only.
+// With hint, the walk's working buffers start at the sizes *hint holds (those the caller's last
+// walk reached, which this walk writes back), so a rebuild of similar particles grows none of them:
+// a grown output pool copies what it holds, and a grown per-step buffer waits for the stream.
+struct WalkHint {
+	int64_t out[8] = {};    // entries of each output kind
+	int64_t lists = 0;      // the longest checklist set of a step
+	int64_t counts = 0;     // per-step count buffer
+	int64_t codes = 0;      // per-step category bytes
+};
 hipError_t device_walk(Tree& T, int device, hipStream_t s, std::string& err, double* t = nullptr, DeviceLists* keep = nullptr,
-		WalkNode* nodes = nullptr, int64_t nn = 0);
+		WalkNode* nodes = nullptr, int64_t nn = 0, WalkHint* hint = nullptr);
 
 struct DeviceTree;
 // the walk's node table of a device-built tree, made on the device (*out: nn nodes, device memory
