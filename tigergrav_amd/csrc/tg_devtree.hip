@@ -91,23 +91,41 @@ __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
 	if (i < n) a[i] = v;
 }
 
-// 1 for every node of the level [lb, lb + nl) that splits; flags[nl] = 0 (the scan's total)
-__global__ void k_split_flags(const DNode* __restrict__ nodes, int lb, int nl, int ppn, int max_level, int32_t* __restrict__ flags) {
+// The level being split is [lev[0], lev[1]) in device memory: the host launches a level's kernels
+// with grids sized by a bound on its node count and reads the counts back only every few levels.
+
+// 1 for every node of the level that splits, 0 from nl up to bound (the scan's total at nl)
+__global__ void k_split_flags(const DNode* __restrict__ nodes, const int32_t* __restrict__ lev, int ppn, int max_level,
+		int32_t* __restrict__ flags, int bound) {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j > nl) return;
-	if (j == nl) {
-		flags[j] = 0;
-		return;
+	if (j > bound) return;
+	const int lb = lev[0], nl = lev[1] - lev[0];
+	int32_t f = 0;
+	if (j < nl) {
+		const DNode nd = nodes[lb + j];
+		f = (nd.pend - nd.pbegin > ppn && nd.level < max_level) ? 1 : 0;
 	}
-	const DNode nd = nodes[lb + j];
-	flags[j] = (nd.pend - nd.pbegin > ppn && nd.level < max_level) ? 1 : 0;
+	flags[j] = f;
+}
+
+// the next level: [le, le + 2 nsplit); log = (lb, le, nsplit) of the level just split
+__global__ void k_next_level(int32_t* __restrict__ lev, const int32_t* __restrict__ rank, int32_t* __restrict__ log) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	const int32_t lb = lev[0], le = lev[1];
+	const int32_t ns = rank[le - lb];
+	log[0] = lb;
+	log[1] = le;
+	log[2] = ns;
+	lev[0] = le;
+	lev[1] = le + 2 * ns;
 }
 
 // the split dimension and midpoint of every splitting node (src/tree.cpp:136-157), its children's
 // index
-__global__ void k_split_setup(const DNode* __restrict__ nodes, int lb, int nl, int le, const int32_t* __restrict__ flags,
+__global__ void k_split_setup(const DNode* __restrict__ nodes, const int32_t* __restrict__ lev, const int32_t* __restrict__ flags,
 		const int32_t* __restrict__ rank, SplitInfo* __restrict__ info) {
 #pragma clang fp contract(off)
+	const int lb = lev[0], le = lev[1], nl = le - lb;
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= nl) return;
 	SplitInfo si{};
@@ -132,10 +150,11 @@ __global__ void k_split_setup(const DNode* __restrict__ nodes, int lb, int nl, i
 }
 
 // per particle: 1 if it lies in a splitting node of the level and below that node's midpoint
-__global__ void k_low_flags(const int32_t* __restrict__ pos, const int32_t* __restrict__ pnode, int64_t n, int lb, int nl,
+__global__ void k_low_flags(const int32_t* __restrict__ pos, const int32_t* __restrict__ pnode, int64_t n, const int32_t* __restrict__ lev,
 		const SplitInfo* __restrict__ info, int32_t* __restrict__ low) {
 	const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (p > n) return;
+	const int lb = lev[0], nl = lev[1] - lev[0];
 	int32_t f = 0;
 	if (p < n) {
 		const int j = pnode[p] - lb;
@@ -149,7 +168,9 @@ __global__ void k_low_flags(const int32_t* __restrict__ pos, const int32_t* __re
 
 // the children of every splitting node: [pbegin, split) and [split, pend), the split point being
 // where the node's low particles end after the partition
-__global__ void k_split_nodes(DNode* __restrict__ nodes, int lb, int nl, SplitInfo* __restrict__ info, const int32_t* __restrict__ S) {
+__global__ void k_split_nodes(DNode* __restrict__ nodes, const int32_t* __restrict__ lev, SplitInfo* __restrict__ info,
+		const int32_t* __restrict__ S) {
+	const int lb = lev[0], nl = lev[1] - lev[0];
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= nl) return;
 	SplitInfo si = info[j];
@@ -168,10 +189,12 @@ __global__ void k_split_nodes(DNode* __restrict__ nodes, int lb, int nl, SplitIn
 }
 
 // the low particles right of the split point, by rank counted from the node's high end
-__global__ void k_high_side_lows(const int32_t* __restrict__ pnode, int64_t n, int lb, int nl, const SplitInfo* __restrict__ info,
-		const DNode* __restrict__ nodes, const int32_t* __restrict__ low, const int32_t* __restrict__ S, int32_t* __restrict__ mr) {
+__global__ void k_high_side_lows(const int32_t* __restrict__ pnode, int64_t n, const int32_t* __restrict__ lev,
+		const SplitInfo* __restrict__ info, const DNode* __restrict__ nodes, const int32_t* __restrict__ low, const int32_t* __restrict__ S,
+		int32_t* __restrict__ mr) {
 	const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (p >= n) return;
+	const int lb = lev[0], nl = lev[1] - lev[0];
 	const int j = pnode[p] - lb;
 	if (j < 0 || j >= nl) return;
 	const SplitInfo si = info[j];
@@ -184,10 +207,11 @@ __global__ void k_high_side_lows(const int32_t* __restrict__ pnode, int64_t n, i
 // every high particle left of the split point exchanges its record with its partner; every
 // particle of a splitting node moves to its child
 __global__ void k_exchange(int32_t* __restrict__ pos, float* __restrict__ mass, int64_t* __restrict__ order, int32_t* __restrict__ pnode,
-		int64_t n, int lb, int nl, const SplitInfo* __restrict__ info, const DNode* __restrict__ nodes, const int32_t* __restrict__ low,
-		const int32_t* __restrict__ S, const int32_t* __restrict__ mr) {
+		int64_t n, const int32_t* __restrict__ lev, const SplitInfo* __restrict__ info, const DNode* __restrict__ nodes,
+		const int32_t* __restrict__ low, const int32_t* __restrict__ S, const int32_t* __restrict__ mr) {
 	const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (p >= n) return;
+	const int lb = lev[0], nl = lev[1] - lev[0];
 	const int j = pnode[p] - lb;
 	if (j < 0 || j >= nl) return;
 	const SplitInfo si = info[j];
@@ -535,72 +559,107 @@ hipError_t device_build_tree(Tree& T, const int32_t* x, const float* masses, int
 		const DNode root{0, int32_t(n), -1, -1, -1, 0, 1};
 		DTCHK(hipMemcpyAsync(dnodes.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
 	}
-	// the particle scan's temporary storage
-	size_t scan_bytes = 0;
+	// Level by level, a batch of DT_BATCH levels at a time without a host wait: each level's kernels
+	// read its extent from dlev and a last kernel moves dlev on and logs (lb, le, nsplit); grids are
+	// sized by bounds (a level's splitting nodes hold more than ppn particles each, so there are at
+	// most split_cap of them, and a level has at most twice the previous level's nodes).  After a
+	// batch the host reads the log, stops at the first level that split nothing (the levels after
+	// it in the batch found no nodes and did nothing), and grows the node array for the next batch.
+	constexpr int DT_BATCH = 4;
+	const int64_t split_cap = n / (int64_t(std::max(1, p.parts_per_node)) + 1) + 1;
+	const int64_t lvl_max = 2 * split_cap;
+	const int64_t fl_cap = lvl_max + 1;
+	Buf dlev, dlog;
+	DTCHK(dflags.alloc(size_t(fl_cap + 1) * 4));
+	DTCHK(drank.alloc(size_t(fl_cap + 1) * 4));
+	DTCHK(dinfo.alloc(size_t(fl_cap) * sizeof(SplitInfo)));
+	DTCHK(dlev.alloc(2 * sizeof(int32_t)));
+	const int log_cap = MAX_SPLIT_LEVEL + 2 + DT_BATCH;
+	DTCHK(dlog.alloc(size_t(3 * log_cap) * sizeof(int32_t)));
+	{
+		const int32_t lev0[2] = {0, 1};
+		DTCHK(hipMemcpyAsync(dlev.p, lev0, sizeof(lev0), hipMemcpyHostToDevice, s));
+	}
+	// the scans' temporary storage: the particle scan and the largest level scan
+	size_t scan_bytes = 0, level_scan_bytes = 0;
 	DTCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, dlow.as<int32_t>(), dS.as<int32_t>(), int(n + 1), s));
-	DTCHK(scan_tmp.alloc(scan_bytes));
+	DTCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, level_scan_bytes, dflags.as<int32_t>(), drank.as<int32_t>(), int(fl_cap), s));
+	DTCHK(scan_tmp.alloc(std::max(scan_bytes, level_scan_bytes)));
 	std::vector<int> level_begin;
-	int lb = 0, le = 1;
-	while (true) {
-		level_begin.push_back(lb);
-		const int nl = le - lb;
-		DTCHK(dflags.alloc(size_t(nl + 1) * 4));
-		DTCHK(drank.alloc(size_t(nl + 1) * 4));
-		DTCHK(dinfo.alloc(size_t(nl) * sizeof(SplitInfo)));
-		hipLaunchKernelGGL(k_split_flags, dim3(blocks(nl + 1, 256)), dim3(256), 0, s, dnodes.as<const DNode>(), lb, nl, p.parts_per_node,
-				MAX_SPLIT_LEVEL, dflags.as<int32_t>());
-		DTLAUNCH("k_split_flags");
-		size_t need = 0;
-		DTCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, dflags.as<int32_t>(), drank.as<int32_t>(), nl + 1, s));
-		if (need > scan_tmp.bytes) {
-			DTCHK(hipStreamSynchronize(s));
-			Buf bigger;
-			DTCHK(bigger.alloc(need));
-			std::swap(scan_tmp.p, bigger.p);
-			std::swap(scan_tmp.bytes, bigger.bytes);
+	int64_t le_h = 1;      // nodes made, as of the host's last look
+	int64_t nl_bound = 1;  // the current level's nodes, at most
+	int level = 0;
+	int le = 1;
+	bool finished = false;
+	while (!finished) {
+		if (level + DT_BATCH > log_cap) {
+			err = "device tree build: more levels than the split limit allows";
+			return hipErrorInvalidValue;
 		}
-		size_t have = scan_tmp.bytes;
-		DTCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, have, dflags.as<int32_t>(), drank.as<int32_t>(), nl + 1, s));
-		int32_t nsplit = 0;
-		DTCHK(hipMemcpyAsync(&nsplit, drank.as<int32_t>() + nl, 4, hipMemcpyDeviceToHost, s));
-		DTCHK(hipStreamSynchronize(s));
-		if (nsplit == 0) break;
-		if (int64_t(le) + 2 * int64_t(nsplit) > INT32_MAX) {
+		// node capacity for this batch
+		int64_t add = 0;
+		for (int64_t i = 0, b = nl_bound; i < DT_BATCH; i++, b = std::min(2 * b, lvl_max)) add += 2 * std::min(b, split_cap);
+		if (le_h + add > INT32_MAX) {
 			err = "device tree build: node count overflow";
 			return hipErrorInvalidValue;
 		}
-		if (int64_t(le) + 2 * nsplit > node_cap) {
+		if (le_h + add > node_cap) {
 			// grow the node array, keeping the nodes made so far
 			int64_t cap = node_cap;
-			while (cap < int64_t(le) + 2 * nsplit) cap *= 2;
+			while (cap < le_h + add) cap *= 2;
 			Buf bigger;
 			DTCHK(bigger.alloc(size_t(cap) * sizeof(DNode)));
-			DTCHK(hipMemcpyAsync(bigger.p, dnodes.p, size_t(le) * sizeof(DNode), hipMemcpyDeviceToDevice, s));
+			DTCHK(hipMemcpyAsync(bigger.p, dnodes.p, size_t(le_h) * sizeof(DNode), hipMemcpyDeviceToDevice, s));
 			DTCHK(hipStreamSynchronize(s));
 			std::swap(dnodes.p, bigger.p);
 			std::swap(dnodes.bytes, bigger.bytes);
 			node_cap = cap;
 		}
-		hipLaunchKernelGGL(k_split_setup, dim3(blocks(nl, 256)), dim3(256), 0, s, dnodes.as<const DNode>(), lb, nl, le,
-				dflags.as<const int32_t>(), drank.as<const int32_t>(), dinfo.as<SplitInfo>());
-		hipLaunchKernelGGL(k_low_flags, dim3(blocks(n + 1, 256)), dim3(256), 0, s, dpos.as<const int32_t>(), dpnode.as<const int32_t>(), n,
-				lb, nl, dinfo.as<const SplitInfo>(), dlow.as<int32_t>());
-		DTLAUNCH("k_split_setup/k_low_flags");
-		have = scan_tmp.bytes;
-		DTCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, have, dlow.as<int32_t>(), dS.as<int32_t>(), int(n + 1), s));
-		hipLaunchKernelGGL(k_split_nodes, dim3(blocks(nl, 256)), dim3(256), 0, s, dnodes.as<DNode>(), lb, nl, dinfo.as<SplitInfo>(),
-				dS.as<const int32_t>());
-		if (n > 0) {
-			hipLaunchKernelGGL(k_high_side_lows, dim3(blocks(n, 256)), dim3(256), 0, s, dpnode.as<const int32_t>(), n, lb, nl,
-					dinfo.as<const SplitInfo>(), dnodes.as<const DNode>(), dlow.as<const int32_t>(), dS.as<const int32_t>(),
-					dmr.as<int32_t>());
-			hipLaunchKernelGGL(k_exchange, dim3(blocks(n, 256)), dim3(256), 0, s, dpos.as<int32_t>(), dmass.as<float>(), dorder.as<int64_t>(),
-					dpnode.as<int32_t>(), n, lb, nl, dinfo.as<const SplitInfo>(), dnodes.as<const DNode>(), dlow.as<const int32_t>(),
-					dS.as<const int32_t>(), dmr.as<const int32_t>());
+		const int first = level;
+		for (int i = 0; i < DT_BATCH; i++, level++) {
+			const int bound = int(nl_bound);
+			const int32_t* lev = dlev.as<const int32_t>();
+			hipLaunchKernelGGL(k_split_flags, dim3(blocks(bound + 1, 256)), dim3(256), 0, s, dnodes.as<const DNode>(), lev, p.parts_per_node,
+					MAX_SPLIT_LEVEL, dflags.as<int32_t>(), bound);
+			DTLAUNCH("k_split_flags");
+			size_t have = scan_tmp.bytes;
+			DTCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, have, dflags.as<int32_t>(), drank.as<int32_t>(), bound + 1, s));
+			hipLaunchKernelGGL(k_split_setup, dim3(blocks(bound, 256)), dim3(256), 0, s, dnodes.as<const DNode>(), lev,
+					dflags.as<const int32_t>(), drank.as<const int32_t>(), dinfo.as<SplitInfo>());
+			hipLaunchKernelGGL(k_low_flags, dim3(blocks(n + 1, 256)), dim3(256), 0, s, dpos.as<const int32_t>(), dpnode.as<const int32_t>(), n,
+					lev, dinfo.as<const SplitInfo>(), dlow.as<int32_t>());
+			DTLAUNCH("k_split_setup/k_low_flags");
+			have = scan_tmp.bytes;
+			DTCHK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, have, dlow.as<int32_t>(), dS.as<int32_t>(), int(n + 1), s));
+			hipLaunchKernelGGL(k_split_nodes, dim3(blocks(bound, 256)), dim3(256), 0, s, dnodes.as<DNode>(), lev, dinfo.as<SplitInfo>(),
+					dS.as<const int32_t>());
+			if (n > 0) {
+				hipLaunchKernelGGL(k_high_side_lows, dim3(blocks(n, 256)), dim3(256), 0, s, dpnode.as<const int32_t>(), n, lev,
+						dinfo.as<const SplitInfo>(), dnodes.as<const DNode>(), dlow.as<const int32_t>(), dS.as<const int32_t>(),
+						dmr.as<int32_t>());
+				hipLaunchKernelGGL(k_exchange, dim3(blocks(n, 256)), dim3(256), 0, s, dpos.as<int32_t>(), dmass.as<float>(), dorder.as<int64_t>(),
+						dpnode.as<int32_t>(), n, lev, dinfo.as<const SplitInfo>(), dnodes.as<const DNode>(), dlow.as<const int32_t>(),
+						dS.as<const int32_t>(), dmr.as<const int32_t>());
+			}
+			hipLaunchKernelGGL(k_next_level, dim3(1), dim3(64), 0, s, dlev.as<int32_t>(), drank.as<const int32_t>(),
+					dlog.as<int32_t>() + 3 * level);
+			DTLAUNCH("k_split_nodes/k_high_side_lows/k_exchange/k_next_level");
+			nl_bound = std::min(2 * nl_bound, lvl_max);
 		}
-		DTLAUNCH("k_split_nodes/k_high_side_lows/k_exchange");
-		lb = le;
-		le = le + 2 * nsplit;
+		int32_t lg[3 * DT_BATCH];
+		DTCHK(hipMemcpyAsync(lg, dlog.as<int32_t>() + 3 * first, sizeof(lg), hipMemcpyDeviceToHost, s));
+		DTCHK(hipStreamSynchronize(s));
+		for (int i = 0; i < DT_BATCH; i++) {
+			const int32_t lb_i = lg[3 * i], le_i = lg[3 * i + 1], ns = lg[3 * i + 2];
+			level_begin.push_back(lb_i);
+			le = le_i;
+			if (ns == 0) {
+				finished = true;
+				break;
+			}
+		}
+		le_h = int64_t(lg[3 * DT_BATCH - 2]) + 2 * int64_t(lg[3 * DT_BATCH - 1]);
+		nl_bound = std::max<int64_t>(2 * int64_t(lg[3 * DT_BATCH - 1]), 1);  // the next level's nodes, exactly
 	}
 	level_begin.push_back(le);
 	const int nn = le;
