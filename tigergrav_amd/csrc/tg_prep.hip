@@ -191,7 +191,7 @@ __device__ int64_t expand_ranges(const int32_t* __restrict__ v, int64_t kb, int6
 			b = v[j];
 			const int e = v[j + 1];
 			len = e > b ? e - b : 0;
-			if (len > 0 && !keep(b, e)) len = 0;
+			if (len > 0 && !keep(b, e, j)) len = 0;
 		}
 		int s = len;
 		for (int o = 1; o < 64; o <<= 1) {
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(64) void k_node_expand(NodeIn A, const int64_t* __r
 		const int64_t kb = A.ptr[q][nd], ke = A.ptr[q][nd + 1];
 		if (ke == kb) continue;
 		int32_t* out = (q == 1 ? cp_d : cp_e) + oent[q * np + nd];
-		expand_ranges(A.val[q], kb, ke, [](int, int) { return true; }, out);
+		expand_ranges(A.val[q], kb, ke, [](int, int, int64_t) { return true; }, out);
 	}
 }
 
@@ -242,6 +242,7 @@ struct LeafIn {
 	const int* pleaf;        // particle -> leaf
 	const WalkNode* nodes;
 	const int4* box;         // per node, 2 records: lowest and highest particle coordinates (leaves)
+	uint8_t* near;           // PP direct: per range (entry pair j / 2), the near test's outcome
 	int64_t nl;
 	int ewald;
 	double hnear;
@@ -337,7 +338,11 @@ __global__ __launch_bounds__(64) void k_leaf_count(LeafIn A, int64_t* __restrict
 				const int b = A.val[q][j], e = A.val[q][j + 1];
 				if (e <= b) continue;
 				ent += e - b;
-				if (q == 1 && range_near(A, slo, shi, b, e)) nearl += e - b;
+				if (q == 1) {
+					const bool nr = range_near(A, slo, shi, b, e);
+					A.near[j >> 1] = nr ? 1 : 0;  // for the write pass
+					if (nr) nearl += e - b;
+				}
 			}
 			ent = wave_sum(ent);
 			nearl = wave_sum(nearl);
@@ -382,11 +387,11 @@ __global__ __launch_bounds__(64) void k_leaf_write(LeafIn A, const int64_t* __re
 			int32_t* out = O.list[q] + lb;
 			if (q == 1) {
 				nearl = lnear[i];
-				const LeafIn& a = A;
-				const int64_t w = expand_ranges(A.val[q], kb, ke, [&](int b, int e) { return range_near(a, slo, shi, b, e); }, out);
-				expand_ranges(A.val[q], kb, ke, [&](int b, int e) { return !range_near(a, slo, shi, b, e); }, out + w);
+				const uint8_t* nr = A.near;
+				const int64_t w = expand_ranges(A.val[q], kb, ke, [&](int, int, int64_t j) { return nr[j >> 1] != 0; }, out);
+				expand_ranges(A.val[q], kb, ke, [&](int, int, int64_t j) { return nr[j >> 1] == 0; }, out + w);
 			} else {
-				expand_ranges(A.val[q], kb, ke, [](int, int) { return true; }, out);
+				expand_ranges(A.val[q], kb, ke, [](int, int, int64_t) { return true; }, out);
 			}
 		} else {
 			lb = kb;
@@ -854,6 +859,8 @@ int Solver::upload_lists_device() {
 	int64_t* sum_tot = static_cast<int64_t*>(scratch(8 * 8));     // their totals
 	if (!box || !lent || !olent || !litem || !olitem || !lnear || !sums || !sum_tot) PCHK(hipErrorOutOfMemory);
 	li.box = box;
+	li.near = static_cast<uint8_t*>(scratch(size_t(std::max<int64_t>(L.len[5], 2)) / 2 + 1));
+	if (!li.near) PCHK(hipErrorOutOfMemory);
 	if (nl > 0) {
 		hipLaunchKernelGGL(k_leaf_boxes, dim3(unsigned(nl)), dim3(64), 0, s, li.leaf_items, li.parts, box);
 		PCHK(hipGetLastError());
