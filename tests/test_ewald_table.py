@@ -98,8 +98,9 @@ def test_generator_symmetries():
 def interpolate(x, inv=64, npts=33):
     """The kernels' scheme in FP64 from float-rounded table values: |x| sorted into the wedge
     u0 >= u1 >= u2 by the compare-exchanges (0,1), (1,2), (0,1); a third-order Taylor expansion about
-    the nearest grid point there; the exchanges undone on the components in reverse order; signs by
-    octant."""
+    the nearest grid point there for the components that hold the third axis at most once, the
+    others from the trace relation (the Laplacian is -4 pi); the exchanges undone on the components
+    in reverse order; signs by octant."""
     u = np.abs(x).copy()
     sg = np.where(x < 0, -1.0, 1.0)
     swaps = []
@@ -115,6 +116,10 @@ def interpolate(x, inv=64, npts=33):
     mono = [j for j in range(1, 20)]  # slots of orders 1..3
     for k in range(35):
         c = SLOTS[k]
+        if c[2] >= 2:
+            lap = -4.0 * math.pi if c == (0, 0, 2) else 0.0
+            out[:, k] = lap - out[:, SIDX[(c[0] + 2, c[1], c[2] - 2)]] - out[:, SIDX[(c[0], c[1] + 2, c[2] - 2)]]
+            continue
         v = T[:, k].copy()
         for j in mono:
             J = SLOTS[j]
@@ -131,6 +136,25 @@ def interpolate(x, inv=64, npts=33):
     for k in range(35):
         out[:, k] *= np.prod(sg ** np.array(SLOTS[k]), axis=1)
     return out
+
+
+def test_constant_laplacian():
+    """The correction's Laplacian is -4 pi over the cell (the periodic potential's uniform
+    background; the nearest image's own Laplacian is the point mass), so every higher derivative
+    tensor is traceless: D_aa = -4 pi and sum_a D_{k+2a} = 0 for |k| = 1..5 (the relation the
+    kernels use for the components that hold the wedge's third axis twice or more), to 1e-8 of
+    each order's scale."""
+    rng = np.random.default_rng(21)
+    X = rng.uniform(-0.5, 0.5, (300, 3))
+    X = X[np.linalg.norm(X, axis=1) > 0.05]
+    D = g64(X)
+    np.testing.assert_allclose(D[:, SIDX[(2, 0, 0)]] + D[:, SIDX[(0, 2, 0)]] + D[:, SIDX[(0, 0, 2)]], -4.0 * math.pi,
+                               rtol=1e-9)
+    for k, c in enumerate(SLOTS):
+        if 1 <= sum(c) <= ORDER - 2:
+            tr = sum(D[:, SIDX[(c[0] + 2 * (a == 0), c[1] + 2 * (a == 1), c[2] + 2 * (a == 2))]] for a in range(3))
+            scale = np.max(np.abs(D[:, ORD == sum(c) + 2]))
+            assert np.max(np.abs(tr)) < 1e-8 * scale, (c, np.max(np.abs(tr)), scale)
 
 
 @pytest.mark.parametrize("region", ["cell", "near_origin", "near_corner"])

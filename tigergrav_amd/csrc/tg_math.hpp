@@ -355,17 +355,31 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 		for (int e = 0; e < n2; e++) m *= d[2];
 		w[j] = m;
 	});
+	// The correction has a constant Laplacian in the cell (-4 pi: the periodic potential's uniform
+	// background), so every derivative tensor's trace is fixed by lower slots: a component holding
+	// the wedge's third axis twice or more is C - D[k - 2z + 2x] - D[k - 2z + 2y], C = -4 pi for
+	// D_zz and 0 otherwise.  Those components (10 of the 35 to order 4) come from that relation,
+	// in slot order (their two terms come earlier), the others from the Taylor step.
+	const float lap = z ? 0.0f : -12.566370614359172f;  // the coincident row's D_zz is 0 too
 	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
 		constexpr int k = decltype(k_)::value;
-		// monomials of orders 1..3 (slots 1..19), or 1..2 (slots 1..9) where order 8 would be needed
-		constexpr int jn = ewt::ST.c[k].n0 + ewt::ST.c[k].n1 + ewt::ST.c[k].n2 + 3 <= TG_EWT_ORDER ? 19 : 9;
-		float v = t[k];
-		ewt_for_seq(std::make_integer_sequence<int, jn>{}, [&](auto j_) {
-			constexpr int j = decltype(j_)::value + 1;
-			static_assert(ewt::merge(k, j) >= 0 && ewt::merge(k, j) < NLOAD, "slot read");
-			v = fmaf(t[ewt_cx<ewt::merge(k, j)>], w[j], v);
-		});
-		D[k] = v;
+		constexpr int n0 = ewt::ST.c[k].n0, n1 = ewt::ST.c[k].n1, n2 = ewt::ST.c[k].n2;
+		if constexpr (n2 >= 2) {
+			constexpr int ka = ewt::slot_of(n0 + 2, n1, n2 - 2), kb = ewt::slot_of(n0, n1 + 2, n2 - 2);
+			static_assert(ka >= 0 && ka < k && kb >= 0 && kb < k, "trace terms come first");
+			const float c = (n0 == 0 && n1 == 0 && n2 == 2) ? lap : 0.0f;
+			D[k] = c - D[ka] - D[kb];
+		} else {
+			// monomials of orders 1..3 (slots 1..19), or 1..2 (slots 1..9) where order 8 would be needed
+			constexpr int jn = n0 + n1 + n2 + 3 <= TG_EWT_ORDER ? 19 : 9;
+			float v = t[k];
+			ewt_for_seq(std::make_integer_sequence<int, jn>{}, [&](auto j_) {
+				constexpr int j = decltype(j_)::value + 1;
+				static_assert(ewt::merge(k, j) >= 0 && ewt::merge(k, j) < NLOAD, "slot read");
+				v = fmaf(t[ewt_cx<ewt::merge(k, j)>], w[j], v);
+			});
+			D[k] = v;
+		}
 	});
 	// back from the wedge to the separation's own axes
 	ewt_transpose_if<NOUT, 0, 1>(D, s3);
