@@ -97,10 +97,11 @@ def test_generator_symmetries():
 
 def interpolate(x, inv=64, npts=33):
     """The kernels' scheme in FP64 from float-rounded table values: |x| sorted into the wedge
-    u0 >= u1 >= u2 by the compare-exchanges (0,1), (1,2), (0,1); a third-order Taylor expansion about
-    the nearest grid point there for the components that hold the third axis at most once, the
-    others from the trace relation (the Laplacian is -4 pi); the exchanges undone on the components
-    in reverse order; signs by octant."""
+    u0 >= u1 >= u2 by the compare-exchanges (0,1), (1,2), (0,1); the grid point's slots that hold
+    the third axis at most once from the table, the others from the trace relation (the Laplacian
+    is -4 pi); a third-order Taylor expansion about the point for the components that hold the
+    third axis at most once, the others again from the trace relation; the exchanges undone on the
+    components in reverse order; signs by octant."""
     u = np.abs(x).copy()
     sg = np.where(x < 0, -1.0, 1.0)
     swaps = []
@@ -112,6 +113,12 @@ def interpolate(x, inv=64, npts=33):
     assert np.all(gi[:, 0] >= gi[:, 1]) and np.all(gi[:, 1] >= gi[:, 2])
     d = u - gi / inv
     T = g64(gi / inv).astype(np.float32).astype(np.float64)
+    # the table stores the slots holding the third axis at most once; the others come from the
+    # trace relation, in slot order
+    for k, c in enumerate(SLOTS):
+        if c[2] >= 2:
+            lap = -4.0 * math.pi if c == (0, 0, 2) else 0.0
+            T[:, k] = lap - T[:, SIDX[(c[0] + 2, c[1], c[2] - 2)]] - T[:, SIDX[(c[0], c[1] + 2, c[2] - 2)]]
     out = np.zeros((len(x), 35))
     mono = [j for j in range(1, 20)]  # slots of orders 1..3
     for k in range(35):

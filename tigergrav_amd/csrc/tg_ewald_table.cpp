@@ -167,7 +167,8 @@ void ewald_table_build(std::vector<float>& T) {
 				double D[TG_EWT_S];
 				ewald_green64(x, D);
 				float* row = T.data() + size_t(ewt::wedge_index(i, j, l)) * TG_EWT_STRIDE;
-				for (int k = 0; k < TG_EWT_S; k++) row[k] = float(D[k]);
+				for (int k = 0; k < TG_EWT_S; k++)
+					if (ewt::stored_index(k) >= 0) row[ewt::stored_index(k)] = float(D[k]);
 			}
 }
 

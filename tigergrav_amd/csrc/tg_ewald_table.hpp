@@ -24,7 +24,8 @@
 #define TG_EWT_S 120    // slots of orders 0..7
 #define TG_EWT_P 6545   // grid points with i >= j >= l: TG_EWT_N (TG_EWT_N + 1) (TG_EWT_N + 2) / 6
 #define TG_EWT_ROWS (TG_EWT_P + 1)  // and one row for coincident points: the reference's constants
-#define TG_EWT_STRIDE 124  // floats per row: 496 bytes (measured: rows of 480 or 512 bytes gather slower)
+#define TG_EWT_C 64      // slots stored per row: those holding the third axis at most once (2 n + 1 of order n)
+#define TG_EWT_STRIDE 68  // floats per row: 272 bytes
 
 namespace tg {
 namespace ewt {
@@ -65,7 +66,19 @@ constexpr int swap_ab(int k, int a, int b) {
 // table row of the sorted grid point i >= j >= l
 constexpr int wedge_index(int i, int j, int l) { return i * (i + 1) * (i + 2) / 6 + j * (j + 1) / 2 + l; }
 static_assert(wedge_index(TG_EWT_N - 1, TG_EWT_N - 1, TG_EWT_N - 1) == TG_EWT_P - 1, "wedge size");
-static_assert(TG_EWT_STRIDE >= TG_EWT_S && TG_EWT_STRIDE % 4 == 0, "row stride");
+// a row's position of slot k (stored slots in slot order), or -1 for a slot the trace relation
+// gives: the correction's Laplacian is constant (-4 pi), so D_{k} = C - D_{k - 2z + 2x} - D_{k - 2z + 2y}
+// for every slot holding the third axis twice or more (C = -4 pi for D_zz, else 0)
+constexpr int stored_index(int k) {
+	if (ST.c[k].n2 > 1) return -1;
+	int c = 0;
+	for (int i = 0; i < k; i++) c += ST.c[i].n2 <= 1 ? 1 : 0;
+	return c;
+}
+// stored slots of orders 0..m
+constexpr int stored_count(int m) { return (m + 1) * (m + 1); }
+static_assert(stored_count(TG_EWT_ORDER) == TG_EWT_C, "stored slots");
+static_assert(TG_EWT_STRIDE >= TG_EWT_C && TG_EWT_STRIDE % 4 == 0, "row stride");
 // slot k with one more index a, with indices a and b
 constexpr int up1(int k, int a) { return slot_of(ST.c[k].n0 + (a == 0), ST.c[k].n1 + (a == 1), ST.c[k].n2 + (a == 2)); }
 constexpr int up2(int k, int a, int b) { return up1(up1(k, a), b); }
@@ -90,7 +103,8 @@ static_assert(slot_of(2, 0, 0) == 4 && slot_of(0, 1, 1) == 8 && slot_of(1, 1, 1)
 // FP64 value of the correction and its TG_EWT_S derivatives (orders 0..TG_EWT_ORDER) at x (host);
 // at x = 0 the analytic limit (the kernels apply the reference's r -> 0 convention themselves)
 void ewald_green64(const double* x, double* D);
-// the table: TG_EWT_ROWS rows of TG_EWT_STRIDE floats (TG_EWT_S slots, then zeros): the sorted grid points (row
+// the table: TG_EWT_ROWS rows of TG_EWT_STRIDE floats (the TG_EWT_C stored slots in slot order, then
+// zeros): the sorted grid points (row
 // wedge_index(i, j, l)), then row TG_EWT_P = (2.8372975, 0, ..., 0), the reference's values at
 // r -> 0, which a kernel reads with a zero Taylor offset for a coincident pair
 void ewald_table_build(std::vector<float>& T);

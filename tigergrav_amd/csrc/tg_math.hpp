@@ -331,15 +331,34 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 	}
 	const int row = z ? TG_EWT_P : ewt::wedge_index(gi[0], gi[1], gi[2]);
 	const float4* tp = reinterpret_cast<const float4*>(table + size_t(row) * TG_EWT_STRIDE);
-	float t[NLOAD];
+	// the row holds the slots with the third axis at most once; the others follow from the trace
+	// relation (a constant Laplacian, -4 pi; 0 for the coincident row), each from two slots before it
+	constexpr int MAXO = ewt::ST.c[NLOAD - 1].n0 + ewt::ST.c[NLOAD - 1].n1 + ewt::ST.c[NLOAD - 1].n2;  // highest order read
+	constexpr int NSTORED = ewt::stored_count(MAXO);
+	float c[(NSTORED + 3) / 4 * 4];
 #pragma unroll
-	for (int q = 0; q < NLOAD / 4; q++) {
+	for (int q = 0; q < (NSTORED + 3) / 4; q++) {
 		const float4 v = tp[q];
-		t[4 * q] = v.x;
-		t[4 * q + 1] = v.y;
-		t[4 * q + 2] = v.z;
-		t[4 * q + 3] = v.w;
+		c[4 * q] = v.x;
+		c[4 * q + 1] = v.y;
+		c[4 * q + 2] = v.z;
+		c[4 * q + 3] = v.w;
 	}
+	const float lap = z ? 0.0f : -12.566370614359172f;
+	float t[NLOAD];
+	ewt_for_seq(std::make_integer_sequence<int, NLOAD>{}, [&](auto k_) {
+		constexpr int k = decltype(k_)::value;
+		constexpr int n0 = ewt::ST.c[k].n0, n1 = ewt::ST.c[k].n1, n2 = ewt::ST.c[k].n2;
+		if constexpr (n2 <= 1) {
+			constexpr int si = ewt::stored_index(k);
+			static_assert(si >= 0 && si < (NSTORED + 3) / 4 * 4, "stored slot read");
+			t[k] = c[si];
+		} else {
+			constexpr int ka = ewt::slot_of(n0 + 2, n1, n2 - 2), kb = ewt::slot_of(n0, n1 + 2, n2 - 2);
+			static_assert(ka >= 0 && ka < k && kb >= 0 && kb < k, "trace terms come first");
+			t[k] = ((n0 == 0 && n1 == 0 && n2 == 2) ? lap : 0.0f) - t[ka] - t[kb];
+		}
+	});
 	// the Taylor monomials d^j / j! of orders 1..3 (slots 1..19)
 	float w[20];
 	w[0] = 1.0f;
@@ -360,7 +379,6 @@ __device__ __forceinline__ void green_ewald_tab(float* D, const float* x, const 
 	// the wedge's third axis twice or more is C - D[k - 2z + 2x] - D[k - 2z + 2y], C = -4 pi for
 	// D_zz and 0 otherwise.  Those components (10 of the 35 to order 4) come from that relation,
 	// in slot order (their two terms come earlier), the others from the Taylor step.
-	const float lap = z ? 0.0f : -12.566370614359172f;  // the coincident row's D_zz is 0 too
 	ewt_for_seq(std::make_integer_sequence<int, NOUT>{}, [&](auto k_) {
 		constexpr int k = decltype(k_)::value;
 		constexpr int n0 = ewt::ST.c[k].n0, n1 = ewt::ST.c[k].n1, n2 = ewt::ST.c[k].n2;
