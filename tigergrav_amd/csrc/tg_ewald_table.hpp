@@ -6,10 +6,12 @@
 // tigergrav/green.hpp:3-59 truncates them) is smooth over the wrapped cell |x_i| <= 1/2, even in
 // each coordinate and symmetric under permutations of the coordinates.  Its derivatives to
 // seventh order are tabulated in FP64, rounded to float, on the points of the grid
-// {0, 1/64, ..., 1/2}^3 with i >= j >= l (6545 points, 3.2 MB: it stays in one XCD's L2).  A
-// kernel sorts |x| into that wedge, takes the derivatives it needs from a third-order Taylor
-// expansion about the nearest point, permutes the components back and flips the sign of every
-// component with an odd count of a negative coordinate's index.
+// {0, 1/64, ..., 1/2}^3 with i >= j >= l (6545 points).  A row stores the 64 slots that hold the
+// third axis at most once (1.8 MB in all: it stays in one XCD's L2); the correction's Laplacian
+// is constant (-4 pi), so the other slots follow from the trace relation.  A kernel sorts |x| into
+// that wedge, takes the derivatives it needs from a third-order Taylor expansion about the
+// nearest point, permutes the components back and flips the sign of every component with an odd
+// count of a negative coordinate's index.
 //
 // Symmetric tensors are stored by slot: orders 0..7 in turn, each order's sorted index tuples in
 // lexicographic order (orders 0..4 are the expansion layout of tigergrav/expansion.hpp:77-137).
