@@ -560,7 +560,7 @@ __global__ __launch_bounds__(64) void k_cc(NodeArgs A) {
 				green_ewald(D, x);
 			else
 				green_direct(D, x);
-			m2l(L, v, D);
+			m2l<!EWALD>(L, v, D);
 		};
 		float4 ma[5];
 		if (EWALD) {

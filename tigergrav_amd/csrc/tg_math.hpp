@@ -190,29 +190,58 @@ __device__ __forceinline__ void green_direct(float* D, const float* x) {
 // ---- contractions (src/gravity.cpp:26-169) ----------------------------------------------------
 // M2L: L += contraction of the multipole M with D (reference gravity.cpp:26-90)
 // Every term is added straight into L, in the reference's order of terms (as its loop adds each
-// term into its L).
+// term into its L).  HARM: D is traceless in any two of its indices (derivatives of 1/r away from
+// the origin), so D_{k,zz} = -D_{k,xx} - D_{k,yy} and a multipole's zz-holding components fold
+// into the others: the quadrupole contracts in 5 terms and the octupole in 7 (each L component
+// whose terms reach D through them: L0, L_a, L_ab), with the 8 folded weights formed once per
+// entry.
+template <bool HARM>
 __device__ __forceinline__ void m2l(float* L, const float* M, const float* D) {
+	float q[6], o[10];
+#pragma unroll
+	for (int s = 0; s < 6; s++) q[s] = M[1 + s] * ef2(s);
+#pragma unroll
+	for (int s = 0; s < 10; s++) o[s] = -M[7 + s] * ef3(s);
+	if (HARM) {
+		// quadrupole slots xx 0, xy 1, xz 2, yy 3, yz 4, zz 5; octupole xxx 0, xxy 1, xxz 2, xyy 3,
+		// xyz 4, xzz 5, yyy 6, yyz 7, yzz 8, zzz 9
+		q[0] -= q[5];
+		q[3] -= q[5];
+		o[0] -= o[5];
+		o[3] -= o[5];
+		o[1] -= o[8];
+		o[6] -= o[8];
+		o[2] -= o[9];
+		o[7] -= o[9];
+	}
+	auto quad = [&](auto slot_of, float acc) {
+#pragma unroll
+		for (int s = 0; s < 6; s++)
+			if (!HARM || s != 5) acc = fmaf(q[s], D[slot_of(s)], acc);
+		return acc;
+	};
+	auto oct = [&](auto slot_of, float acc) {
+#pragma unroll
+		for (int s = 0; s < 10; s++)
+			if (!HARM || (s != 5 && s != 8 && s != 9)) acc = fmaf(o[s], D[slot_of(s)], acc);
+		return acc;
+	};
 	float l0 = fmaf(M[0], D[0], L[0]);
-#pragma unroll
-	for (int s = 0; s < 6; s++) l0 = fmaf(M[1 + s] * ef2(s), D[4 + s], l0);
-#pragma unroll
-	for (int s = 0; s < 10; s++) l0 = fmaf(-M[7 + s] * ef3(s), D[10 + s], l0);
+	l0 = quad([](int s) { return 4 + s; }, l0);
+	l0 = oct([](int s) { return 10 + s; }, l0);
 	L[0] = l0;
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
 		float la = fmaf(M[0], D[1 + a], L[1 + a]);
-#pragma unroll
-		for (int s = 0; s < 6; s++) la = fmaf(M[1 + s] * ef2(s), D[E3(a, T2[s].i[0], T2[s].i[1])], la);
-#pragma unroll
-		for (int s = 0; s < 10; s++) la = fmaf(-M[7 + s] * ef3(s), D[E4(a, T3[s].i[0], T3[s].i[1], T3[s].i[2])], la);
+		la = quad([a](int s) { return E3(a, T2[s].i[0], T2[s].i[1]); }, la);
+		la = oct([a](int s) { return E4(a, T3[s].i[0], T3[s].i[1], T3[s].i[2]); }, la);
 		L[1 + a] = la;
 	}
 #pragma unroll
 	for (int t = 0; t < 6; t++) {
 		const int a = T2[t].i[0], b = T2[t].i[1];
 		float lab = fmaf(M[0], D[4 + t], L[4 + t]);
-#pragma unroll
-		for (int s = 0; s < 6; s++) lab = fmaf(M[1 + s] * ef2(s), D[E4(a, b, T2[s].i[0], T2[s].i[1])], lab);
+		lab = quad([a, b](int s) { return E4(a, b, T2[s].i[0], T2[s].i[1]); }, lab);
 		L[4 + t] = lab;
 	}
 #pragma unroll
